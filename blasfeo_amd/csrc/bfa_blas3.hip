@@ -1,0 +1,304 @@
+// bfa_blas3.hip -- batched level-3 kernels for gfx950: gemm_nt, gemm_nn,
+// syrk_ln(_mn), trmm_rlnn and trsm_rltn on panel-major matrices.
+//
+// Launch model: the grid's x dimension walks the batch (one record per
+// workgroup), y walks output tiles of an instance (tile t, t+gridDim.y, ...),
+// so a batch of small instances uses one workgroup each and a batch of large
+// ones spreads its 64x64 tiles over several (tile engine: bfa_gemm_core.hpp).
+#include "bfa_gemm_core.hpp"
+#include "bfa_internal.hpp"
+
+namespace bfa {
+
+template <typename T, int PS, int OP>
+__global__ __launch_bounds__(NT3) void k_gemm(const typename GemmArg<T>::type *__restrict__ args)
+{
+	__shared__ T As[KC * LDT];
+	__shared__ T Bs[KC * LDT];
+	const typename GemmArg<T>::type &a = args[blockIdx.x];
+	const int m = a.m;
+	const int n = (OP == OP_SYRK) ? a.m : a.n;
+	if (m <= 0 || n <= 0)
+		return;
+	if (threadIdx.x == 0 && blockIdx.y == 0)
+		a.sD->use_dA = 0;
+	gemm_core<T, PS, OP>(As, Bs, m, n, a.k, a.alpha, a.beta,
+		view<T, PS>(a.sA), a.ai, a.aj, view<T, PS>(a.sB), a.bi, a.bj,
+		view<T, PS>(a.sC), a.ci, a.cj, view<T, PS>(a.sD), a.di, a.dj);
+}
+
+// D = alpha * B * A, A lower n x n (blasfeo_ref/x_blas3_ref.c:3218-3330)
+template <typename T, int PS>
+__global__ __launch_bounds__(NT3) void k_trmm_rlnn(const typename TrsmArg<T>::type *__restrict__ args)
+{
+	__shared__ T As[KC * LDT];
+	__shared__ T Bs[KC * LDT];
+	const typename TrsmArg<T>::type &a = args[blockIdx.x];
+	if (a.m <= 0 || a.n <= 0)
+		return;
+	if (threadIdx.x == 0 && blockIdx.y == 0)
+		a.sD->use_dA = 0;
+	const PM<T, PS> gA = view<T, PS>(a.sA);
+	gemm_core<T, PS, OP_TRMM_RLNN>(As, Bs, a.m, a.n, a.n, a.alpha, T(0),
+		gA, a.ai, a.aj, view<T, PS>(a.sB), a.bi, a.bj,
+		gA, 0, 0, view<T, PS>(a.sD), a.di, a.dj);
+}
+
+// ---------------------------------------------------------------------------
+// trsm_rltn: D = alpha * B * A^{-T}, A lower n x n (blasfeo_ref/x_blas3_ref.c:1642-1752)
+// A workgroup solves a 64-row block of B: D's row block lives in LDS while
+// the columns are produced 16 at a time: the already-solved columns are
+// applied with matrix-core steps (X -= D(:,0:j) A(j,0:j)^T), then one wave
+// solves the 16x16 diagonal triangle row by row with the reciprocal diagonal.
+// ---------------------------------------------------------------------------
+constexpr int TRSM_NMAX = 256;
+constexpr int LDX = TM + 1;
+
+template <typename T, int PS>
+__global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::type *__restrict__ args, int nmax)
+{
+	typedef typename Vec4<T>::type V4;
+	extern __shared__ unsigned char smem_raw[];
+	T *Xs = reinterpret_cast<T *>(smem_raw);      // [npad][LDX] column-major row block
+	__shared__ T inv_s[TRSM_NMAX];
+	__shared__ T Ad[16 * 17];                       // diagonal 16x16 block of A
+
+	const typename TrsmArg<T>::type &a = args[blockIdx.x];
+	const int m = a.m, n = a.n;
+	if (m <= 0 || n <= 0 || n > nmax)
+		return;
+	const PM<T, PS> gA = view<T, PS>(a.sA);
+	const PM<T, PS> gB = view<T, PS>(a.sB);
+	PM<T, PS> gD = view<T, PS>(a.sD);
+	const T alpha = a.alpha;
+	const int lane = lane_id();
+	const int w = wave_id();
+
+	// reciprocal diagonal: cached copy when valid at launch, else 1/A_jj
+	// (an output aliasing A invalidates the cache first, as in the reference)
+	const bool cached = (a.ai == 0 && a.aj == 0 && a.sD != a.sA && a.sA->use_dA >= n);
+	for (int j = threadIdx.x; j < n; j += NT3)
+		inv_s[j] = cached ? a.sA->dA[j] : T(1) / gA.at(a.ai + j, a.aj + j);
+
+	const int mt = (m + TM - 1) / TM;
+	const int nb = (n + 15) / 16;
+	for (int t = blockIdx.y; t < mt; t += gridDim.y)
+	{
+		const int i0 = t * TM;
+		const int mv = min(TM, m - i0);
+		__syncthreads();
+		// X <- alpha * B(row block, :)
+		g2s<T, PS, NT3, false>(gB, a.bi + i0, a.bj, TM, nb * 16, mv, n, Xs, LDX);
+		__syncthreads();
+		for (int e = threadIdx.x; e < nb * 16 * LDX; e += NT3)
+			Xs[e] *= alpha;
+		for (int cb = 0; cb < nb; cb++)
+		{
+			const int c0 = cb * 16;
+			const int cv = min(16, n - c0);
+			// diagonal block of A into LDS (zero outside the valid triangle)
+			__syncthreads();
+			for (int e = threadIdx.x; e < 256; e += NT3)
+			{
+				const int r = e & 15, c = e >> 4;
+				T v = T(0);
+				if (r < cv && c < cv && r > c)
+					v = gA.at(a.ai + c0 + r, a.aj + c0 + c);
+				Ad[r * 17 + c] = v;
+			}
+			// X(:, c0:c0+16) -= X(:, 0:c0) * A(c0:c0+16, 0:c0)^T ; wave w owns rows 16w..16w+15
+			if (c0 > 0)
+			{
+				V4 acc = V4{0, 0, 0, 0};
+				const int jrow = a.ai + c0 + (lane & 15);
+				const bool jok = (lane & 15) < cv;
+				for (int kk = 0; kk < c0; kk += 4)
+				{
+					const int kq = kk + (lane >> 4);
+					T av = Xs[kq * LDX + 16 * w + (lane & 15)];
+					T bv = jok ? gA.at(jrow, a.aj + kq) : T(0);
+					acc = mfma16(av, bv, acc);
+				}
+#pragma unroll
+				for (int r = 0; r < 4; r++)
+				{
+					const int il = 16 * w + acc_row<T>(lane, r);
+					const int jl = c0 + (lane & 15);
+					Xs[jl * LDX + il] -= acc[r];
+				}
+			}
+			__syncthreads();
+			// triangular solve of the 16 columns, one row per thread of wave 0
+			if (w == 0)
+			{
+				const int il = lane;
+				T x[16];
+#pragma unroll
+				for (int j = 0; j < 16; j++)
+					x[j] = Xs[(c0 + j) * LDX + il];
+#pragma unroll
+				for (int j = 0; j < 16; j++)
+				{
+					if (j < cv)
+					{
+						T c = x[j];
+#pragma unroll
+						for (int l = 0; l < j; l++)
+							c -= x[l] * Ad[j * 17 + l];
+						x[j] = c * inv_s[c0 + j];
+					}
+				}
+#pragma unroll
+				for (int j = 0; j < 16; j++)
+					Xs[(c0 + j) * LDX + il] = x[j];
+			}
+		}
+		__syncthreads();
+		s2g<T, PS, NT3, false>(gD, a.di + i0, a.dj, mv, n, Xs, LDX, 0);
+	}
+}
+
+// Cache update after trsm_rltn, in the reference's order
+// (blasfeo_hp_pm/d_blas3_lib4.c:4157-4178): invalidate D's cache, then
+// refresh A's reciprocal diagonal unless it was valid.  Run as its own launch
+// so that no workgroup of the solve reads a dA another one is writing.
+template <typename T, int PS>
+__global__ void k_trsm_rltn_dA(const typename TrsmArg<T>::type *__restrict__ args)
+{
+	const typename TrsmArg<T>::type &a = args[blockIdx.x];
+	const int n = a.n;
+	if (a.m <= 0 || n <= 0)
+		return;
+	const bool zero_off = (a.ai == 0 && a.aj == 0);
+	const int use0 = (a.sD == a.sA) ? 0 : a.sA->use_dA;
+	__syncthreads();
+	if (threadIdx.x == 0)
+		a.sD->use_dA = 0;
+	if (zero_off && use0 >= n)
+		return;
+	const PM<T, PS> gA = view<T, PS>(a.sA);
+	for (int j = threadIdx.x; j < n; j += blockDim.x)
+		a.sA->dA[j] = T(1) / gA.at(a.ai + j, a.aj + j);
+	__syncthreads();
+	if (threadIdx.x == 0)
+		a.sA->use_dA = zero_off ? n : 0;
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static int tiles_y(const blasfeo_amd_hint *hint, int tm, int tn)
+{
+	if (!hint || hint->max_m <= 0)
+		return 1;
+	int mt = (hint->max_m + tm - 1) / tm;
+	int nt = hint->max_n > 0 ? (hint->max_n + tn - 1) / tn : 1;
+	int y = mt * nt;
+	return y < 1 ? 1 : (y > 64 ? 64 : y);
+}
+
+static int check_common(int ps, int batch, const void *args, bool is_float, const char *name)
+{
+	if (batch < 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: batch=%d < 0", name, batch);
+	if (batch > 0 && !args)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: args is NULL", name);
+	const bool ok = is_float ? (ps == 4 || ps == 8 || ps == 16) : (ps == 4 || ps == 8);
+	if (!ok)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: unsupported panel size %d", name, ps);
+	return BLASFEO_AMD_OK;
+}
+
+template <typename T, int OP>
+static int launch_gemm(int ps, int batch, const typename GemmArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
+{
+	int rc = check_common(ps, batch, args, sizeof(T) == 4, name);
+	if (rc || batch == 0)
+		return rc;
+	dim3 grid(batch, tiles_y(hint, TM, TN));
+	if (ps == 4)
+		hipLaunchKernelGGL((k_gemm<T, 4, OP>), grid, dim3(NT3), 0, st, args);
+	else if (ps == 8)
+		hipLaunchKernelGGL((k_gemm<T, 8, OP>), grid, dim3(NT3), 0, st, args);
+	else
+		hipLaunchKernelGGL((k_gemm<T, 16, OP>), grid, dim3(NT3), 0, st, args);
+	return check_launch(name);
+}
+
+template <typename T>
+static int launch_trmm(int ps, int batch, const typename TrsmArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
+{
+	int rc = check_common(ps, batch, args, sizeof(T) == 4, name);
+	if (rc || batch == 0)
+		return rc;
+	dim3 grid(batch, tiles_y(hint, TM, TN));
+	if (ps == 4)
+		hipLaunchKernelGGL((k_trmm_rlnn<T, 4>), grid, dim3(NT3), 0, st, args);
+	else if (ps == 8)
+		hipLaunchKernelGGL((k_trmm_rlnn<T, 8>), grid, dim3(NT3), 0, st, args);
+	else
+		hipLaunchKernelGGL((k_trmm_rlnn<T, 16>), grid, dim3(NT3), 0, st, args);
+	return check_launch(name);
+}
+
+template <typename T, int PS>
+static int launch_trsm_ps(int batch, const typename TrsmArg<T>::type *args, dim3 grid, size_t shm, int nmax, hipStream_t st, const char *name)
+{
+	hipLaunchKernelGGL((k_trsm_rltn<T, PS>), grid, dim3(NT3), shm, st, args, nmax);
+	int rc = check_launch(name);
+	if (rc)
+		return rc;
+	hipLaunchKernelGGL((k_trsm_rltn_dA<T, PS>), dim3(batch), dim3(64), 0, st, args);
+	return check_launch(name);
+}
+
+template <typename T>
+static int launch_trsm(int ps, int batch, const typename TrsmArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
+{
+	int rc = check_common(ps, batch, args, sizeof(T) == 4, name);
+	if (rc || batch == 0)
+		return rc;
+	if (!hint || hint->max_n <= 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: hint->max_n (upper bound of n) is required", name);
+	if (hint->max_n > TRSM_NMAX)
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: n up to %d supported, got %d", name, TRSM_NMAX, hint->max_n);
+	const int npad = (hint->max_n + 15) / 16 * 16;
+	const size_t shm = (size_t)npad * LDX * sizeof(T);
+	const int gy = hint->max_m > 0 ? min(64, (hint->max_m + TM - 1) / TM) : 1;
+	dim3 grid(batch, gy);
+	if (ps == 4)
+		return launch_trsm_ps<T, 4>(batch, args, grid, shm, npad, st, name);
+	if (ps == 8)
+		return launch_trsm_ps<T, 8>(batch, args, grid, shm, npad, st, name);
+	return launch_trsm_ps<T, 16>(batch, args, grid, shm, npad, st, name);
+}
+
+} // namespace bfa
+
+using namespace bfa;
+
+extern "C" {
+
+int blasfeo_amd_dgemm_nt_batched(int ps, int batch, const blasfeo_amd_dgemm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_gemm<double, OP_NT>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dgemm_nt_batched"); }
+int blasfeo_amd_dgemm_nn_batched(int ps, int batch, const blasfeo_amd_dgemm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_gemm<double, OP_NN>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dgemm_nn_batched"); }
+int blasfeo_amd_dsyrk_ln_batched(int ps, int batch, const blasfeo_amd_dgemm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_gemm<double, OP_SYRK>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dsyrk_ln_batched"); }
+int blasfeo_amd_dsyrk_ln_mn_batched(int ps, int batch, const blasfeo_amd_dgemm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_gemm<double, OP_SYRK_MN>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dsyrk_ln_mn_batched"); }
+int blasfeo_amd_dtrmm_rlnn_batched(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_trmm<double>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrmm_rlnn_batched"); }
+int blasfeo_amd_dtrsm_rltn_batched(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_trsm<double>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_rltn_batched"); }
+
+int blasfeo_amd_sgemm_nt_batched(int ps, int batch, const blasfeo_amd_sgemm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_gemm<float, OP_NT>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_sgemm_nt_batched"); }
+int blasfeo_amd_sgemm_nn_batched(int ps, int batch, const blasfeo_amd_sgemm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_gemm<float, OP_NN>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_sgemm_nn_batched"); }
+int blasfeo_amd_ssyrk_ln_batched(int ps, int batch, const blasfeo_amd_sgemm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_gemm<float, OP_SYRK>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_ssyrk_ln_batched"); }
+int blasfeo_amd_strsm_rltn_batched(int ps, int batch, const blasfeo_amd_strsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_trsm<float>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_strsm_rltn_batched"); }
+
+} // extern "C"

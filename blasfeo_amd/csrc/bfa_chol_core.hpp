@@ -1,0 +1,155 @@
+// bfa_chol_core.hpp -- left-looking blocked Cholesky of one instance by one
+// workgroup, shared by the potrf kernels and the fused Riccati sweep.
+#pragma once
+#include "bfa_common.hpp"
+
+namespace bfa {
+
+constexpr int NTL = 256;
+constexpr int PW = 16;  // panel width
+
+__device__ __forceinline__ double readlane(double v, int l)
+{
+	int2 x = __builtin_bit_cast(int2, v);
+	x.x = __builtin_amdgcn_readlane(x.x, l);
+	x.y = __builtin_amdgcn_readlane(x.y, l);
+	return __builtin_bit_cast(double, x);
+}
+__device__ __forceinline__ float readlane(float v, int l)
+{
+	return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+// Largest m a launch supports (LDS panel of (m+1) x 16 elements).
+template <typename T> constexpr int fact_mmax() { return sizeof(T) == 8 ? 1024 : 2048; }
+
+// ---------------------------------------------------------------------------
+// Cholesky family: D(m x n lower trapezoid) = chol(C + A*B^T), n <= m.
+// k == 0 gives potrf_l / potrf_l_mn.  Zero-pivot rule of
+// blasfeo_ref/x_lapack_ref.c:84-92: pivot <= 0 (or NaN) stores L_jj = c*0 and dA = 0.
+// ---------------------------------------------------------------------------
+template <typename T, int PS>
+__device__ void chol_instance(T *P, int LDP, T *L11, T *invs,
+	int m, int n, int k,
+	const PM<T, PS> gA, int ai, int aj, const PM<T, PS> gB, int bi, int bj,
+	const PM<T, PS> gC, int ci, int cj, PM<T, PS> gD, int di, int dj, T *dD)
+{
+	typedef typename Vec4<T>::type V4;
+	const int lane = lane_id();
+	const int w = wave_id();
+	for (int c0 = 0; c0 < n; c0 += PW)
+	{
+		const int cw = min(PW, n - c0);
+		const int nr = m - c0;            // panel rows c0..m-1
+		const int nrt = (nr + 15) / 16;   // 16-row tiles
+		__syncthreads();
+		g2s<T, PS, NTL, false>(gC, ci + c0, cj + c0, nrt * 16, PW, nr, cw, P, LDP);
+		__syncthreads();
+		// matrix-core updates of the panel, 16-row tile per wave
+		if (k > 0 || c0 > 0)
+		{
+			for (int rt = w; rt < nrt; rt += NTL / WAVE)
+			{
+				V4 acc = V4{0, 0, 0, 0};
+				const int ir = rt * 16 + (lane & 15);     // panel-local row of the A operand
+				const bool iok = ir < nr;
+				const bool jok = (lane & 15) < cw;
+				// + A(c0+i, :) * B(c0+j, :)^T
+				for (int kk = 0; kk < k; kk += 4)
+				{
+					const int kq = kk + (lane >> 4);
+					const bool kok = kq < k;
+					T av = (iok && kok) ? gA.at(ai + c0 + ir, aj + kq) : T(0);
+					T bv = (jok && kok) ? gB.at(bi + c0 + (lane & 15), bj + kq) : T(0);
+					acc = mfma16(av, bv, acc);
+				}
+				// - L(c0+i, 0:c0) * L(c0+j, 0:c0)^T  (c0 is a multiple of 16)
+				for (int kk = 0; kk < c0; kk += 4)
+				{
+					const int kq = kk + (lane >> 4);
+					T av = iok ? gD.at(di + c0 + ir, dj + kq) : T(0);
+					T bv = jok ? gD.at(di + c0 + (lane & 15), dj + kq) : T(0);
+					acc = mfma16(-av, bv, acc);
+				}
+#pragma unroll
+				for (int r = 0; r < 4; r++)
+				{
+					const int il = rt * 16 + acc_row<T>(lane, r);
+					P[(lane & 15) * LDP + il] += acc[r];
+				}
+			}
+			__syncthreads();
+		}
+		// wave 0: rows 0..63 of the panel in registers (lane = row)
+		if (w == 0)
+		{
+			T x[PW];
+#pragma unroll
+			for (int j = 0; j < PW; j++)
+				x[j] = (lane < nr) ? P[j * LDP + lane] : T(0);
+#pragma unroll
+			for (int j = 0; j < PW; j++)
+			{
+				if (j < cw)
+				{
+					const T c = readlane(x[j], j);
+					const T inv = (c > T(0)) ? rsqrt_ref<T>(c) : T(0);
+					if (lane > j)
+						x[j] *= inv;
+					else if (lane == j)
+						x[j] = c * inv;
+					if (lane == 0)
+						invs[j] = inv;
+#pragma unroll
+					for (int jj = j + 1; jj < PW; jj++)
+					{
+						if (jj < cw)
+						{
+							const T l = readlane(x[j], jj);
+							if (lane >= jj)
+								x[jj] -= x[j] * l;
+						}
+					}
+				}
+			}
+#pragma unroll
+			for (int j = 0; j < PW; j++)
+			{
+				if (lane < nr)
+					P[j * LDP + lane] = x[j];
+				if (lane < PW)
+					L11[lane * (PW + 1) + j] = x[j];
+			}
+		}
+		__syncthreads();
+		// rows 64.. of the panel: solve against L11 (x_j = (x_j - sum_l<j x_l L_jl) * inv_j)
+		for (int il = 64 + threadIdx.x; il < nr; il += NTL)
+		{
+			T x[PW];
+#pragma unroll
+			for (int j = 0; j < PW; j++)
+				x[j] = P[j * LDP + il];
+#pragma unroll
+			for (int j = 0; j < PW; j++)
+			{
+				if (j < cw)
+				{
+					T c = x[j];
+#pragma unroll
+					for (int l = 0; l < j; l++)
+						c -= x[l] * L11[j * (PW + 1) + l];
+					x[j] = c * invs[j];
+				}
+			}
+#pragma unroll
+			for (int j = 0; j < PW; j++)
+				P[j * LDP + il] = x[j];
+		}
+		__syncthreads();
+		s2g<T, PS, NTL, true>(gD, di + c0, dj + c0, nr, cw, P, LDP, 0);
+		for (int j = threadIdx.x; j < cw; j += NTL)
+			dD[c0 + j] = invs[j];
+	}
+}
+
+} // namespace bfa

@@ -1,0 +1,174 @@
+// bfa_common.hpp -- device building blocks shared by the gfx950 kernels.
+//
+// Panel-major addressing follows include/blasfeo_common.h:112: element (i,j)
+// of a matrix with panel size PS and panel stride cn is at
+// pA[(i - i%PS)*cn + j*PS + i%PS].  All indices below are absolute (the
+// caller's row/column offsets already added).
+//
+// Matrix-core (MFMA) operand layout for the 16x16x4 shapes used here
+// (v_mfma_f64_16x16x4f64, v_mfma_f32_16x16x4f32), wave of 64 lanes:
+//   A (16x4):  lane l holds A[l%16][l/16]
+//   B (4x16):  lane l holds B[l/16][l%16]
+//   D (16x16): lane l, register r holds D[acc_row(l, r)][l%16] with
+//              acc_row = l/16 + 4*r for f64 and 4*(l/16) + r for f32
+//              (measured on gfx950 with a known-answer product).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/blasfeo_amd.h"
+
+namespace bfa {
+
+constexpr int WAVE = 64;
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Vec4;
+template <> struct Vec4<double> { typedef d4_t type; };
+template <> struct Vec4<float> { typedef f4_t type; };
+
+template <typename T> struct MatOf;
+template <> struct MatOf<double> { typedef blasfeo_dmat type; };
+template <> struct MatOf<float> { typedef blasfeo_smat type; };
+
+// one 16x16x4 matrix-core step: acc += a * b (layout above)
+__device__ __forceinline__ d4_t mfma16(double a, double b, d4_t acc)
+{
+	return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma16(float a, float b, f4_t acc)
+{
+	return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+}
+
+// Row of the 16x16 accumulator tile that register r of lane l holds.
+template <typename T> __device__ __forceinline__ int acc_row(int l, int r);
+template <> __device__ __forceinline__ int acc_row<double>(int l, int r) { return (l >> 4) + 4 * r; }
+template <> __device__ __forceinline__ int acc_row<float>(int l, int r) { return 4 * (l >> 4) + r; }
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+__device__ __forceinline__ int wave_id() { return threadIdx.x / WAVE; }
+
+// Keeps the compiler from moving LDS/global accesses across this point; waves
+// execute their own memory operations in order, so this is the only ordering
+// needed when a single wave hands data between its lanes through LDS.
+__device__ __forceinline__ void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Panel-major matrix view.
+template <typename T, int PS>
+struct PM
+{
+	T *p;
+	int cn;
+	__device__ __forceinline__ size_t off(int i, int j) const
+	{
+		return (size_t)(i & ~(PS - 1)) * (size_t)cn + (size_t)j * PS + (i & (PS - 1));
+	}
+	__device__ __forceinline__ T &at(int i, int j) const { return p[off(i, j)]; }
+};
+
+template <typename T, int PS>
+__device__ __forceinline__ PM<T, PS> view(const typename MatOf<T>::type *s)
+{
+	PM<T, PS> v;
+	v.p = s->pA;
+	v.cn = s->cn;
+	return v;
+}
+
+// Cooperative copy of the R x C block at absolute (r0, c0) of a panel-major
+// matrix into LDS, element (i,c) stored at lds[c*ld + i] (TRANS=false) or
+// lds[i*ld + c] (TRANS=true).  Elements with i >= Rv or c >= Cv are written as
+// zero and never read from memory.  Threads walk the covering panels in
+// memory order, so a wave reads contiguous runs of C*PS elements.
+template <typename T, int PS, int NT, bool TRANS>
+__device__ __forceinline__ void g2s(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, T *lds, int ld)
+{
+	const int rr = r0 & (PS - 1);
+	const int np = (rr + R + PS - 1) / PS;
+	const int run = PS * C;
+	const int total = np * run;
+	const T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
+	for (int e = threadIdx.x; e < total; e += NT)
+	{
+		const int p = e / run;
+		const int rem = e - p * run;
+		const int c = rem / PS;
+		const int q = rem - c * PS;
+		const int i = p * PS + q - rr;
+		if (i < 0 || i >= R)
+			continue;
+		T v = T(0);
+		if (i < Rv && c < Cv)
+			v = base[(size_t)p * PS * g.cn + (size_t)c * PS + q];
+		if (TRANS)
+			lds[i * ld + c] = v;
+		else
+			lds[c * ld + i] = v;
+	}
+}
+
+// Cooperative copy of LDS (lds[c*ld + i]) back to the Rv x Cv block at
+// absolute (r0, c0); LOWER keeps only elements with i - c >= diag.
+template <typename T, int PS, int NT, bool LOWER>
+__device__ __forceinline__ void s2g(PM<T, PS> g, int r0, int c0, int Rv, int Cv, const T *lds, int ld, int diag)
+{
+	const int rr = r0 & (PS - 1);
+	const int np = (rr + Rv + PS - 1) / PS;
+	const int run = PS * Cv;
+	const int total = np * run;
+	T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
+	for (int e = threadIdx.x; e < total; e += NT)
+	{
+		const int p = e / run;
+		const int rem = e - p * run;
+		const int c = rem / PS;
+		const int q = rem - c * PS;
+		const int i = p * PS + q - rr;
+		if (i < 0 || i >= Rv)
+			continue;
+		if (LOWER && i - c < diag)
+			continue;
+		base[(size_t)p * PS * g.cn + (size_t)c * PS + q] = lds[c * ld + i];
+	}
+}
+
+// wave-wide reductions via cross-lane moves
+__device__ __forceinline__ double shfl_xor_d(double v, int mask)
+{
+	return __shfl_xor(v, mask, WAVE);
+}
+
+// Wave-wide argmax of |v| with the reference's pivot rule: a candidate wins
+// only if strictly larger (so NaN never wins), ties go to the smaller row
+// index.  Returns the winning row index (uniform across the wave).
+template <typename T>
+__device__ __forceinline__ int wave_argmax_abs(T best, int idx)
+{
+	for (int m = 1; m < WAVE; m <<= 1)
+	{
+		T ob = __shfl_xor(best, m, WAVE);
+		int oi = __shfl_xor(idx, m, WAVE);
+		if (ob > best || (ob == best && oi < idx))
+		{
+			best = ob;
+			idx = oi;
+		}
+	}
+	return __builtin_amdgcn_readfirstlane(idx);
+}
+
+template <typename T> __device__ __forceinline__ T rsqrt_ref(T c);
+// 1/sqrt(c) as the reference forms it (1.0/sqrt(c), correctly rounded sqrt
+// followed by a correctly rounded division).
+template <> __device__ __forceinline__ double rsqrt_ref<double>(double c) { return 1.0 / __builtin_sqrt(c); }
+template <> __device__ __forceinline__ float rsqrt_ref<float>(float c) { return 1.0f / __builtin_sqrtf(c); }
+
+} // namespace bfa

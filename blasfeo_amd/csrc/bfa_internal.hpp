@@ -1,0 +1,18 @@
+// bfa_internal.hpp -- host-side helpers shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "../../include/blasfeo_amd.h"
+
+namespace bfa {
+
+// Records a message for blasfeo_amd_last_error() (thread-local) and returns code.
+int set_error(int code, const char *fmt, ...);
+
+// Checks the launch that was just enqueued.
+int check_launch(const char *what);
+
+// Number of compute units of the current device (cached per device).
+int num_cus();
+
+} // namespace bfa

@@ -1,0 +1,364 @@
+// bfa_lapack.hip -- batched factorisations for gfx950: potrf_l, potrf_l_mn,
+// syrk_dpotrf_ln(_mn) and getrf_rp on panel-major matrices.
+//
+// One workgroup (4 waves) per instance.  Both factorisations are left-looking
+// over 16-column panels held in LDS at full height (the role of the
+// reference's 12x4 / 8x8 register blocks, blasfeo_hp_pm/d_lapack_lib4.c:954-985
+// and :2443-2634):
+//   potrf: panel <- C(+A*B^T) - L(:,0:c0) L(c0:c0+16,0:c0)^T on the matrix
+//          cores, then one wave factors the top 64 rows in registers (the
+//          diagonal 16x16 Cholesky, broadcasting L(jj,j) with v_readlane) and
+//          all waves solve the remaining rows against it;
+//   getrf: panel <- rows gathered through the current row permutation, the
+//          previous unit-lower panels applied block by block (unit solve of
+//          the 16-row block, matrix-core update of the rows below), then one
+//          wave factors the panel column by column with a wave-wide argmax
+//          for the pivot (first index of the largest |x|, as idamax and
+//          blasfeo_ref/x_lapack_ref.c:1275-1302), and the panel's swaps are
+//          applied to the finished columns on the left.
+#include "bfa_chol_core.hpp"
+#include "bfa_internal.hpp"
+
+namespace bfa {
+
+template <typename T> struct FactArg;
+template <> struct FactArg<double> { typedef blasfeo_amd_dfact_arg type; };
+template <> struct FactArg<float> { typedef blasfeo_amd_sfact_arg type; };
+
+// mode 0: potrf_l (use_dA = m), 1: potrf_l_mn (use_dA = 1)
+template <typename T, int PS, int MODE>
+__global__ __launch_bounds__(NTL) void k_potrf(const typename FactArg<T>::type *__restrict__ args, int mmax)
+{
+	extern __shared__ unsigned char smem_raw[];
+	__shared__ T L11[PW * (PW + 1)];
+	__shared__ T invs[PW];
+	T *P = reinterpret_cast<T *>(smem_raw);
+	const typename FactArg<T>::type &a = args[blockIdx.x];
+	const int m = a.m;
+	int n = MODE == 0 ? a.m : a.n;
+	if (m <= 0 || n <= 0 || m > mmax)
+		return;
+	if (n > m)
+		n = m;
+	const int LDP = (mmax + 15) / 16 * 16 + 1;
+	PM<T, PS> gD = view<T, PS>(a.sD);
+	chol_instance<T, PS>(P, LDP, L11, invs, m, n, 0, gD, 0, 0, gD, 0, 0,
+		view<T, PS>(a.sC), a.ci, a.cj, gD, a.di, a.dj, a.sD->dA);
+	if (threadIdx.x == 0)
+		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? (MODE == 0 ? m : 1) : 0;
+}
+
+template <typename T> struct SPArg;
+template <> struct SPArg<double> { typedef blasfeo_amd_dsyrk_dpotrf_arg type; };
+
+template <typename T, int PS>
+__global__ __launch_bounds__(NTL) void k_syrk_potrf(const typename SPArg<T>::type *__restrict__ args, int mmax)
+{
+	extern __shared__ unsigned char smem_raw[];
+	__shared__ T L11[PW * (PW + 1)];
+	__shared__ T invs[PW];
+	T *P = reinterpret_cast<T *>(smem_raw);
+	const typename SPArg<T>::type &a = args[blockIdx.x];
+	const int m = a.m;
+	int n = a.n;
+	if (m <= 0 || n <= 0 || m > mmax)
+		return;
+	if (n > m)
+		n = m;
+	const int LDP = (mmax + 15) / 16 * 16 + 1;
+	chol_instance<T, PS>(P, LDP, L11, invs, m, n, a.k, view<T, PS>(a.sA), a.ai, a.aj,
+		view<T, PS>(a.sB), a.bi, a.bj, view<T, PS>(a.sC), a.ci, a.cj,
+		view<T, PS>(a.sD), a.di, a.dj, a.sD->dA);
+	if (threadIdx.x == 0)
+		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// LU with partial row pivoting (blasfeo_ref/x_lapack_ref.c:1156-1536 semantics)
+// ---------------------------------------------------------------------------
+template <typename T, int PS>
+__global__ __launch_bounds__(NTL) void k_getrf(const typename FactArg<T>::type *__restrict__ args, int mmax)
+{
+	typedef typename Vec4<T>::type V4;
+	extern __shared__ unsigned char smem_raw[];
+	__shared__ T invs[PW];
+	__shared__ int pivs[PW];
+	const int LDP = (mmax + 15) / 16 * 16 + 1;
+	T *P = reinterpret_cast<T *>(smem_raw);                              // [PW][LDP]
+	int *perm = reinterpret_cast<int *>(smem_raw + sizeof(T) * PW * LDP);  // [mmax]
+
+	const typename FactArg<T>::type &a = args[blockIdx.x];
+	const int m = a.m, n = a.n;
+	if (threadIdx.x == 0)
+		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? 1 : 0;
+	if (m <= 0 || n <= 0 || m > mmax)
+		return;
+	const PM<T, PS> gC = view<T, PS>(a.sC);
+	PM<T, PS> gD = view<T, PS>(a.sD);
+	const int ci = a.ci, cj = a.cj, di = a.di, dj = a.dj;
+	T *dD = a.sD->dA;
+	int *ipiv = a.ipiv;
+	const int p = min(m, n);
+	const int lane = lane_id();
+	const int w = wave_id();
+	const int mt = (m + 15) / 16;
+
+	for (int i = threadIdx.x; i < m; i += NTL)
+		perm[i] = i;
+
+	for (int c0 = 0; c0 < n; c0 += PW)
+	{
+		const int cw = min(PW, n - c0);
+		__syncthreads();
+		// gather the panel through the current row permutation
+		for (int e = threadIdx.x; e < PW * mt * 16; e += NTL)
+		{
+			const int c = e / (mt * 16);
+			const int i = e - c * (mt * 16);
+			T v = T(0);
+			if (i < m && c < cw)
+				v = gC.at(ci + perm[i], cj + c0 + c);
+			P[c * LDP + i] = v;
+		}
+		// apply the finished unit-lower panels, 16 rows at a time
+		const int rbmax = min(c0, p);
+		for (int r0 = 0; r0 < rbmax; r0 += 16)
+		{
+			const int rh = min(16, rbmax - r0);
+			__syncthreads();
+			// unit-lower solve of rows r0..r0+rh: one lane per panel column
+			if (w == 0 && lane < cw)
+			{
+				T x[16];
+#pragma unroll
+				for (int r = 0; r < 16; r++)
+					x[r] = P[lane * LDP + r0 + r];
+#pragma unroll
+				for (int r = 1; r < 16; r++)
+				{
+					if (r < rh)
+					{
+						T c = x[r];
+#pragma unroll
+						for (int l = 0; l < r; l++)
+							c -= gD.at(di + r0 + r, dj + r0 + l) * x[l];
+						x[r] = c;
+					}
+				}
+#pragma unroll
+				for (int r = 0; r < 16; r++)
+					P[lane * LDP + r0 + r] = x[r];
+			}
+			__syncthreads();
+			// rows below: P(r0+16.., :) -= L(r0+16.., r0:r0+16) * P(r0:r0+16, :)
+			const int rs = r0 + 16;
+			const int ntile = (m - rs + 15) / 16;
+			for (int rt = w; rt < ntile; rt += NTL / WAVE)
+			{
+				V4 acc = V4{0, 0, 0, 0};
+				const int ir = rs + rt * 16 + (lane & 15);
+				const bool iok = ir < m;
+#pragma unroll
+				for (int kk = 0; kk < 16; kk += 4)
+				{
+					const int kq = kk + (lane >> 4);
+					const bool kok = kq < rh;
+					T av = (iok && kok) ? gD.at(di + ir, dj + r0 + kq) : T(0);
+					T bv = kok ? P[(lane & 15) * LDP + r0 + kq] : T(0);
+					acc = mfma16(av, bv, acc);
+				}
+#pragma unroll
+				for (int r = 0; r < 4; r++)
+				{
+					const int il = rs + rt * 16 + acc_row<T>(lane, r);
+					P[(lane & 15) * LDP + il] -= acc[r];
+				}
+			}
+		}
+		__syncthreads();
+		// factor the panel rows c0..m-1 (only columns that have a pivot row)
+		const int cf = min(cw, p - c0);
+		if (w == 0 && cf > 0)
+		{
+			for (int j = 0; j < cf; j++)
+			{
+				const int jr = c0 + j;
+				// pivot: first row of largest |x| among jr..m-1, from (0, jr)
+				T best = T(0);
+				int bi = jr;
+				for (int i = jr + lane; i < m; i += WAVE)
+				{
+					const T v = P[j * LDP + i];
+					const T av = v > T(0) ? v : -v;
+					if (av > best)
+					{
+						best = av;
+						bi = i;
+					}
+				}
+				const int ip = wave_argmax_abs<T>(best, bi);
+				if (lane == 0)
+					pivs[j] = ip;
+				if (ip != jr)
+				{
+					if (lane < PW)
+					{
+						const T t0 = P[lane * LDP + jr];
+						P[lane * LDP + jr] = P[lane * LDP + ip];
+						P[lane * LDP + ip] = t0;
+					}
+					if (lane == 0)
+					{
+						const int t1 = perm[jr];
+						perm[jr] = perm[ip];
+						perm[ip] = t1;
+					}
+				}
+				wave_sync();
+				const T piv = P[j * LDP + jr];
+				const T inv = piv != T(0) ? T(1) / piv : T(0);
+				if (lane == 0)
+					invs[j] = inv;
+				for (int i = jr + 1 + lane; i < m; i += WAVE)
+				{
+					const T l = P[j * LDP + i] * inv;
+					P[j * LDP + i] = l;
+					for (int jj = j + 1; jj < cw; jj++)
+						P[jj * LDP + i] -= l * P[jj * LDP + jr];
+				}
+				wave_sync();
+			}
+		}
+		__syncthreads();
+		// write the panel (all rows, current order) and its pivot data
+		s2g<T, PS, NTL, false>(gD, di, dj + c0, m, cw, P, LDP, 0);
+		for (int j = threadIdx.x; j < cf; j += NTL)
+		{
+			ipiv[c0 + j] = pivs[j];
+			dD[c0 + j] = invs[j];
+		}
+		__syncthreads();
+		// this panel's swaps on the finished columns 0..c0-1 (one thread per column)
+		for (int c = threadIdx.x; c < c0; c += NTL)
+		{
+			for (int j = 0; j < cf; j++)
+			{
+				const int r1 = c0 + j, r2 = pivs[j];
+				if (r1 != r2)
+				{
+					T &u = gD.at(di + r1, dj + c);
+					T &v = gD.at(di + r2, dj + c);
+					const T t = u;
+					u = v;
+					v = t;
+				}
+			}
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+static int fact_mmax_arg(const blasfeo_amd_hint *hint, const char *name, int *mmax)
+{
+	int mm = (hint && hint->max_m > 0) ? hint->max_m : 256;
+	if (mm > fact_mmax<T>())
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: m up to %d supported, got %d", name, fact_mmax<T>(), mm);
+	*mmax = mm;
+	return BLASFEO_AMD_OK;
+}
+
+static int fact_check(int ps, int batch, const void *args, bool is_float, const char *name)
+{
+	if (batch < 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: batch=%d < 0", name, batch);
+	if (batch > 0 && !args)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: args is NULL", name);
+	const bool ok = is_float ? (ps == 4 || ps == 8 || ps == 16) : (ps == 4 || ps == 8);
+	if (!ok)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: unsupported panel size %d", name, ps);
+	return BLASFEO_AMD_OK;
+}
+
+template <typename T, int MODE>
+static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
+{
+	int rc = fact_check(ps, batch, args, sizeof(T) == 4, name);
+	if (rc || batch == 0)
+		return rc;
+	int mmax;
+	if ((rc = fact_mmax_arg<T>(hint, name, &mmax)))
+		return rc;
+	const size_t shm = sizeof(T) * PW * ((mmax + 15) / 16 * 16 + 1);
+	if (ps == 4)
+		hipLaunchKernelGGL((k_potrf<T, 4, MODE>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+	else if (ps == 8)
+		hipLaunchKernelGGL((k_potrf<T, 8, MODE>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+	else
+		hipLaunchKernelGGL((k_potrf<T, 16, MODE>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+	return check_launch(name);
+}
+
+template <typename T>
+static int launch_getrf(int ps, int batch, const typename FactArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
+{
+	int rc = fact_check(ps, batch, args, sizeof(T) == 4, name);
+	if (rc || batch == 0)
+		return rc;
+	int mmax;
+	if ((rc = fact_mmax_arg<T>(hint, name, &mmax)))
+		return rc;
+	const size_t shm = sizeof(T) * PW * ((mmax + 15) / 16 * 16 + 1) + sizeof(int) * mmax;
+	if (ps == 4)
+		hipLaunchKernelGGL((k_getrf<T, 4>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+	else if (ps == 8)
+		hipLaunchKernelGGL((k_getrf<T, 8>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+	else
+		hipLaunchKernelGGL((k_getrf<T, 16>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+	return check_launch(name);
+}
+
+static int launch_syrk_potrf(int ps, int batch, const blasfeo_amd_dsyrk_dpotrf_arg *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
+{
+	int rc = fact_check(ps, batch, args, false, name);
+	if (rc || batch == 0)
+		return rc;
+	int mmax;
+	if ((rc = fact_mmax_arg<double>(hint, name, &mmax)))
+		return rc;
+	const size_t shm = sizeof(double) * PW * ((mmax + 15) / 16 * 16 + 1);
+	if (ps == 4)
+		hipLaunchKernelGGL((k_syrk_potrf<double, 4>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+	else
+		hipLaunchKernelGGL((k_syrk_potrf<double, 8>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+	return check_launch(name);
+}
+
+} // namespace bfa
+
+using namespace bfa;
+
+extern "C" {
+
+int blasfeo_amd_dpotrf_l_batched(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_potrf<double, 0>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dpotrf_l_batched"); }
+// Switched off: its first GPU run ended in an illegal-address fault whose
+// cause has not been found (every global access of the kernel was checked
+// against the operands' bounds on the host); it stays disabled until one is.
+int blasfeo_amd_dpotrf_l_mn_batched(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{
+	(void)ps; (void)batch; (void)args; (void)hint; (void)s;
+	return set_error(BLASFEO_AMD_EINVAL, "blasfeo_amd_dpotrf_l_mn_batched: disabled in this build");
+}
+int blasfeo_amd_dsyrk_dpotrf_ln_batched(int ps, int batch, const blasfeo_amd_dsyrk_dpotrf_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_syrk_potrf(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dsyrk_dpotrf_ln_batched"); }
+int blasfeo_amd_dgetrf_rp_batched(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_getrf<double>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dgetrf_rp_batched"); }
+int blasfeo_amd_spotrf_l_batched(int ps, int batch, const blasfeo_amd_sfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_potrf<float, 0>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_spotrf_l_batched"); }
+int blasfeo_amd_sgetrf_rp_batched(int ps, int batch, const blasfeo_amd_sfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_getrf<float>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_sgetrf_rp_batched"); }
+
+} // extern "C"

@@ -1,0 +1,144 @@
+"""Batches of backward Riccati factorisations (the chain of
+examples/example_d_riccati_recursion.c:115-140) laid out for the fused GPU
+sweep ``blasfeo_amd_d_back_ric_trf_batched``.
+
+Instance i has horizon N, state size nx_i and input size nu_i on every stage
+(nu = 0 on the last stage, as in the reference example).  Per instance the
+time-invariant data are stored once and every stage's descriptor points at
+them, as the reference example does (:737-747):
+    BAbt = [B^T; A^T]   (nu+nx) x nx
+    RSQ  = [R S; S^T Q] (nu+nx) x (nu+nx)      (lower triangle used)
+    QN   = Q            nx x nx                 (last stage)
+Outputs: L[0..N-1] (nu+nx) x (nu+nx), L[N] nx x nx, plus one work matrix.
+"""
+import numpy as np
+
+from . import DMAT, Layout, RICCATI_ARG
+
+
+class RiccatiBatch:
+    def __init__(self, nxs, nus, N, ps=4):
+        self.nxs = np.asarray(nxs, dtype=np.int64)
+        self.nus = np.asarray(nus, dtype=np.int64)
+        self.N = int(N)
+        self.ps = ps
+        self.batch = len(self.nxs)
+        self.inp = Layout(np.float64, ps)
+        self.out = Layout(np.float64, ps)
+        self.i_babt, self.i_rsq, self.i_qn = [], [], []
+        self.i_L, self.i_work = [], []
+        for nx, nu in zip(self.nxs, self.nus):
+            nx, nu = int(nx), int(nu)
+            self.i_babt.append(self.inp.add(nu + nx, nx))
+            self.i_rsq.append(self.inp.add(nu + nx, nu + nx))
+            self.i_qn.append(self.inp.add(nx, nx))
+            Ls = [self.out.add(nu + nx, nu + nx) for _ in range(self.N)]
+            Ls.append(self.out.add(nx, nx))
+            self.i_L.append(Ls)
+            self.i_work.append(self.out.add(nu + nx, nx))
+        self.max_nk = int((self.nxs + self.nus).max()) if self.batch else 0
+
+    # -- data ---------------------------------------------------------------
+    def fill_inputs(self, rng, A_scale=0.1, R=None, Q=None):
+        """A = I + A_scale*N(0,1), B ~ N(0,1), R = Q = I, S = 0 (host buffer)."""
+        buf = self.inp.host_buffer()
+        for i in range(self.batch):
+            nx, nu = int(self.nxs[i]), int(self.nus[i])
+            A = np.eye(nx) + A_scale * rng.standard_normal((nx, nx))
+            B = rng.standard_normal((nx, nu))
+            self.inp.put(buf, self.i_babt[i], np.vstack([B.T, A.T]))
+            self.inp.put(buf, self.i_rsq[i], np.eye(nu + nx))
+            self.inp.put(buf, self.i_qn[i], np.eye(nx))
+        return buf
+
+    def fill_custom(self, fn):
+        """fn(i, nx, nu) -> (BAbt, RSQ, QN) dense arrays."""
+        buf = self.inp.host_buffer()
+        for i in range(self.batch):
+            nx, nu = int(self.nxs[i]), int(self.nus[i])
+            babt, rsq, qn = fn(i, nx, nu)
+            self.inp.put(buf, self.i_babt[i], babt)
+            self.inp.put(buf, self.i_rsq[i], rsq)
+            self.inp.put(buf, self.i_qn[i], qn)
+        return buf
+
+    # -- descriptor arrays ----------------------------------------------------
+    def tables(self, in_base, out_base):
+        """Per-stage descriptor tables and nx/nu arrays for buffers at the given
+        addresses: returns (babt[batch*N], rsq[batch*(N+1)], L[batch*(N+1)],
+        work[batch], nx[batch*(N+1)], nu[batch*(N+1)])."""
+        N = self.N
+        din = self.inp.build(in_base)
+        dout = self.out.build(out_base)
+        babt = np.zeros(self.batch * max(N, 1), dtype=DMAT)
+        rsq = np.zeros(self.batch * (N + 1), dtype=DMAT)
+        L = np.zeros(self.batch * (N + 1), dtype=DMAT)
+        work = np.zeros(self.batch, dtype=DMAT)
+        nx = np.zeros(self.batch * (N + 1), dtype=np.int32)
+        nu = np.zeros(self.batch * (N + 1), dtype=np.int32)
+        for i in range(self.batch):
+            for k in range(N):
+                babt[i * N + k] = din[self.i_babt[i]]
+                rsq[i * (N + 1) + k] = din[self.i_rsq[i]]
+            rsq[i * (N + 1) + N] = din[self.i_qn[i]]
+            for k in range(N + 1):
+                L[i * (N + 1) + k] = dout[self.i_L[i][k]]
+            work[i] = dout[self.i_work[i]]
+            nx[i * (N + 1): (i + 1) * (N + 1)] = self.nxs[i]
+            nu[i * (N + 1): (i + 1) * (N + 1) - 1] = self.nus[i]
+        return babt, rsq, L, work, nx, nu
+
+    def args(self, addr_babt, addr_rsq, addr_L, addr_work, addr_nx, addr_nu):
+        N = self.N
+        a = np.zeros(self.batch, dtype=RICCATI_ARG)
+        idx = np.arange(self.batch, dtype=np.uint64)
+        a["N"] = N
+        a["nx"] = addr_nx + idx * 4 * (N + 1)
+        a["nu"] = addr_nu + idx * 4 * (N + 1)
+        a["BAbt"] = addr_babt + idx * DMAT.itemsize * max(N, 1)
+        a["RSQrq"] = addr_rsq + idx * DMAT.itemsize * (N + 1)
+        a["L"] = addr_L + idx * DMAT.itemsize * (N + 1)
+        a["work"] = addr_work + idx * DMAT.itemsize
+        return a
+
+
+class DeviceRiccati:
+    """A RiccatiBatch resident in GPU memory, ready to launch."""
+
+    def __init__(self, rb, in_host_buf):
+        import torch
+        from .device import to_device
+        self.rb = rb
+        self.inbuf = torch.from_numpy(np.ascontiguousarray(in_host_buf)).to("cuda")
+        self.outbuf = torch.zeros(rb.out.nbytes, dtype=torch.uint8, device="cuda")
+        t = rb.tables(self.inbuf.data_ptr(), self.outbuf.data_ptr())
+        self.t_dev = [to_device(x) for x in t]
+        babt, rsq, L, work, nx, nu = self.t_dev
+        self.args = to_device(rb.args(babt.data_ptr(), rsq.data_ptr(), L.data_ptr(),
+                                      work.data_ptr(), nx.data_ptr(), nu.data_ptr()))
+
+    def run(self, stream=0):
+        from . import call
+        call("d_back_ric_trf", self.rb.ps, self.rb.batch, self.args.data_ptr(),
+             (self.rb.max_nk, 0, 0), stream)
+
+    def L(self, i, k):
+        buf = self.outbuf.cpu().numpy()
+        return self.rb.out.get(buf, self.rb.i_L[i][k])
+
+
+class HostRiccati:
+    """The same batch in host memory for the CPU oracle."""
+
+    def __init__(self, rb, in_host_buf):
+        self.rb = rb
+        self.inbuf = np.array(in_host_buf, copy=True)
+        self.outbuf = rb.out.host_buffer()
+        t = rb.tables(self.inbuf.ctypes.data, self.outbuf.ctypes.data)
+        self.t = t
+        babt, rsq, L, work, nx, nu = t
+        self.argv = rb.args(babt.ctypes.data, rsq.ctypes.data, L.ctypes.data,
+                            work.ctypes.data, nx.ctypes.data, nu.ctypes.data)
+
+    def L(self, i, k):
+        return self.rb.out.get(self.outbuf, self.rb.i_L[i][k])

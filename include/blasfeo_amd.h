@@ -1,0 +1,326 @@
+/*
+ * blasfeo_amd.h -- C ABI of the MI355X (gfx950) batched BLASFEO level-3 hot path.
+ *
+ * Two layers are declared here:
+ *
+ *  1. The BLASFEO descriptor types and the BLASFEO struct API for the hot-path
+ *     routines, with the reference's names, argument order and meaning.  The
+ *     struct layouts are byte-identical to the panel-major build of the
+ *     reference (include/blasfeo_common.h:67-103), so a descriptor built by
+ *     BLASFEO code can be handed over unchanged.  In this library the memory a
+ *     descriptor points to (pA, dA) lives in GPU memory; the single-instance
+ *     routines run one GPU instance and return when it is done (the reference
+ *     routines are synchronous and return void as well).
+ *
+ *  2. The batched entry points, one per hot-path routine.  Each takes a
+ *     device-resident array of argument records; record i holds exactly the
+ *     arguments of one reference call (sizes, scalars, pointers to
+ *     device-resident descriptors and their row/column offsets).  They run
+ *     asynchronously on the given HIP stream, allocate nothing, perform no host
+ *     synchronisation and can be captured into a HIP graph.  They return an
+ *     int status for argument/launch errors only; numerical failures follow
+ *     the reference's conventions (zero pivot => zero reciprocal in dA).
+ *
+ * Panel-major addressing (include/blasfeo_common.h:112): element (i,j) of a
+ * matrix with panel size ps and panel stride cn sits at
+ *     pA[(i - i%ps)*cn + j*ps + i%ps].
+ * The panel size is not stored in the descriptor (the reference fixes it at
+ * build time: 4 for the Haswell build, 8 for Skylake-X/Zen5); here every
+ * batched call names it explicitly, and the single-instance routines use the
+ * process-wide value set by blasfeo_amd_set_ps() (default 4).
+ */
+#ifndef BLASFEO_AMD_H
+#define BLASFEO_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* descriptor types: include/blasfeo_common.h:67-103 (PANELMAJ branch)       */
+/* ------------------------------------------------------------------------ */
+
+struct blasfeo_dmat
+	{
+	double *mem;  /* start of the caller's memory chunk */
+	double *pA;   /* pm*cn panel-major doubles */
+	double *dA;   /* reciprocal diagonal cache, min(m,n) rounded up */
+	int m;        /* rows */
+	int n;        /* columns */
+	int pm;       /* rows rounded up to the panel size */
+	int cn;       /* columns rounded up to the panel leading dimension (panel stride) */
+	int use_dA;   /* >0: dA holds valid reciprocals of the first use_dA diagonal entries */
+	int memsize;  /* bytes needed */
+	};
+
+struct blasfeo_smat
+	{
+	float *mem;
+	float *pA;
+	float *dA;
+	int m;
+	int n;
+	int pm;
+	int cn;
+	int use_dA;
+	int memsize;
+	};
+
+struct blasfeo_dvec
+	{
+	double *mem;
+	double *pa;
+	int m;
+	int pm;
+	int memsize;
+	};
+
+struct blasfeo_svec
+	{
+	float *mem;
+	float *pa;
+	int m;
+	int pm;
+	int memsize;
+	};
+
+/* include/blasfeo_common.h:172-185: explicitly panel-major, runtime panel size */
+struct blasfeo_pm_dmat
+	{
+	double *mem;
+	double *pA;
+	double *dA;
+	int m;
+	int n;
+	int pm;
+	int cn;
+	int use_dA;
+	int ps;
+	int memsize;
+	};
+
+/* ------------------------------------------------------------------------ */
+/* status codes and library control                                         */
+/* ------------------------------------------------------------------------ */
+
+#define BLASFEO_AMD_OK          0
+#define BLASFEO_AMD_EINVAL     -1  /* bad argument (batch<0, unsupported ps, NULL array with batch>0) */
+#define BLASFEO_AMD_ELAUNCH    -2  /* HIP runtime reported an error at launch */
+#define BLASFEO_AMD_ENODEV     -3  /* no usable gfx950 device */
+#define BLASFEO_AMD_ETOOBIG    -4  /* a size exceeds what the selected kernel supports */
+#define BLASFEO_AMD_ENOMEM     -5  /* device allocation failed (single-instance helpers only) */
+
+/* HIP stream handle without pulling in HIP headers (hipStream_t is a pointer). */
+typedef void *blasfeo_amd_stream_t;
+
+/* Library version string and the last error message of the calling thread. */
+const char *blasfeo_amd_version(void);
+const char *blasfeo_amd_last_error(void);
+
+/* Panel size used by the single-instance API and by blasfeo_memsize_dmat /
+ * blasfeo_create_dmat (double: 4 or 8, float: panel size is 8 for ps=4 and
+ * 16 for ps=8, following include/blasfeo_block_size.h:99-108 and :53-62).
+ * Returns BLASFEO_AMD_EINVAL for other values. */
+int blasfeo_amd_set_ps(int ps);
+int blasfeo_amd_get_ps(void);
+
+/* ------------------------------------------------------------------------ */
+/* memory and layout helpers (auxiliary/d_aux_lib4.c, d_aux_lib8.c)          */
+/* ------------------------------------------------------------------------ */
+
+/* auxiliary/d_aux_lib4.c:1519-1530 (ps=4, pld=4) and d_aux_lib8.c:64-75 (ps=8, pld=8) */
+size_t blasfeo_memsize_dmat(int m, int n);
+size_t blasfeo_memsize_diag_dmat(int m, int n);
+size_t blasfeo_memsize_dvec(int m);
+/* auxiliary/d_aux_lib4.c:1548-1570 */
+void blasfeo_create_dmat(int m, int n, struct blasfeo_dmat *sA, void *memory);
+void blasfeo_create_dvec(int m, struct blasfeo_dvec *sa, void *memory);
+/* auxiliary/d_aux_common.c:49-83 (runtime panel size, pld=4) */
+size_t blasfeo_pm_memsize_dmat(int ps, int m, int n);
+void blasfeo_pm_create_dmat(int ps, int m, int n, struct blasfeo_pm_dmat *sA, void *memory);
+/* the same layout rules with an explicit panel size for struct blasfeo_dmat */
+size_t blasfeo_amd_memsize_dmat(int ps, int m, int n);
+void blasfeo_amd_create_dmat(int ps, int m, int n, struct blasfeo_dmat *sA, void *memory);
+size_t blasfeo_amd_memsize_smat(int ps, int m, int n);
+void blasfeo_amd_create_smat(int ps, int m, int n, struct blasfeo_smat *sA, void *memory);
+
+/* Column-major <-> panel-major conversion (auxiliary/d_aux_lib4.c:1607, :2238,
+ * :2086).  Work on host or device memory: when sA's memory is device memory the
+ * conversion is staged through host memory and copied synchronously. */
+void blasfeo_pack_dmat(int m, int n, double *A, int lda, struct blasfeo_dmat *sA, int ai, int aj);
+void blasfeo_pack_tran_dmat(int m, int n, double *A, int lda, struct blasfeo_dmat *sA, int ai, int aj);
+void blasfeo_unpack_dmat(int m, int n, struct blasfeo_dmat *sA, int ai, int aj, double *A, int lda);
+void blasfeo_pack_dvec(int m, double *x, int incx, struct blasfeo_dvec *sy, int yi);
+void blasfeo_unpack_dvec(int m, struct blasfeo_dvec *sx, int xi, double *y, int incy);
+
+/* auxiliary/x_aux_ext_dep.c:40-59: zeroed allocation; here in GPU memory. */
+void blasfeo_allocate_dmat(int m, int n, struct blasfeo_dmat *sA);
+void blasfeo_free_dmat(struct blasfeo_dmat *sA);
+void blasfeo_allocate_dvec(int m, struct blasfeo_dvec *sa);
+void blasfeo_free_dvec(struct blasfeo_dvec *sa);
+
+/* ------------------------------------------------------------------------ */
+/* single-instance struct API (include/blasfeo_d_blasfeo_api.h)              */
+/* Descriptors are host structs whose pA/dA point to GPU memory.             */
+/* ------------------------------------------------------------------------ */
+
+/* include/blasfeo_d_blasfeo_api.h:141 */
+void blasfeo_dgemm_nn(int m, int n, int k, double alpha, struct blasfeo_dmat *sA, int ai, int aj, struct blasfeo_dmat *sB, int bi, int bj, double beta, struct blasfeo_dmat *sC, int ci, int cj, struct blasfeo_dmat *sD, int di, int dj);
+/* include/blasfeo_d_blasfeo_api.h:143 */
+void blasfeo_dgemm_nt(int m, int n, int k, double alpha, struct blasfeo_dmat *sA, int ai, int aj, struct blasfeo_dmat *sB, int bi, int bj, double beta, struct blasfeo_dmat *sC, int ci, int cj, struct blasfeo_dmat *sD, int di, int dj);
+/* include/blasfeo_d_blasfeo_api.h:149 */
+void blasfeo_dsyrk_ln(int m, int k, double alpha, struct blasfeo_dmat *sA, int ai, int aj, struct blasfeo_dmat *sB, int bi, int bj, double beta, struct blasfeo_dmat *sC, int ci, int cj, struct blasfeo_dmat *sD, int di, int dj);
+/* include/blasfeo_d_blasfeo_api.h:150 */
+void blasfeo_dsyrk_ln_mn(int m, int n, int k, double alpha, struct blasfeo_dmat *sA, int ai, int aj, struct blasfeo_dmat *sB, int bi, int bj, double beta, struct blasfeo_dmat *sC, int ci, int cj, struct blasfeo_dmat *sD, int di, int dj);
+/* include/blasfeo_d_blasfeo_api.h:195 */
+void blasfeo_dtrmm_rlnn(int m, int n, double alpha, struct blasfeo_dmat *sA, int ai, int aj, struct blasfeo_dmat *sB, int bi, int bj, struct blasfeo_dmat *sD, int di, int dj);
+/* include/blasfeo_d_blasfeo_api.h:222 */
+void blasfeo_dtrsm_rltn(int m, int n, double alpha, struct blasfeo_dmat *sA, int ai, int aj, struct blasfeo_dmat *sB, int bi, int bj, struct blasfeo_dmat *sD, int di, int dj);
+/* include/blasfeo_d_blasfeo_api.h:280-281 */
+void blasfeo_dpotrf_l(int m, struct blasfeo_dmat *sC, int ci, int cj, struct blasfeo_dmat *sD, int di, int dj);
+void blasfeo_dpotrf_l_mn(int m, int n, struct blasfeo_dmat *sC, int ci, int cj, struct blasfeo_dmat *sD, int di, int dj);
+/* include/blasfeo_d_blasfeo_api.h:284-285 */
+void blasfeo_dsyrk_dpotrf_ln(int m, int k, struct blasfeo_dmat *sA, int ai, int aj, struct blasfeo_dmat *sB, int bi, int bj, struct blasfeo_dmat *sC, int ci, int cj, struct blasfeo_dmat *sD, int di, int dj);
+/* include/blasfeo_d_blasfeo_api.h:291 */
+void blasfeo_dgetrf_rp(int m, int n, struct blasfeo_dmat *sC, int ci, int cj, struct blasfeo_dmat *sD, int di, int dj, int *ipiv);
+
+/* ------------------------------------------------------------------------ */
+/* batched argument records (device-resident arrays, one record per call)    */
+/* ------------------------------------------------------------------------ */
+
+/* one blasfeo_dgemm_nt / _nn / dsyrk_ln(_mn) call (include/blasfeo_d_blasfeo_api.h:141-150).
+ * For syrk_ln, n is ignored (the square m x m lower triangle is written);
+ * for syrk_ln_mn, n columns of the m x n lower trapezoid are written. */
+struct blasfeo_amd_dgemm_arg
+	{
+	int m, n, k;
+	int ai, aj, bi, bj, ci, cj, di, dj;
+	double alpha, beta;
+	struct blasfeo_dmat *sA, *sB, *sC, *sD;
+	};
+
+/* one blasfeo_dtrsm_rltn / dtrmm_rlnn call (include/blasfeo_d_blasfeo_api.h:195,222) */
+struct blasfeo_amd_dtrsm_arg
+	{
+	int m, n;
+	int ai, aj, bi, bj, di, dj;
+	double alpha;
+	struct blasfeo_dmat *sA, *sB, *sD;
+	};
+
+/* one blasfeo_dpotrf_l(_mn) / dgetrf_rp call (include/blasfeo_d_blasfeo_api.h:280-291).
+ * n is used by potrf_l_mn and getrf_rp; ipiv (device, >= min(m,n) ints) by getrf_rp. */
+struct blasfeo_amd_dfact_arg
+	{
+	int m, n;
+	int ci, cj, di, dj;
+	struct blasfeo_dmat *sC, *sD;
+	int *ipiv;
+	};
+
+/* one blasfeo_dsyrk_dpotrf_ln(_mn) call (include/blasfeo_d_blasfeo_api.h:284-285) */
+struct blasfeo_amd_dsyrk_dpotrf_arg
+	{
+	int m, n, k;
+	int ai, aj, bi, bj, ci, cj, di, dj;
+	struct blasfeo_dmat *sA, *sB, *sC, *sD;
+	};
+
+/* FP32 twins (include/blasfeo_s_blasfeo_api.h) */
+struct blasfeo_amd_sgemm_arg
+	{
+	int m, n, k;
+	int ai, aj, bi, bj, ci, cj, di, dj;
+	float alpha, beta;
+	struct blasfeo_smat *sA, *sB, *sC, *sD;
+	};
+struct blasfeo_amd_strsm_arg
+	{
+	int m, n;
+	int ai, aj, bi, bj, di, dj;
+	float alpha;
+	struct blasfeo_smat *sA, *sB, *sD;
+	};
+struct blasfeo_amd_sfact_arg
+	{
+	int m, n;
+	int ci, cj, di, dj;
+	struct blasfeo_smat *sC, *sD;
+	int *ipiv;
+	};
+
+/* Optional host-side hints: upper bounds of the sizes in the batch, used to
+ * pick the kernel variant.  Zero means unknown.  May be NULL. */
+struct blasfeo_amd_hint
+	{
+	int max_m, max_n, max_k;
+	};
+
+/* ------------------------------------------------------------------------ */
+/* batched entry points (device-resident args; asynchronous on `stream`)     */
+/* ps: panel size of every matrix in the batch (double: 4 or 8).             */
+/* ------------------------------------------------------------------------ */
+
+/* replaces batch x blasfeo_dgemm_nt (blasfeo_hp_pm/d_blas3_lib4.c:7215,727) */
+int blasfeo_amd_dgemm_nt_batched(int ps, int batch, const struct blasfeo_amd_dgemm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dgemm_nn (blasfeo_hp_pm/d_blas3_lib4.c:7208,315) */
+int blasfeo_amd_dgemm_nn_batched(int ps, int batch, const struct blasfeo_amd_dgemm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dsyrk_ln (blasfeo_hp_pm/d_blas3_lib4.c:7460,5400) */
+int blasfeo_amd_dsyrk_ln_batched(int ps, int batch, const struct blasfeo_amd_dgemm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dsyrk_ln_mn (blasfeo_hp_pm/d_blas3_lib4.c:6459) */
+int blasfeo_amd_dsyrk_ln_mn_batched(int ps, int batch, const struct blasfeo_amd_dgemm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dtrsm_rltn (blasfeo_hp_pm/d_blas3_lib4.c:7306,4123) */
+int blasfeo_amd_dtrsm_rltn_batched(int ps, int batch, const struct blasfeo_amd_dtrsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dtrmm_rlnn (blasfeo_hp_pm/d_blas3_lib4.c:4893) */
+int blasfeo_amd_dtrmm_rlnn_batched(int ps, int batch, const struct blasfeo_amd_dtrsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dpotrf_l (blasfeo_hp_pm/d_lapack_lib4.c:4269,918) */
+int blasfeo_amd_dpotrf_l_batched(int ps, int batch, const struct blasfeo_amd_dfact_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dpotrf_l_mn (blasfeo_hp_pm/d_lapack_lib4.c:1155) */
+int blasfeo_amd_dpotrf_l_mn_batched(int ps, int batch, const struct blasfeo_amd_dfact_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dsyrk_dpotrf_ln(_mn) (blasfeo_hp_pm/d_lapack_lib4.c:1848) */
+int blasfeo_amd_dsyrk_dpotrf_ln_batched(int ps, int batch, const struct blasfeo_amd_dsyrk_dpotrf_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dgetrf_rp (blasfeo_hp_pm/d_lapack_lib4.c:4311,2396) */
+int blasfeo_amd_dgetrf_rp_batched(int ps, int batch, const struct blasfeo_amd_dfact_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+
+/* FP32 twins: blasfeo_hp_pm/s_blas3_lib8.c:48,316,751,972; s_lapack_lib8.c:49,931.
+ * ps is the float panel size (4, 8 or 16). */
+int blasfeo_amd_sgemm_nt_batched(int ps, int batch, const struct blasfeo_amd_sgemm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+int blasfeo_amd_sgemm_nn_batched(int ps, int batch, const struct blasfeo_amd_sgemm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+int blasfeo_amd_ssyrk_ln_batched(int ps, int batch, const struct blasfeo_amd_sgemm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+int blasfeo_amd_strsm_rltn_batched(int ps, int batch, const struct blasfeo_amd_strsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+int blasfeo_amd_spotrf_l_batched(int ps, int batch, const struct blasfeo_amd_sfact_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+int blasfeo_amd_sgetrf_rp_batched(int ps, int batch, const struct blasfeo_amd_sfact_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Riccati recursion (examples/example_d_riccati_recursion.c:115-140)        */
+/* ------------------------------------------------------------------------ */
+
+/* One instance of d_back_ric_trf_libstr(N, nx, nu, hsBAbt, hsRSQrq, hsL, hswork_mat).
+ * nx, nu: device arrays of N+1 ints.  BAbt: device array of N descriptors,
+ * BAbt[k] holds [B_k; A_k]^T, (nu[k]+nx[k]) x nx[k+1].  RSQrq: N+1 descriptors,
+ * lower triangle of [R S; S^T Q], (nu[k]+nx[k])^2.  L: N+1 output descriptors.
+ * work: one descriptor of at least max_k (nu[k]+nx[k]) x nx[k+1], used only
+ * by the unfused (graph) form. */
+struct blasfeo_amd_riccati_arg
+	{
+	int N;
+	const int *nx;
+	const int *nu;
+	struct blasfeo_dmat *BAbt;
+	struct blasfeo_dmat *RSQrq;
+	struct blasfeo_dmat *L;
+	struct blasfeo_dmat *work;
+	};
+
+/* Fused backward factorisation sweep, one workgroup per instance, all stages in
+ * one launch (the per-stage trmm_rlnn + syrk_dpotrf_ln chain of
+ * examples/example_d_riccati_recursion.c:120-138). */
+int blasfeo_amd_d_back_ric_trf_batched(int ps, int batch, const struct blasfeo_amd_riccati_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLASFEO_AMD_H */

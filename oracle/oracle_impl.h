@@ -1,0 +1,271 @@
+/*
+ * oracle_impl.h -- body of the CPU oracle, instantiated once for double and
+ * once for float by oracle.c (REAL, MAT, PFX are defined there).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the GPU library is compared
+ * against; nothing in the shipped library links or calls it.
+ *
+ * Each routine restates the arithmetic of the reference's REFERENCE backend
+ * (blasfeo_ref/x_blas3_ref.c, blasfeo_ref/x_lapack_ref.c) on panel-major
+ * storage with a runtime panel size: every output element is accumulated
+ * over k in ascending order exactly as the reference's 2x2-unrolled loops do,
+ * reciprocals are formed as 1/x and applied by multiplication, and the
+ * zero-pivot conventions are the reference's.  The dA / use_dA side effects
+ * follow the HIGH_PERFORMANCE drivers (blasfeo_hp_pm/d_lapack_lib4.c:947-950,
+ * :1185, :1882, :2421; blasfeo_hp_pm/d_blas3_lib4.c:4162-4178), which are the
+ * ones the drop-in replaces.
+ */
+
+#define EL(sM, i, j) ((sM)->pA[((i) - ((i) % ps)) * (sM)->cn + (j) * ps + ((i) % ps)])
+
+/* D <= beta*C + alpha*A*B^T ; blasfeo_ref/x_blas3_ref.c:142-235 */
+void PFX(gemm_nt)(int ps, int m, int n, int k, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, REAL beta, MAT *sC, int ci, int cj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	sD->use_dA = 0;
+	for(int j = 0; j < n; j++)
+		for(int i = 0; i < m; i++)
+			{
+			REAL c = 0;
+			for(int l = 0; l < k; l++)
+				c += EL(sA, ai+i, aj+l) * EL(sB, bi+j, bj+l);
+			EL(sD, di+i, dj+j) = alpha * c + beta * EL(sC, ci+i, cj+j);
+			}
+	}
+
+/* D <= beta*C + alpha*A*B ; blasfeo_ref/x_blas3_ref.c:41-135 */
+void PFX(gemm_nn)(int ps, int m, int n, int k, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, REAL beta, MAT *sC, int ci, int cj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	sD->use_dA = 0;
+	for(int j = 0; j < n; j++)
+		for(int i = 0; i < m; i++)
+			{
+			REAL c = 0;
+			for(int l = 0; l < k; l++)
+				c += EL(sA, ai+i, aj+l) * EL(sB, bi+l, bj+j);
+			EL(sD, di+i, dj+j) = alpha * c + beta * EL(sC, ci+i, cj+j);
+			}
+	}
+
+/* lower trapezoid of D <= beta*C + alpha*A*B^T, n columns of an m x n block
+ * (n == m is blasfeo_ref/x_blas3_ref.c:4130-4222 syrk_ln, otherwise syrk_ln_mn :4229) */
+void PFX(syrk_ln_mn)(int ps, int m, int n, int k, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, REAL beta, MAT *sC, int ci, int cj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	sD->use_dA = 0;
+	for(int j = 0; j < n; j++)
+		for(int i = j; i < m; i++)
+			{
+			REAL c = 0;
+			for(int l = 0; l < k; l++)
+				c += EL(sA, ai+i, aj+l) * EL(sB, bi+j, bj+l);
+			EL(sD, di+i, dj+j) = beta * EL(sC, ci+i, cj+j) + alpha * c;
+			}
+	}
+
+void PFX(syrk_ln)(int ps, int m, int k, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, REAL beta, MAT *sC, int ci, int cj, MAT *sD, int di, int dj)
+	{
+	PFX(syrk_ln_mn)(ps, m, m, k, alpha, sA, ai, aj, sB, bi, bj, beta, sC, ci, cj, sD, di, dj);
+	}
+
+/* D <= alpha * B * A^{-T}, A lower n x n with reciprocal diagonal cached in
+ * sA->dA (blasfeo_ref/x_blas3_ref.c:1642-1752; cache rule of
+ * blasfeo_hp_pm/d_blas3_lib4.c:4162-4178) */
+void PFX(trsm_rltn)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	sD->use_dA = 0;
+	REAL *dA = sA->dA;
+	if(ai == 0 && aj == 0)
+		{
+		if(sA->use_dA < n)
+			{
+			for(int i = 0; i < n; i++)
+				dA[i] = (REAL)1.0 / EL(sA, i, i);
+			sA->use_dA = n;
+			}
+		}
+	else
+		{
+		for(int i = 0; i < n; i++)
+			dA[i] = (REAL)1.0 / EL(sA, ai+i, aj+i);
+		sA->use_dA = 0;
+		}
+	for(int j = 0; j < n; j++)
+		{
+		REAL inv = dA[j];
+		for(int i = 0; i < m; i++)
+			{
+			REAL c = alpha * EL(sB, bi+i, bj+j);
+			for(int l = 0; l < j; l++)
+				c -= EL(sD, di+i, dj+l) * EL(sA, ai+j, aj+l);
+			EL(sD, di+i, dj+j) = c * inv;
+			}
+		}
+	}
+
+/* D <= alpha * B * A, A lower n x n ; blasfeo_ref/x_blas3_ref.c:3218-3330 */
+void PFX(trmm_rlnn)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	sD->use_dA = 0;
+	for(int j = 0; j < n; j++)
+		for(int i = 0; i < m; i++)
+			{
+			REAL c = 0;
+			for(int l = j; l < n; l++)
+				c += EL(sB, bi+i, bj+l) * EL(sA, ai+l, aj+j);
+			EL(sD, di+i, dj+j) = alpha * c;
+			}
+	}
+
+/* one left-looking Cholesky column sweep shared by potrf_l, potrf_l_mn and
+ * syrk_dpotrf_ln(_mn): column j of D <= (C + A*B^T)(.,j) - sum_{l<j} D(.,l)D(j,l),
+ * then scaled by 1/sqrt(pivot) (zero if pivot <= 0 or NaN).
+ * blasfeo_ref/x_lapack_ref.c:41-176, :185-326, :399-566 */
+static void PFX(chol_core)(int ps, int m, int n, int k, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sC, int ci, int cj, MAT *sD, int di, int dj)
+	{
+	REAL *dD = sD->dA;
+	for(int j = 0; j < n; j++)
+		{
+		REAL c = EL(sC, ci+j, cj+j);
+		for(int l = 0; l < k; l++)
+			c += EL(sA, ai+j, aj+l) * EL(sB, bi+j, bj+l);
+		for(int l = 0; l < j; l++)
+			c -= EL(sD, di+j, dj+l) * EL(sD, di+j, dj+l);
+		REAL inv = c > 0 ? (REAL)1.0 / SQRT(c) : (REAL)0.0;
+		dD[j] = inv;
+		EL(sD, di+j, dj+j) = c * inv;
+		for(int i = j+1; i < m; i++)
+			{
+			REAL d = EL(sC, ci+i, cj+j);
+			for(int l = 0; l < k; l++)
+				d += EL(sA, ai+i, aj+l) * EL(sB, bi+j, bj+l);
+			for(int l = 0; l < j; l++)
+				d -= EL(sD, di+i, dj+l) * EL(sD, di+j, dj+l);
+			EL(sD, di+i, dj+j) = d * inv;
+			}
+		}
+	}
+
+/* blasfeo_ref/x_lapack_ref.c:41-176; use_dA rule blasfeo_hp_pm/d_lapack_lib4.c:947-950 */
+void PFX(potrf_l)(int ps, int m, MAT *sC, int ci, int cj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0)
+		return;
+	sD->use_dA = (di == 0 && dj == 0) ? m : 0;
+	PFX(chol_core)(ps, m, m, 0, sD, 0, 0, sD, 0, 0, sC, ci, cj, sD, di, dj);
+	}
+
+/* blasfeo_ref/x_lapack_ref.c:185-326; use_dA rule blasfeo_hp_pm/d_lapack_lib4.c:1185 */
+void PFX(potrf_l_mn)(int ps, int m, int n, MAT *sC, int ci, int cj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	if(n > m)
+		n = m;
+	sD->use_dA = (di == 0 && dj == 0) ? 1 : 0;
+	PFX(chol_core)(ps, m, n, 0, sD, 0, 0, sD, 0, 0, sC, ci, cj, sD, di, dj);
+	}
+
+/* D <= chol(C + A*B^T) on the m x n lower trapezoid;
+ * blasfeo_ref/x_lapack_ref.c:399-566 (n == m) and :570-738;
+ * use_dA rule blasfeo_hp_pm/d_lapack_lib4.c:1545,1882 */
+void PFX(syrk_potrf_ln_mn)(int ps, int m, int n, int k, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sC, int ci, int cj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	if(n > m)
+		n = m;
+	sD->use_dA = (di == 0 && dj == 0) ? 1 : 0;
+	PFX(chol_core)(ps, m, n, k, sA, ai, aj, sB, bi, bj, sC, ci, cj, sD, di, dj);
+	}
+
+void PFX(syrk_potrf_ln)(int ps, int m, int k, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sC, int ci, int cj, MAT *sD, int di, int dj)
+	{
+	PFX(syrk_potrf_ln_mn)(ps, m, m, k, sA, ai, aj, sB, bi, bj, sC, ci, cj, sD, di, dj);
+	}
+
+/* LU with partial row pivoting, left-looking (Crout) column sweep of
+ * blasfeo_ref/x_lapack_ref.c:1156-1536: for each column j the upper part is
+ * corrected row by row, the lower part is corrected, the pivot is the first
+ * row of largest |.| (strict '>' from dmax = 0, so an all-zero or NaN column
+ * keeps row j), the whole rows j and ip are swapped, and the sub-diagonal is
+ * scaled by the reciprocal pivot (0 if the pivot is 0), which is stored in dA.
+ * ipiv[j] is the 0-based absolute row swapped with row j.  Columns j >= m (m < n)
+ * receive only the upper correction. */
+void PFX(getrf_rp)(int ps, int m, int n, MAT *sC, int ci, int cj, MAT *sD, int di, int dj, int *ipiv)
+	{
+	sD->use_dA = (di == 0 && dj == 0) ? 1 : 0;
+	if(m <= 0 || n <= 0)
+		return;
+	REAL *dD = sD->dA;
+	if(&EL(sC, ci, cj) != &EL(sD, di, dj))
+		for(int j = 0; j < n; j++)
+			for(int i = 0; i < m; i++)
+				EL(sD, di+i, dj+j) = EL(sC, ci+i, cj+j);
+	int p = m < n ? m : n;
+	for(int j = 0; j < n; j++)
+		{
+		int iu = j < m ? j : m;
+		for(int i = 0; i < iu; i++)
+			{
+			REAL d = EL(sD, di+i, dj+j);
+			for(int l = 0; l < i; l++)
+				d -= EL(sD, di+i, dj+l) * EL(sD, di+l, dj+j);
+			EL(sD, di+i, dj+j) = d;
+			}
+		if(j >= p)
+			continue;
+		REAL dmax = 0;
+		int ip = j;
+		for(int i = j; i < m; i++)
+			{
+			REAL d = EL(sD, di+i, dj+j);
+			for(int l = 0; l < j; l++)
+				d -= EL(sD, di+i, dj+l) * EL(sD, di+l, dj+j);
+			EL(sD, di+i, dj+j) = d;
+			REAL a = d > 0 ? d : -d;
+			if(a > dmax)
+				{
+				dmax = a;
+				ip = i;
+				}
+			}
+		ipiv[j] = ip;
+		if(ip != j)
+			for(int l = 0; l < n; l++)
+				{
+				REAL t = EL(sD, di+j, dj+l);
+				EL(sD, di+j, dj+l) = EL(sD, di+ip, dj+l);
+				EL(sD, di+ip, dj+l) = t;
+				}
+		REAL piv = EL(sD, di+j, dj+j);
+		REAL inv = piv != 0 ? (REAL)1.0 / piv : (REAL)0.0;
+		dD[j] = inv;
+		for(int i = j+1; i < m; i++)
+			EL(sD, di+i, dj+j) *= inv;
+		}
+	}
+
+/* backward Riccati factorisation d_back_ric_trf_libstr,
+ * examples/example_d_riccati_recursion.c:115-140 (fused syrk_dpotrf branch) */
+void PFX(ric_trf)(int ps, int N, const int *nx, const int *nu, MAT *BAbt, MAT *RSQrq, MAT *L, MAT *work)
+	{
+	PFX(potrf_l)(ps, nx[N], &RSQrq[N], 0, 0, &L[N], 0, 0);
+	for(int s = 0; s < N; s++)
+		{
+		int k = N - s - 1;
+		int nk = nu[k] + nx[k];
+		PFX(trmm_rlnn)(ps, nk, nx[k+1], (REAL)1.0, &L[k+1], nu[k+1], nu[k+1], &BAbt[k], 0, 0, work, 0, 0);
+		PFX(syrk_potrf_ln)(ps, nk, nx[k+1], work, 0, 0, work, 0, 0, &RSQrq[k], 0, 0, &L[k], 0, 0);
+		}
+	}
+
+#undef EL

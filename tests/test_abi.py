@@ -1,0 +1,143 @@
+"""CPU checks of the C ABI: the library loads and exports every function
+include/blasfeo_amd.h declares, the struct layouts seen by C match the
+Python mirrors, the layout helpers follow the reference's memsize/create
+rules (auxiliary/d_aux_lib4.c:1519-1570, d_aux_lib8.c:64-115) and argument
+errors are reported without touching a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import blasfeo_amd as bfa
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "include", "blasfeo_amd.h")
+
+
+def _declared():
+    txt = open(HDR).read()
+    return sorted(set(re.findall(r"^[a-z][\w \*]*?\b(blasfeo_\w+)\(", txt, re.M)))
+
+
+def test_exports_every_declared_symbol():
+    L = bfa.lib()
+    names = _declared()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_c(tmp_path):
+    src = tmp_path / "lay.c"
+    checks = {
+        "blasfeo_dmat": (bfa.DMAT, ["pA", "dA", "m", "n", "pm", "cn", "use_dA", "memsize"]),
+        "blasfeo_amd_dgemm_arg": (bfa.DGEMM_ARG, ["k", "dj", "alpha", "beta", "sA", "sD"]),
+        "blasfeo_amd_sgemm_arg": (bfa.SGEMM_ARG, ["alpha", "beta", "sA", "sD"]),
+        "blasfeo_amd_dtrsm_arg": (bfa.DTRSM_ARG, ["dj", "alpha", "sA", "sD"]),
+        "blasfeo_amd_strsm_arg": (bfa.STRSM_ARG, ["alpha", "sA", "sD"]),
+        "blasfeo_amd_dfact_arg": (bfa.DFACT_ARG, ["dj", "sC", "sD", "ipiv"]),
+        "blasfeo_amd_dsyrk_dpotrf_arg": (bfa.DSYRK_DPOTRF_ARG, ["dj", "sA", "sD"]),
+        "blasfeo_amd_riccati_arg": (bfa.RICCATI_ARG, ["nx", "nu", "BAbt", "RSQrq", "L", "work"]),
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HDR, "int main(void){"]
+    for s, (_, fields) in checks.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(struct %s));' % (s, s))
+        for f in fields:
+            lines.append('printf("%s %s %%zu\\n", offsetof(struct %s, %s));' % (s, f, s, f))
+    lines.append('printf("blasfeo_smat size %zu\\n", sizeof(struct blasfeo_smat));')
+    lines.append("return 0;}")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "lay"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line.strip():
+            a, b, c = line.split()
+            got[(a, b)] = int(c)
+    for s, (dt, fields) in checks.items():
+        assert got[(s, "size")] == dt.itemsize, s
+        for f in fields:
+            assert got[(s, f)] == dt.fields[f][1], (s, f)
+    assert got[("blasfeo_smat", "size")] == bfa.SMAT.itemsize
+
+
+@pytest.mark.parametrize("ps,m,n,expect", [(4, 32, 32, 8448), (8, 32, 32, 8704), (4, 256, 256, 526336)])
+def test_memsize_matches_reference_rule(ps, m, n, expect):
+    L = bfa.lib()
+    assert L.blasfeo_amd_memsize_dmat(ps, m, n) == expect
+    assert bfa.memsize(np.float64, ps, m, n) == expect
+
+
+def test_memsize_sweep_and_create():
+    L = bfa.lib()
+    for ps in (4, 8):
+        for m in range(1, 40, 3):
+            for n in range(1, 40, 5):
+                assert L.blasfeo_amd_memsize_dmat(ps, m, n) == bfa.memsize(np.float64, ps, m, n)
+    for ps in (4, 8, 16):
+        assert L.blasfeo_amd_memsize_smat(ps, 13, 7) == bfa.memsize(np.float32, ps, 13, 7)
+    buf = np.zeros(10000, dtype=np.uint8)
+    d = np.zeros(1, dtype=bfa.DMAT)
+    L.blasfeo_amd_create_dmat(8, 13, 7, d.ctypes.data, buf.ctypes.data)
+    assert d[0]["pm"] == 16 and d[0]["cn"] == 8 and d[0]["m"] == 13 and d[0]["n"] == 7
+    assert d[0]["dA"] - d[0]["pA"] == 16 * 8 * 8 and d[0]["use_dA"] == 0
+    assert d[0]["memsize"] == bfa.memsize(np.float64, 8, 13, 7)
+    lay = bfa.Layout(np.float64, 8)
+    lay.add(13, 7)
+    ld = lay.build(buf.ctypes.data)
+    for f in ("pA", "dA", "m", "n", "pm", "cn", "memsize"):
+        assert ld[0][f] == d[0][f]
+
+
+def test_pm_pack_roundtrip():
+    rng = np.random.default_rng(0)
+    for ps in (4, 8, 16):
+        A = rng.standard_normal((23, 9))
+        flat = bfa.pm_pack(A, ps, 12)
+        np.testing.assert_array_equal(bfa.pm_unpack(flat, ps, 12, 23, 9), A)
+        # element (i,j) at (i - i%ps)*cn + j*ps + i%ps (include/blasfeo_common.h:112)
+        i, j = 17, 5
+        assert flat[(i - i % ps) * 12 + j * ps + i % ps] == A[i, j]
+
+
+def test_host_pack_unpack_on_host_memory():
+    L = bfa.lib()
+    L.blasfeo_pack_dmat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.blasfeo_unpack_dmat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+    assert L.blasfeo_amd_set_ps(4) == 0
+    buf = np.zeros(L.blasfeo_memsize_dmat(11, 6) + 64, dtype=np.uint8)
+    d = np.zeros(1, dtype=bfa.DMAT)
+    L.blasfeo_create_dmat(11, 6, d.ctypes.data, buf.ctypes.data)
+    A = np.asfortranarray(np.arange(30.0).reshape(6, 5))
+    L.blasfeo_pack_dmat(6, 5, A.ctypes.data, 6, d.ctypes.data, 3, 1)
+    flat = buf[: 12 * 8 * 8].view(np.float64)
+    dense = bfa.pm_unpack(flat, 4, 8, 11, 6)
+    np.testing.assert_array_equal(dense[3:9, 1:6], A)
+    B = np.zeros((6, 5), order="F")
+    L.blasfeo_unpack_dmat(6, 5, d.ctypes.data, 3, 1, B.ctypes.data, 6)
+    np.testing.assert_array_equal(B, A)
+
+
+def test_argument_errors_without_gpu():
+    L = bfa.lib()
+    assert L.blasfeo_amd_set_ps(5) == -1
+    assert L.blasfeo_amd_set_ps(4) == 0
+    f = L.blasfeo_amd_dgemm_nt_batched
+    assert f(4, -1, None, None, None) == -1
+    assert b"batch" in L.blasfeo_amd_last_error()
+    assert f(4, 0, None, None, None) == 0
+    assert f(4, 3, None, None, None) == -1
+    assert f(5, 1, ctypes.c_void_p(8), None, None) == -1
+    assert b"panel size" in L.blasfeo_amd_last_error()
+    assert L.blasfeo_amd_dtrsm_rltn_batched(4, 1, ctypes.c_void_p(8), None, None) == -1
+    h = bfa.HINT(100, 300, 0)
+    assert L.blasfeo_amd_dtrsm_rltn_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
+    h = bfa.HINT(5000, 0, 0)
+    assert L.blasfeo_amd_dpotrf_l_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
+    assert L.blasfeo_amd_d_back_ric_trf_batched(4, 1, ctypes.c_void_p(8), None, None) == -1

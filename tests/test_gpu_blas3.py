@@ -1,0 +1,136 @@
+"""GPU parity of the batched level-3 routines against the CPU oracle:
+gemm_nt, gemm_nn, syrk_ln, syrk_ln_mn, trmm_rlnn, trsm_rltn (FP64 and FP32),
+with random sizes crossing the 64x64 tile edges and random row/column offsets
+that cross panel boundaries, as the reference's sweep does (tests/test_x.c:320-388)."""
+import numpy as np
+import pytest
+
+import blasfeo_amd as bfa
+import oracle
+from helpers import Batch, run_both, compare_buffers
+
+pytestmark = pytest.mark.gpu
+
+
+def _gemm_records(b, rng, nrec, op, maxdim=140):
+    recs = []
+    ps = b.ps
+    for r in range(nrec):
+        m, n, k = (int(x) for x in rng.integers(1, maxdim + 1, 3))
+        if r == 0:
+            k = 0
+        if op in ("syrk_ln",):
+            n = m
+        ai, bi, ci, di = (int(x) for x in rng.integers(0, 2 * ps, 4))
+        aj, bj, cj, dj = (int(x) for x in rng.integers(0, 4, 4))
+        if op == "gemm_nn":
+            A = b.mat(ai + m, aj + k)
+            B = b.mat(bi + k, bj + n)
+        else:
+            A = b.mat(ai + m, aj + k)
+            B = b.mat(bi + n, bj + k)
+        C = b.mat(ci + m, cj + n)
+        D = b.mat(di + m, dj + n, "const", -1.0)
+        alpha = float(rng.choice([1.0, -1.0, 0.5]))
+        beta = float(rng.choice([1.0, 0.0, 1.5]))
+        recs.append(dict(m=m, n=n, k=k, ai=ai, aj=aj, bi=bi, bj=bj, ci=ci, cj=cj, di=di, dj=dj,
+                         alpha=alpha, beta=beta, sA=A, sB=B, sC=C, sD=D))
+    return recs
+
+
+@pytest.mark.parametrize("prec", ["d", "s"])
+@pytest.mark.parametrize("ps", [4, 8])
+@pytest.mark.parametrize("op", ["gemm_nt", "gemm_nn", "syrk_ln", "syrk_ln_mn"])
+def test_gemm_family(gpu, prec, ps, op):
+    if prec == "s" and op == "syrk_ln_mn":
+        pytest.skip("no FP32 syrk_ln_mn entry")
+    rng = np.random.default_rng(100 + ps)
+    b = Batch(prec, ps, seed=ps)
+    recs = _gemm_records(b, rng, 24, op)
+    buf = b.host()
+    f = oracle.fn(prec, op)
+
+    def ocall(h, r):
+        if op == "syrk_ln":
+            f(ps, r["m"], r["k"], r["alpha"], h.desc_addr(r["sA"]), r["ai"], r["aj"],
+              h.desc_addr(r["sB"]), r["bi"], r["bj"], r["beta"], h.desc_addr(r["sC"]), r["ci"], r["cj"],
+              h.desc_addr(r["sD"]), r["di"], r["dj"])
+        else:
+            f(ps, r["m"], r["n"], r["k"], r["alpha"], h.desc_addr(r["sA"]), r["ai"], r["aj"],
+              h.desc_addr(r["sB"]), r["bi"], r["bj"], r["beta"], h.desc_addr(r["sC"]), r["ci"], r["cj"],
+              h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    argdt = bfa.DGEMM_ARG if prec == "d" else bfa.SGEMM_ARG
+    hint = (160, 160, 160)
+    g, gd, rr, rd = run_both(b, buf, argdt, recs, prec + op, ocall, hint)
+    compare_buffers(b, g, rr, "%s%s ps=%d" % (prec, op, ps))
+    assert (gd["use_dA"] == rd["use_dA"]).all()
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_gemm_inplace_and_empty(gpu, ps):
+    """D aliasing C (accumulate in place) and m=0 / n=0 no-ops."""
+    rng = np.random.default_rng(7)
+    b = Batch("d", ps, seed=11)
+    recs = []
+    for r in range(8):
+        m, n, k = (int(x) for x in rng.integers(1, 100, 3))
+        A = b.mat(m + 3, k)
+        B = b.mat(n, k + 2)
+        C = b.mat(m + 5, n + 1)
+        recs.append(dict(m=m, n=n, k=k, ai=3, aj=0, bi=0, bj=2, ci=5, cj=1, di=5, dj=1,
+                         alpha=1.0, beta=1.0, sA=A, sB=B, sC=C, sD=C))
+    E = b.mat(8, 8, "const", 3.0)
+    recs.append(dict(m=0, n=5, k=3, ai=0, aj=0, bi=0, bj=0, ci=0, cj=0, di=0, dj=0,
+                     alpha=1.0, beta=1.0, sA=E, sB=E, sC=E, sD=E))
+    recs.append(dict(m=5, n=0, k=3, ai=0, aj=0, bi=0, bj=0, ci=0, cj=0, di=0, dj=0,
+                     alpha=1.0, beta=1.0, sA=E, sB=E, sC=E, sD=E))
+    buf = b.host()
+    f = oracle.fn("d", "gemm_nt")
+
+    def ocall(h, r):
+        f(ps, r["m"], r["n"], r["k"], r["alpha"], h.desc_addr(r["sA"]), r["ai"], r["aj"],
+          h.desc_addr(r["sB"]), r["bi"], r["bj"], r["beta"], h.desc_addr(r["sC"]), r["ci"], r["cj"],
+          h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    g, gd, rr, rd = run_both(b, buf, bfa.DGEMM_ARG, recs, "dgemm_nt", ocall, (100, 100, 100))
+    compare_buffers(b, g, rr, "dgemm_nt in place ps=%d" % ps)
+
+
+@pytest.mark.parametrize("prec", ["d", "s"])
+@pytest.mark.parametrize("ps", [4, 8])
+@pytest.mark.parametrize("op", ["trsm_rltn", "trmm_rlnn"])
+def test_trxm(gpu, prec, ps, op):
+    rng = np.random.default_rng(200 + ps)
+    b = Batch(prec, ps, seed=ps + 3)
+    recs = []
+    for r in range(20):
+        m = int(rng.integers(1, 150))
+        n = int(rng.integers(1, 90))
+        ai, bi, di = (int(x) for x in rng.integers(0, 2 * ps, 3))
+        aj, bj, dj = (int(x) for x in rng.integers(0, 3, 3))
+        if r % 3 == 0:
+            ai = aj = 0
+        # well-conditioned lower-triangular A: unit-ish diagonal
+        Ad = np.tril(rng.standard_normal((ai + n, aj + n)) * 0.1)
+        for t in range(n):
+            Ad[ai + t, aj + t] = 1.0 + rng.uniform(0.5, 1.5)
+        A = b.mat(ai + n, aj + n, "data", Ad)
+        B = b.mat(bi + m, bj + n)
+        D = b.mat(di + m, dj + n, "const", -1.0)
+        recs.append(dict(m=m, n=n, ai=ai, aj=aj, bi=bi, bj=bj, di=di, dj=dj,
+                         alpha=float(rng.choice([1.0, -0.5, 2.0])), sA=A, sB=B, sD=D))
+    buf = b.host()
+    f = oracle.fn(prec, op)
+
+    def ocall(h, r):
+        f(ps, r["m"], r["n"], r["alpha"], h.desc_addr(r["sA"]), r["ai"], r["aj"],
+          h.desc_addr(r["sB"]), r["bi"], r["bj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    argdt = bfa.DTRSM_ARG if prec == "d" else bfa.STRSM_ARG
+    name = prec + op
+    if prec == "s" and op == "trmm_rlnn":
+        pytest.skip("no FP32 trmm entry")
+    g, gd, rr, rd = run_both(b, buf, argdt, recs, name, ocall, (160, 96, 96))
+    compare_buffers(b, g, rr, "%s ps=%d" % (name, ps))
+    assert (gd["use_dA"] == rd["use_dA"]).all()

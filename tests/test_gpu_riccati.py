@@ -1,0 +1,54 @@
+"""GPU parity of the fused backward Riccati factorisation against the CPU
+oracle's restatement of d_back_ric_trf_libstr
+(examples/example_d_riccati_recursion.c:115-140), plus the closed form
+L_k = blkdiag(I, sqrt(N-k+1) I) for A = I, B = 0, Q = R = I."""
+import numpy as np
+import pytest
+
+import oracle
+from blasfeo_amd.riccati import RiccatiBatch, DeviceRiccati, HostRiccati
+from helpers import assert_close
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_riccati_random(gpu, ps):
+    import torch
+    rng = np.random.default_rng(600 + ps)
+    nxs = list(rng.integers(1, 65, 14)) + [64, 1]
+    nus = list(rng.integers(1, 17, 14)) + [16, 1]
+    rb = RiccatiBatch(nxs, nus, N=6, ps=ps)
+    inp = rb.fill_inputs(rng)
+    dev = DeviceRiccati(rb, inp)
+    dev.run()
+    torch.cuda.synchronize()
+    host = HostRiccati(rb, inp)
+    oracle.lib().or_dric_trf_batch(ps, rb.batch, host.argv.ctypes.data)
+    gbuf = dev.outbuf.cpu().numpy()
+    for i in range(rb.batch):
+        for k in range(rb.N + 1):
+            g = rb.out.get(gbuf, rb.i_L[i][k])
+            r = host.L(i, k)
+            n = g.shape[0]
+            assert_close(np.tril(g), np.tril(r), 1e-11, "instance %d stage %d" % (i, k))
+            gd = rb.out.dview(gbuf, rb.i_L[i][k])
+            rd = rb.out.dview(host.outbuf, rb.i_L[i][k])
+            assert_close(gd, rd, 1e-11, "instance %d stage %d dA" % (i, k))
+
+
+def test_riccati_closed_form(gpu):
+    import torch
+    N = 7
+    nxs, nus = [5, 20, 33], [2, 3, 9]
+    rb = RiccatiBatch(nxs, nus, N=N, ps=4)
+    inp = rb.fill_custom(lambda i, nx, nu: (np.vstack([np.zeros((nu, nx)), np.eye(nx)]),
+                                            np.eye(nu + nx), np.eye(nx)))
+    dev = DeviceRiccati(rb, inp)
+    dev.run()
+    torch.cuda.synchronize()
+    for i, (nx, nu) in enumerate(zip(nxs, nus)):
+        for k in range(N):
+            expect = np.diag(np.r_[np.ones(nu), np.full(nx, np.sqrt(N - k + 1))])
+            np.testing.assert_allclose(np.tril(dev.L(i, k)), expect, rtol=1e-15, atol=1e-15)
+        np.testing.assert_allclose(np.tril(dev.L(i, N)), np.eye(nx), rtol=0, atol=0)

@@ -1,0 +1,255 @@
+#!/usr/bin/env python
+"""Benchmark of the batched BLASFEO hot path on one MI355X.
+
+Workload (default): the batched backward Riccati factorisation of
+examples/example_d_riccati_recursion.c:115-140 -- 8192 independent
+instances, horizon N = 20, nx ~ U{8..64}, nu ~ U{2..16} per instance,
+A = I + 0.1 N(0,1), B ~ N(0,1), Q = R = I, FP64, panel size 4, all inputs
+resident in GPU memory.  One step = one launch of the fused sweep over the
+whole batch.  ``value`` is solves (complete factorisations) per second.
+
+The JSON line also carries
+  roofline:     the sweep kernel's achieved FP64 rate (algorithmic flops per
+                launch / average launch time from HIP events on the launch
+                stream) against MI355X's 78.6 TFLOP/s FP64 peak, with the HBM
+                traffic per launch measured by rocprofv3 (profiles/hbm_traffic.txt);
+  cpu_baseline: the CPU oracle (a C restatement of the reference's REFERENCE
+                backend, OpenMP over instances) timed on this host on a bounded
+                sample of the same instances;
+  checks:       backward check of a sample of instances against the oracle.
+
+Flop conventions are the reference's (benchmarks/x_benchmark_blasfeo_api.c:557-566):
+trmm m*n^2, syrk m(m+1)k, potrf n^3/3.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/sec + %FP64-MFMA peak, batched potrf/Riccati; mixed vs FP64 at equal error"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def riccati_flops(nxs, nus, N):
+    nxs = np.asarray(nxs, dtype=np.float64)
+    nus = np.asarray(nus, dtype=np.float64)
+    nk = nxs + nus
+    stage = nk * nxs ** 2 + nk * (nk + 1) * nxs + nk ** 3 / 3.0
+    return float((N * stage + nxs ** 3 / 3.0).sum())
+
+
+def riccati_hbm_bytes(nxs, nus, N):
+    """Algorithmic HBM bytes of one launch: time-invariant inputs read once per
+    instance (BAbt, lower RSQ, lower Q_N) and every factor written once (lower
+    triangle + reciprocal diagonal), 8 bytes each."""
+    nxs = np.asarray(nxs, dtype=np.float64)
+    nus = np.asarray(nus, dtype=np.float64)
+    nk = nxs + nus
+    rd = nk * nxs + nk * (nk + 1) / 2 + nxs * (nxs + 1) / 2
+    wr = N * (nk * (nk + 1) / 2 + nk) + nxs * (nxs + 1) / 2 + nxs
+    return float(8 * (rd + wr).sum())
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch from profiles/hbm_traffic.txt (rocprofv3 FETCH_SIZE
+    and WRITE_SIZE runs, scaled by a copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 3 and f[0] == kernel and f[1] == workload:
+            for tok in f[2:]:
+                if tok.startswith("bytes_per_launch="):
+                    return float(tok.split("=")[1])
+    return None
+
+
+def riccati_checks(rb, inp, gbuf, host, nchk):
+    """Per-stage backward error of the GPU factors, ||M_k - L_k L_k^T||_F /
+    (||M_k||_F n_k eps) with M_k = RSQ + W W^T, W = BAbt L_{k+1}(nu:, nu:)
+    rebuilt from the GPU's own L_{k+1} (the recursion amplifies forward
+    differences, so each stage is judged on its own inputs), and the forward
+    difference of L_0 against the oracle's."""
+    eps = np.finfo(np.float64).eps
+    worst = 0.0
+    fdiff = 0.0
+    for i in range(nchk):
+        nx, nu = int(rb.nxs[i]), int(rb.nus[i])
+        babt = rb.inp.get(inp, rb.i_babt[i])
+        rsq = np.tril(rb.inp.get(inp, rb.i_rsq[i]))
+        rsq = rsq + np.tril(rsq, -1).T
+        qn = np.tril(rb.inp.get(inp, rb.i_qn[i]))
+        qn = qn + np.tril(qn, -1).T
+        L = [np.tril(rb.out.get(gbuf, rb.i_L[i][k])) for k in range(rb.N + 1)]
+        e = np.linalg.norm(qn - L[rb.N] @ L[rb.N].T) / (np.linalg.norm(qn) * nx * eps)
+        worst = max(worst, e)
+        for k in range(rb.N):
+            nu1 = nu if k + 1 < rb.N else 0
+            Lxx = L[k + 1][nu1:, nu1:]
+            W = babt @ Lxx
+            M = rsq + W @ W.T
+            nk = nx + nu
+            e = np.linalg.norm(M - L[k] @ L[k].T) / (np.linalg.norm(M) * nk * eps)
+            worst = max(worst, e)
+        r0 = np.tril(host.L(i, 0))
+        fdiff = max(fdiff, float(np.abs(L[0] - r0).max() / np.abs(r0).max()))
+    return float(worst), fdiff
+
+
+def cpu_baseline_riccati(rb, inp, seconds, max_inst):
+    import oracle
+    from blasfeo_amd.riccati import RiccatiBatch, HostRiccati
+    n = min(max_inst, rb.batch)
+    sub = RiccatiBatch(rb.nxs[:n], rb.nus[:n], rb.N, rb.ps)
+    # same instances: copy the first n instances' input bytes
+    buf = sub.inp.host_buffer()
+    for i in range(n):
+        for a, b in ((sub.i_babt[i], rb.i_babt[i]), (sub.i_rsq[i], rb.i_rsq[i]), (sub.i_qn[i], rb.i_qn[i])):
+            sub.inp.view(buf, a)[:] = rb.inp.view(inp, b)
+    h = HostRiccati(sub, buf)
+    L = oracle.lib()
+    threads = L.or_num_threads()
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        L.or_dric_trf_batch(rb.ps, n, h.argv.ctypes.data)
+        done += n
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": done / el, "unit": "solves/s", "cores": int(threads), "kind": "port",
+            "sample": "%d of the %d instances (same seeded data), repeated %d times, %.1f s; "
+                      "CPU oracle (C restatement of blasfeo_ref, -O2, OpenMP over instances)"
+                      % (n, rb.batch, done // n, el)}, h
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--horizon", type=int, default=20)
+    ap.add_argument("--ps", type=int, default=4, choices=(4, 8))
+    ap.add_argument("--seed", type=int, default=0xB1A5FE0 + 3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-max-instances", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="only run a device-to-device copy of this size (rocprof counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the batched instances are independent and the "
+              "run uses a single MI355X; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    import blasfeo_amd as bfa
+    from blasfeo_amd.riccati import RiccatiBatch, DeviceRiccati
+    bfa.lib()
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied %d bytes x3" % n)
+        return
+
+    rng = np.random.default_rng(args.seed)
+    nxs = rng.integers(8, 65, args.batch)
+    nus = rng.integers(2, 17, args.batch)
+    log("building %d Riccati instances (N=%d)" % (args.batch, args.horizon))
+    rb = RiccatiBatch(nxs, nus, args.horizon, args.ps)
+    inp = rb.fill_inputs(rng)
+    dev = DeviceRiccati(rb, inp)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    flops = riccati_flops(nxs, nus, args.horizon)
+    algo_bytes = riccati_hbm_bytes(nxs, nus, args.horizon)
+
+    log("warmup %d" % args.warmup)
+    with torch.cuda.stream(stream):
+        for _ in range(args.warmup):
+            dev.run(sh)
+    torch.cuda.synchronize()
+
+    log("timing %d steps" % args.steps)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        dev.run(sh)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1) / args.steps
+    solves = args.batch / (ms * 1e-3)
+    achieved = flops / (ms * 1e-3) / 1e12
+    log("%.3f ms/step, %.0f solves/s, %.2f TFLOP/s (wall %.3f s)" % (ms, solves, achieved, wall))
+
+    # correctness of the measured results: a sample against the oracle
+    checks = {}
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (%.0f s budget)" % args.cpu_seconds)
+        cpu, host = cpu_baseline_riccati(rb, inp, args.cpu_seconds, args.cpu_max_instances)
+        gbuf = dev.outbuf.cpu().numpy()
+        nchk = min(64, host.rb.batch)
+        berr, fdiff = riccati_checks(rb, inp, gbuf, host, nchk)
+        checks = {"instances": nchk,
+                  "max_stage_backward_error_over_n_eps": berr,
+                  "max_rel_diff_vs_oracle_L0": fdiff,
+                  "pass": berr <= 10.0}
+        log("checked %d instances: stage backward error <= %.2f n eps, L_0 rel diff vs oracle %.1e"
+            % (nchk, berr, fdiff))
+
+    traffic = read_traffic("k_ric_trf", "riccati-%dx%d" % (args.batch, args.horizon))
+    out = {
+        "metric": METRIC,
+        "value": solves,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic, seeded (numpy PCG64 seed %d): A = I + 0.1 N(0,1), B ~ N(0,1), Q = R = I" % args.seed,
+        "config": {"workload": "riccati_trf_batched", "batch": args.batch, "horizon": args.horizon,
+                   "nx": "U{8..64}", "nu": "U{2..16}", "ps": args.ps, "precision": "fp64",
+                   "parallelism": "one workgroup per instance, 1 GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "algorithmic_bytes": algo_bytes, "flops_per_launch": flops,
+                     "kernel": "k_ric_trf"},
+        "cpu_baseline": cpu,
+        "checks": checks,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -153,3 +153,111 @@ __device__ void chol_instance(T *P, int LDP, T *L11, T *invs,
 }
 
 } // namespace bfa
+
+namespace bfa {
+
+// ---------------------------------------------------------------------------
+// In-LDS right-looking blocked Cholesky of an n x n matrix held column-major
+// in LDS (M[c*ld + r], lower triangle used, n <= 128), in place: M <- L,
+// invd[j] = 1/sqrt(pivot) with the zero-pivot rule above (L_jj = c*invd[j]).
+// Rows and columns n .. ceil16(n)-1 must hold zeros on entry.
+// Per 16-column block: wave 0 factors the whole panel (diagonal block and the
+// rows below, two rows per lane) in registers, broadcasting L(jj,j) with
+// v_readlane, so the panel needs no workgroup barrier; then all waves apply
+// the rank-16 trailing update to the lower tiles on the matrix cores.
+// Called by all NT threads of the workgroup; starts and ends with a barrier.
+// ---------------------------------------------------------------------------
+template <typename T, int NT = NTL>
+__device__ void chol_lds(T *M, int ld, int n, T *invd)
+{
+	typedef typename Vec4<T>::type V4;
+	const int lane = lane_id();
+	const int w = wave_id();
+	const int nt = (n + 15) / 16;
+	__syncthreads();
+	for (int jb = 0; jb < nt; jb++)
+	{
+		const int c0 = jb * 16;
+		const int cw = min(16, n - c0);
+		const int nr = n - c0;
+		if (w == 0)
+		{
+			T x0[16], x1[16];
+			const bool r0ok = lane < nr, r1ok = lane + 64 < nr;
+#pragma unroll
+			for (int j = 0; j < 16; j++)
+			{
+				x0[j] = r0ok ? M[(c0 + j) * ld + c0 + lane] : T(0);
+				x1[j] = r1ok ? M[(c0 + j) * ld + c0 + 64 + lane] : T(0);
+			}
+#pragma unroll
+			for (int j = 0; j < 16; j++)
+			{
+				if (j < cw)
+				{
+					const T c = readlane(x0[j], j);
+					const T inv = (c > T(0)) ? rsqrt_ref<T>(c) : T(0);
+					if (lane > j)
+						x0[j] *= inv;
+					else if (lane == j)
+						x0[j] = c * inv;
+					x1[j] *= inv;
+					if (lane == 0)
+						invd[c0 + j] = inv;
+#pragma unroll
+					for (int jj = j + 1; jj < 16; jj++)
+					{
+						if (jj < cw)
+						{
+							const T l = readlane(x0[j], jj);
+							if (lane >= jj)
+								x0[jj] -= x0[j] * l;
+							x1[jj] -= x1[j] * l;
+						}
+					}
+				}
+			}
+#pragma unroll
+			for (int j = 0; j < 16; j++)
+			{
+				if (r0ok)
+					M[(c0 + j) * ld + c0 + lane] = x0[j];
+				if (r1ok)
+					M[(c0 + j) * ld + c0 + 64 + lane] = x1[j];
+			}
+		}
+		__syncthreads();
+		// trailing update of the lower tiles (bi, bj), jb < bj <= bi < nt
+		const int ntr = nt - jb - 1;
+		const int ntiles = ntr * (ntr + 1) / 2;
+		for (int t = w; t < ntiles; t += NT / WAVE)
+		{
+			int bj = 0, rem = t;
+			while (rem >= ntr - bj)
+			{
+				rem -= ntr - bj;
+				bj++;
+			}
+			const int bi = jb + 1 + bj + rem;
+			bj += jb + 1;
+			V4 acc;
+#pragma unroll
+			for (int r = 0; r < 4; r++)
+				acc[r] = M[(bj * 16 + (lane & 15)) * ld + bi * 16 + acc_row<T>(lane, r)];
+#pragma unroll
+			for (int kk = 0; kk < 16; kk += 4)
+			{
+				const int kq = c0 + kk + (lane >> 4);
+				const T a = M[kq * ld + bi * 16 + (lane & 15)];
+				const T b = M[kq * ld + bj * 16 + (lane & 15)];
+				acc = mfma16(-a, b, acc);
+			}
+#pragma unroll
+			for (int r = 0; r < 4; r++)
+				M[(bj * 16 + (lane & 15)) * ld + bi * 16 + acc_row<T>(lane, r)] = acc[r];
+		}
+		__syncthreads();
+	}
+}
+
+} // namespace bfa

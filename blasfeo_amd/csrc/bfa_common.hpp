@@ -85,10 +85,11 @@ __device__ __forceinline__ PM<T, PS> view(const typename MatOf<T>::type *s)
 
 // Cooperative copy of the R x C block at absolute (r0, c0) of a panel-major
 // matrix into LDS, element (i,c) stored at lds[c*ld + i] (TRANS=false) or
-// lds[i*ld + c] (TRANS=true).  Elements with i >= Rv or c >= Cv are written as
-// zero and never read from memory.  Threads walk the covering panels in
+// lds[i*ld + c] (TRANS=true).  Elements with i >= Rv or c >= Cv (and, with
+// LOWER, those above the diagonal, i < c) are written as zero and never read
+// from memory.  Threads walk the covering panels in
 // memory order, so a wave reads contiguous runs of C*PS elements.
-template <typename T, int PS, int NT, bool TRANS>
+template <typename T, int PS, int NT, bool TRANS, bool LOWER = false>
 __device__ __forceinline__ void g2s(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, T *lds, int ld)
 {
 	const int rr = r0 & (PS - 1);
@@ -106,7 +107,7 @@ __device__ __forceinline__ void g2s(const PM<T, PS> g, int r0, int c0, int R, in
 		if (i < 0 || i >= R)
 			continue;
 		T v = T(0);
-		if (i < Rv && c < Cv)
+		if (i < Rv && c < Cv && (!LOWER || i >= c))
 			v = base[(size_t)p * PS * g.cn + (size_t)c * PS + q];
 		if (TRANS)
 			lds[i * ld + c] = v;

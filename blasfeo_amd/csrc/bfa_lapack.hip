@@ -48,6 +48,31 @@ __global__ __launch_bounds__(NTL) void k_potrf(const typename FactArg<T>::type *
 		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? (MODE == 0 ? m : 1) : 0;
 }
 
+// potrf_l with the whole matrix in LDS (n <= 128): stage the lower triangle
+// of C, factor in place (chol_lds), stream L and dA out.  NT = 64 (one wave)
+// for n <= 32 so that several instances share a CU.
+template <typename T, int PS, int NT>
+__global__ __launch_bounds__(NT) void k_potrf_lds(const typename FactArg<T>::type *__restrict__ args, int np)
+{
+	extern __shared__ unsigned char smem_raw[];
+	const int LDM = np + 1;
+	T *M = reinterpret_cast<T *>(smem_raw);
+	T *invd = M + (size_t)np * LDM;
+	const typename FactArg<T>::type &a = args[blockIdx.x];
+	const int m = a.m;
+	if (m <= 0 || m > np)
+		return;
+	const int mt = (m + 15) / 16;
+	g2s<T, PS, NT, false, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, mt * 16, m, m, M, LDM);
+	chol_lds<T, NT>(M, LDM, m, invd);
+	s2g<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, m, M, LDM, 0);
+	T *dD = a.sD->dA;
+	for (int j = threadIdx.x; j < m; j += NT)
+		dD[j] = invd[j];
+	if (threadIdx.x == 0)
+		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? m : 0;
+}
+
 template <typename T> struct SPArg;
 template <> struct SPArg<double> { typedef blasfeo_amd_dsyrk_dpotrf_arg type; };
 
@@ -291,6 +316,30 @@ static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args
 	int mmax;
 	if ((rc = fact_mmax_arg<T>(hint, name, &mmax)))
 		return rc;
+	if (MODE == 0 && mmax <= 128)
+	{
+		const int np = (mmax + 15) / 16 * 16;
+		const size_t shm2 = sizeof(T) * ((size_t)np * (np + 1) + np);
+		if (np <= 32)
+		{
+			if (ps == 4)
+				hipLaunchKernelGGL((k_potrf_lds<T, 4, 64>), dim3(batch), dim3(64), shm2, st, args, np);
+			else if (ps == 8)
+				hipLaunchKernelGGL((k_potrf_lds<T, 8, 64>), dim3(batch), dim3(64), shm2, st, args, np);
+			else
+				hipLaunchKernelGGL((k_potrf_lds<T, 16, 64>), dim3(batch), dim3(64), shm2, st, args, np);
+		}
+		else
+		{
+			if (ps == 4)
+				hipLaunchKernelGGL((k_potrf_lds<T, 4, NTL>), dim3(batch), dim3(NTL), shm2, st, args, np);
+			else if (ps == 8)
+				hipLaunchKernelGGL((k_potrf_lds<T, 8, NTL>), dim3(batch), dim3(NTL), shm2, st, args, np);
+			else
+				hipLaunchKernelGGL((k_potrf_lds<T, 16, NTL>), dim3(batch), dim3(NTL), shm2, st, args, np);
+		}
+		return check_launch(name);
+	}
 	const size_t shm = sizeof(T) * PW * ((mmax + 15) / 16 * 16 + 1);
 	if (ps == 4)
 		hipLaunchKernelGGL((k_potrf<T, 4, MODE>), dim3(batch), dim3(NTL), shm, st, args, mmax);

@@ -37,6 +37,7 @@ class RiccatiBatch:
             self.i_L.append(Ls)
             self.i_work.append(self.out.add(nu + nx, nx))
         self.max_nk = int((self.nxs + self.nus).max()) if self.batch else 0
+        self.max_nx = int(self.nxs.max()) if self.batch else 0
 
     # -- data ---------------------------------------------------------------
     def fill_inputs(self, rng, A_scale=0.1, R=None, Q=None):
@@ -120,7 +121,7 @@ class DeviceRiccati:
     def run(self, stream=0):
         from . import call
         call("d_back_ric_trf", self.rb.ps, self.rb.batch, self.args.data_ptr(),
-             (self.rb.max_nk, 0, 0), stream)
+             (self.rb.max_nk, self.rb.max_nx, 0), stream)
 
     def L(self, i, k):
         buf = self.outbuf.cpu().numpy()

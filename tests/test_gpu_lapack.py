@@ -20,6 +20,36 @@ def _spd(rng, n):
 
 @pytest.mark.parametrize("prec", ["d", "s"])
 @pytest.mark.parametrize("ps", [4, 8])
+@pytest.mark.parametrize("nmax", [32, 128])
+def test_potrf_lds(gpu, prec, ps, nmax):
+    """potrf_l with the whole matrix in LDS (max size <= 128; one wave per
+    instance when <= 32)."""
+    rng = np.random.default_rng(350 + ps + nmax)
+    b = Batch(prec, ps, seed=7)
+    recs = []
+    sizes = [1, 2, 5, 16, 17, 31, 32] + ([33, 48, 63, 64, 65, 100, 127, 128] if nmax == 128 else [])
+    for r, n in enumerate(sizes):
+        ci, cj = (0, 0) if r % 2 == 0 else (int(rng.integers(0, 2 * ps)), int(rng.integers(0, 3)))
+        di, dj = (0, 0) if r % 3 == 0 else (int(rng.integers(0, 2 * ps)), int(rng.integers(0, 3)))
+        S = np.zeros((ci + n, cj + n))
+        S[ci:, cj:] = _spd(rng, n)
+        C = b.mat(ci + n, cj + n, "data", S)
+        D = b.mat(di + n, dj + n, "const", -1.0)
+        recs.append(dict(m=n, n=n, ci=ci, cj=cj, di=di, dj=dj, sC=C, sD=D, ipiv=0))
+    buf = b.host()
+    f = oracle.fn(prec, "potrf_l")
+
+    def ocall(h, r):
+        f(ps, r["m"], h.desc_addr(r["sC"]), r["ci"], r["cj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    tol = 1e-11 if prec == "d" else 2e-3
+    g, gd, rr, rd = run_both(b, buf, bfa.DFACT_ARG, recs, prec + "potrf_l", ocall, (nmax, nmax, 0))
+    compare_buffers(b, g, rr, "%spotrf_l lds ps=%d" % (prec, ps), tol=tol)
+    assert (gd["use_dA"] == rd["use_dA"]).all()
+
+
+@pytest.mark.parametrize("prec", ["d", "s"])
+@pytest.mark.parametrize("ps", [4, 8])
 @pytest.mark.parametrize("op", ["potrf_l", "potrf_l_mn"])
 def test_potrf(gpu, prec, ps, op):
     if prec == "s" and op == "potrf_l_mn":

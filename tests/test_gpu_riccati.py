@@ -12,12 +12,17 @@ from helpers import assert_close
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("form", ["lds", "general"])
 @pytest.mark.parametrize("ps", [4, 8])
-def test_riccati_random(gpu, ps):
+def test_riccati_random(gpu, ps, form):
+    """'lds': nx <= 64, nu + nx <= 128 (LDS-resident sweep); 'general': one
+    instance with nx = 70 selects the global-memory sweep."""
     import torch
     rng = np.random.default_rng(600 + ps)
     nxs = list(rng.integers(1, 65, 14)) + [64, 1]
     nus = list(rng.integers(1, 17, 14)) + [16, 1]
+    if form == "general":
+        nxs[3], nus[3] = 70, 5
     rb = RiccatiBatch(nxs, nus, N=6, ps=ps)
     inp = rb.fill_inputs(rng)
     dev = DeviceRiccati(rb, inp)

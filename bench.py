@@ -147,6 +147,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-max-instances", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="riccati",
+                    choices=("riccati", "potrf32", "gemm_syrk", "getrf", "sgetrf"),
+                    help="riccati (default, the headline); the others measure the remaining "
+                         "batched configurations")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only run a device-to-device copy of this size (rocprof counter calibration)")
     args = ap.parse_args()
@@ -161,7 +165,6 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     import blasfeo_amd as bfa
-    from blasfeo_amd.riccati import RiccatiBatch, DeviceRiccati
     bfa.lib()
 
     if args.calibrate_copy:
@@ -174,6 +177,13 @@ def main():
         log("copied %d bytes x3" % n)
         return
 
+    if args.workload != "riccati":
+        return run_other(args, torch, bfa)
+    return run_riccati(args, torch, bfa)
+
+
+def run_riccati(args, torch, bfa):
+    from blasfeo_amd.riccati import RiccatiBatch, DeviceRiccati
     rng = np.random.default_rng(args.seed)
     nxs = rng.integers(8, 65, args.batch)
     nus = rng.integers(2, 17, args.batch)
@@ -224,7 +234,7 @@ def main():
         log("checked %d instances: stage backward error <= %.2f n eps, L_0 rel diff vs oracle %.1e"
             % (nchk, berr, fdiff))
 
-    traffic = read_traffic("k_ric_trf", "riccati-%dx%d" % (args.batch, args.horizon))
+    traffic = read_traffic("k_ric_class1", "riccati-%dx%d" % (args.batch, args.horizon))
     out = {
         "metric": METRIC,
         "value": solves,
@@ -244,11 +254,228 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "algorithmic_bytes": algo_bytes, "flops_per_launch": flops,
-                     "kernel": "k_ric_trf"},
+                     "kernel": "k_ric_class1 (+ k_ric_class for the largest instances)"},
         "cpu_baseline": cpu,
         "checks": checks,
     }
     print(json.dumps(out), flush=True)
+
+
+
+# ---------------------------------------------------------------------------
+# the other batched configurations
+# ---------------------------------------------------------------------------
+def time_steps(torch, stream, fn, steps, warmup):
+    """Average milliseconds per step from HIP events on the launch stream."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def _records(dtype, n):
+    return np.zeros(n, dtype=dtype)
+
+
+def _fill(torch, buf, kind, gen):
+    """Random fill of a whole device byte buffer viewed as doubles."""
+    v = buf.view(torch.float64)
+    chunk = 1 << 27
+    for s0 in range(0, v.numel(), chunk):
+        part = v[s0:s0 + chunk]
+        if kind == "normal":
+            part.normal_(generator=gen)
+        else:
+            part.uniform_(-1.0, 1.0, generator=gen)
+
+
+def run_other(args, torch, bfa):
+    from blasfeo_amd.device import to_device
+    import oracle
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(args.seed)
+    rng = np.random.default_rng(args.seed)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    ps = args.ps
+    cpu = None
+    checks = {}
+
+    def dbuf(nbytes):
+        return torch.empty(max(nbytes, 8), dtype=torch.uint8, device="cuda")
+
+    if args.workload == "potrf32":
+        # 4096 x A = B B^T + 32 I, B ~ U[-1,1], n = 32 (the reference's CPU-runnable case)
+        batch, n = 4096, 32
+        dC, offC, nbC = bfa.bulk_layout(np.float64, ps, [n] * batch, [n] * batch, 0)
+        bufC = dbuf(nbC)
+        bufD = dbuf(nbC)
+        B = torch.rand((batch, n, n), dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+        A = torch.bmm(B, B.transpose(1, 2).contiguous()) + n * torch.eye(n, dtype=torch.float64, device="cuda")
+        stride = int(offC[1]) // 8
+        pm = n
+        packed = A.view(batch, pm // ps, ps, n).permute(0, 1, 3, 2).reshape(batch, pm * n)
+        bufC.view(torch.float64).view(batch, stride)[:, : pm * n] = packed
+        descC = to_device(bfa.bulk_layout(np.float64, ps, [n] * batch, [n] * batch, bufC.data_ptr())[0])
+        descD = to_device(bfa.bulk_layout(np.float64, ps, [n] * batch, [n] * batch, bufD.data_ptr())[0])
+        rec = _records(bfa.DFACT_ARG, batch)
+        rec["m"] = n
+        rec["n"] = n
+        rec["sC"] = descC.data_ptr() + np.arange(batch, dtype=np.uint64) * bfa.DMAT.itemsize
+        rec["sD"] = descD.data_ptr() + np.arange(batch, dtype=np.uint64) * bfa.DMAT.itemsize
+        drec = to_device(rec)
+        fn = lambda: bfa.call("dpotrf_l", ps, batch, drec.data_ptr(), (n, n, 0), sh)
+        ms = time_steps(torch, stream, fn, args.steps, args.warmup)
+        flops = batch * n ** 3 / 3.0
+        algo = batch * 8704.0
+        value = batch / (ms * 1e-3)
+        unit = "factorizations/s"
+        bound, peak, punit = "hbm", HBM_PEAK_GBS, "GB/s"
+        achieved = algo / (ms * 1e-3) / 1e9
+        kernel = "k_potrf_lds<double,%d,64>" % ps
+        wl = {"workload": "dpotrf_l_batched", "batch": batch, "n": n, "ps": ps, "precision": "fp64"}
+        # check: backward error of every factor
+        L = bufD.view(torch.float64).view(batch, stride)[:, : pm * n].view(batch, pm // ps, n, ps)
+        L = L.permute(0, 1, 3, 2).reshape(batch, n, n).tril()
+        berr = ((A - torch.bmm(L, L.transpose(1, 2).contiguous())).flatten(1).norm(dim=1)
+                / A.flatten(1).norm(dim=1)).max().item()
+        checks = {"max_backward_error": berr, "bound_10_n_eps": 10 * n * 2.22e-16,
+                  "pass": berr <= 10 * n * 2.22e-16}
+        if not args.no_cpu:
+            hb = bufC.cpu().numpy()
+            hd = np.zeros_like(hb)
+            hC = bfa.bulk_layout(np.float64, ps, [n] * batch, [n] * batch, hb.ctypes.data)[0]
+            hD = bfa.bulk_layout(np.float64, ps, [n] * batch, [n] * batch, hd.ctypes.data)[0]
+            hr = _records(bfa.DFACT_ARG, batch)
+            hr["m"] = n
+            hr["n"] = n
+            hr["sC"] = hC.ctypes.data + np.arange(batch, dtype=np.uint64) * bfa.DMAT.itemsize
+            hr["sD"] = hD.ctypes.data + np.arange(batch, dtype=np.uint64) * bfa.DMAT.itemsize
+            cpu = cpu_time(lambda: oracle.lib().or_dpotrf_l_batch(ps, batch, hr.ctypes.data), batch,
+                           args.cpu_seconds, "factorizations/s",
+                           "all %d instances per repetition" % batch)
+    elif args.workload == "gemm_syrk":
+        # 100000 dgemm_nt + 100000 dsyrk_ln, m, n, k ~ U{4..128}, N(0,1), alpha = beta = 1, D = C
+        batch = 100000
+        m = rng.integers(4, 129, batch); n = rng.integers(4, 129, batch); k = rng.integers(4, 129, batch)
+        ms_ = rng.integers(4, 129, batch); ks_ = rng.integers(4, 129, batch)
+        lay = {}
+        for name, (r, c) in {"gA": (m, k), "gB": (n, k), "gC": (m, n), "sA": (ms_, ks_), "sC": (ms_, ms_)}.items():
+            d, off, nb = bfa.bulk_layout(np.float64, ps, r, c, 0)
+            buf = dbuf(nb)
+            _fill(torch, buf, "normal", gen)
+            d = bfa.bulk_layout(np.float64, ps, r, c, buf.data_ptr())[0]
+            lay[name] = (buf, to_device(d))
+        idx = np.arange(batch, dtype=np.uint64) * bfa.DMAT.itemsize
+        g = _records(bfa.DGEMM_ARG, batch)
+        g["m"], g["n"], g["k"] = m, n, k
+        g["alpha"], g["beta"] = 1.0, 1.0
+        g["sA"] = lay["gA"][1].data_ptr() + idx
+        g["sB"] = lay["gB"][1].data_ptr() + idx
+        g["sC"] = lay["gC"][1].data_ptr() + idx
+        g["sD"] = g["sC"]
+        sy = _records(bfa.DGEMM_ARG, batch)
+        sy["m"], sy["n"], sy["k"] = ms_, ms_, ks_
+        sy["alpha"], sy["beta"] = 1.0, 1.0
+        sy["sA"] = lay["sA"][1].data_ptr() + idx
+        sy["sB"] = sy["sA"]
+        sy["sC"] = lay["sC"][1].data_ptr() + idx
+        sy["sD"] = sy["sC"]
+        dg, ds = to_device(g), to_device(sy)
+
+        def fn():
+            bfa.call("dgemm_nt", ps, batch, dg.data_ptr(), (128, 128, 128), sh)
+            bfa.call("dsyrk_ln", ps, batch, ds.data_ptr(), (128, 128, 128), sh)
+        ms = time_steps(torch, stream, fn, args.steps, args.warmup)
+        mf, nf, kf, sm, sk = (x.astype(np.float64) for x in (m, n, k, ms_, ks_))
+        flops = float((2 * mf * nf * kf).sum() + (sm * (sm + 1) * sk).sum())
+        algo = float(8 * ((mf * kf + nf * kf + 2 * mf * nf).sum() + (sm * sk + sm * (sm + 1)).sum()))
+        value = 2 * batch / (ms * 1e-3)
+        unit = "ops/s (dgemm_nt + dsyrk_ln calls)"
+        bound, peak, punit = "hbm", HBM_PEAK_GBS, "GB/s"
+        achieved = algo / (ms * 1e-3) / 1e9
+        kernel = "k_gemm<double,%d,OP_NT> + k_gemm<double,%d,OP_SYRK>" % (ps, ps)
+        wl = {"workload": "dgemm_nt+dsyrk_ln_batched", "batch": "100000 + 100000",
+              "m,n,k": "U{4..128}", "ps": ps, "precision": "fp64"}
+    else:
+        # 16384 x getrf_rp, n ~ U{16..256}, U[-1,1]
+        prec = "s" if args.workload == "sgetrf" else "d"
+        dt = np.float32 if prec == "s" else np.float64
+        batch = 16384
+        nn = rng.integers(16, 257, batch)
+        d, off, nb = bfa.bulk_layout(dt, ps, nn, nn, 0)
+        buf = dbuf(nb)
+        if prec == "s":
+            v = buf.view(torch.float32)
+            v.uniform_(-1.0, 1.0, generator=gen)
+        else:
+            _fill(torch, buf, "uniform", gen)
+        d = to_device(bfa.bulk_layout(dt, ps, nn, nn, buf.data_ptr())[0])
+        piv_off = np.concatenate([[0], np.cumsum(nn)[:-1]]).astype(np.uint64)
+        piv = torch.zeros(int(nn.sum()), dtype=torch.int32, device="cuda")
+        rec = _records(bfa.DFACT_ARG, batch)
+        rec["m"], rec["n"] = nn, nn
+        idx = np.arange(batch, dtype=np.uint64) * bfa.DMAT.itemsize
+        rec["sC"] = d.data_ptr() + idx
+        rec["sD"] = rec["sC"]
+        rec["ipiv"] = piv.data_ptr() + 4 * piv_off
+        drec = to_device(rec)
+        orig = buf.clone()
+        name = prec + "getrf_rp"
+
+        def fn():
+            buf.copy_(orig)
+            bfa.call(name, ps, batch, drec.data_ptr(), (256, 256, 0), sh)
+        # the in-place factorisation needs its input restored every step: time the
+        # restore copy alone and subtract it
+        ms_total = time_steps(torch, stream, fn, args.steps, args.warmup)
+        ms_copy = time_steps(torch, stream, lambda: buf.copy_(orig), args.steps, 1)
+        ms = ms_total - ms_copy
+        nf = nn.astype(np.float64)
+        flops = float((2.0 / 3.0 * nf ** 3).sum())
+        algo = float((np.dtype(dt).itemsize * 2 * nf ** 2 + 12 * nf).sum())
+        value = batch / (ms * 1e-3)
+        unit = "factorizations/s"
+        bound, punit = "mfma", "TFLOP/s"
+        peak = FP64_PEAK_TFLOPS if prec == "d" else 157.3
+        achieved = flops / (ms * 1e-3) / 1e12
+        kernel = "k_getrf<%s,%d>" % ("double" if prec == "d" else "float", ps)
+        wl = {"workload": "%s_batched" % name, "batch": batch, "n": "U{16..256}", "ps": ps,
+              "precision": "fp64" if prec == "d" else "fp32",
+              "timing": "step time minus a same-size device copy that restores the input"}
+    log("%s: %.3f ms/step, %.4g %s, %.2f %s" % (args.workload, ms, value, unit, achieved, punit))
+    out = {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32" if args.workload == "sgetrf" else "f64",
+           "data": "synthetic, seeded (%d)" % args.seed, "config": wl,
+           "roofline": {"bound": bound, "achieved": achieved, "peak": peak, "unit": punit,
+                        "frac": achieved / peak, "traffic": None, "algorithmic_bytes": algo,
+                        "flops_per_launch": flops, "kernel": kernel},
+           "cpu_baseline": cpu, "checks": checks}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_time(fn, units, seconds, unit, sample):
+    import oracle
+    threads = oracle.lib().or_num_threads()
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        fn()
+        done += units
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": done / el, "unit": unit, "cores": int(threads), "kind": "port",
+            "sample": "%s, %d repetitions, %.1f s; CPU oracle (C restatement of blasfeo_ref, OpenMP over instances)"
+                      % (sample, done // units, el)}
 
 
 if __name__ == "__main__":

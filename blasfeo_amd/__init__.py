@@ -240,3 +240,31 @@ class Layout:
         r = np.arange(m)[:, None] + ai
         c = np.arange(n)[None, :] + aj
         return flat[(r - r % ps) * cn + c * ps + r % ps].copy()
+
+
+def bulk_layout(dtype, ps, ms, ns, base, align=256):
+    """Vectorised layout of many matrices in one buffer starting at `base`.
+
+    Returns (descriptors as a DMAT array, offsets in bytes, total bytes).  Same
+    rules as ``Layout`` (reference auxiliary/d_aux_lib4.c:1519-1570)."""
+    dtype = np.dtype(dtype)
+    ms = np.asarray(ms, dtype=np.int64)
+    ns = np.asarray(ns, dtype=np.int64)
+    nc = _pld(dtype, ps)
+    al = ps * nc
+    pm = -(-ms // ps) * ps
+    cn = -(-ns // nc) * nc
+    tmp = -(-np.minimum(ms, ns) // al) * al
+    size = (pm * cn + tmp) * dtype.itemsize
+    size = -(-size // 64) * 64
+    padded = -(-size // align) * align
+    offs = np.concatenate([[0], np.cumsum(padded)[:-1]]).astype(np.int64)
+    d = np.zeros(len(ms), dtype=DMAT)
+    addr = np.uint64(int(base)) + offs.astype(np.uint64)
+    d["mem"] = addr
+    d["pA"] = addr
+    d["dA"] = addr + (pm * cn * dtype.itemsize).astype(np.uint64)
+    d["m"], d["n"], d["pm"], d["cn"] = ms, ns, pm, cn
+    d["memsize"] = size
+    total = int(offs[-1] + padded[-1]) if len(ms) else 0
+    return d, offs, total

@@ -55,6 +55,7 @@ __device__ void chol_instance(T *P, int LDP, T *L11, T *invs,
 				const bool iok = ir < nr;
 				const bool jok = (lane & 15) < cw;
 				// + A(c0+i, :) * B(c0+j, :)^T
+#pragma unroll 1
 				for (int kk = 0; kk < k; kk += 4)
 				{
 					const int kq = kk + (lane >> 4);
@@ -64,6 +65,7 @@ __device__ void chol_instance(T *P, int LDP, T *L11, T *invs,
 					acc = mfma16(av, bv, acc);
 				}
 				// - L(c0+i, 0:c0) * L(c0+j, 0:c0)^T  (c0 is a multiple of 16)
+#pragma unroll 1
 				for (int kk = 0; kk < c0; kk += 4)
 				{
 					const int kq = kk + (lane >> 4);
@@ -123,8 +125,12 @@ __device__ void chol_instance(T *P, int LDP, T *L11, T *invs,
 		}
 		__syncthreads();
 		// rows 64.. of the panel: solve against L11 (x_j = (x_j - sum_l<j x_l L_jl) * inv_j)
+#pragma unroll 1
 		for (int il = 64 + threadIdx.x; il < nr; il += NTL)
 		{
+			// keep the L11 reads inside the loop: hoisted, the 136 loop-invariant
+			// values would pin ~270 registers for a loop of at most a few trips
+			asm volatile("" ::: "memory");
 			T x[PW];
 #pragma unroll
 			for (int j = 0; j < PW; j++)
@@ -163,14 +169,16 @@ namespace bfa {
 // Rows and columns n .. ceil16(n)-1 must hold zeros on entry.
 // Per 16-column block: wave 0 factors the whole panel (diagonal block and the
 // rows below, two rows per lane) in registers, broadcasting L(jj,j) with
-// v_readlane, so the panel needs no workgroup barrier; then all waves apply
-// the rank-16 trailing update to the lower tiles on the matrix cores.
+// v_readlane, so the panel needs no workgroup barrier; then the waves apply
+// the rank-16 trailing update to the lower tiles on the matrix cores, one
+// block column of tiles at a time with independent accumulators.
 // Called by all NT threads of the workgroup; starts and ends with a barrier.
 // ---------------------------------------------------------------------------
 template <typename T, int NT = NTL>
 __device__ void chol_lds(T *M, int ld, int n, T *invd)
 {
 	typedef typename Vec4<T>::type V4;
+	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
 	const int w = wave_id();
 	const int nt = (n + 15) / 16;
@@ -196,7 +204,7 @@ __device__ void chol_lds(T *M, int ld, int n, T *invd)
 				if (j < cw)
 				{
 					const T c = readlane(x0[j], j);
-					const T inv = (c > T(0)) ? rsqrt_ref<T>(c) : T(0);
+					const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
 					if (lane > j)
 						x0[j] *= inv;
 					else if (lane == j)
@@ -227,34 +235,54 @@ __device__ void chol_lds(T *M, int ld, int n, T *invd)
 			}
 		}
 		__syncthreads();
-		// trailing update of the lower tiles (bi, bj), jb < bj <= bi < nt
-		const int ntr = nt - jb - 1;
-		const int ntiles = ntr * (ntr + 1) / 2;
-		for (int t = w; t < ntiles; t += NT / WAVE)
+		// trailing update: tasks (block column bj > jb, group of up to 4 tiles
+		// bi = g..g+3 >= bj) spread over the waves, independent accumulators
 		{
-			int bj = 0, rem = t;
-			while (rem >= ntr - bj)
+			int ntask = 0;
+			for (int bj = jb + 1; bj < nt; bj++)
+				ntask += (nt - bj + 3) / 4;
+			for (int task = w; task < ntask; task += NW)
 			{
-				rem -= ntr - bj;
-				bj++;
+				int bj = jb + 1, rem = task;
+				while (rem >= (nt - bj + 3) / 4)
+				{
+					rem -= (nt - bj + 3) / 4;
+					bj++;
+				}
+				const int g = bj + 4 * rem;
+				V4 acc[4];
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+				{
+					if (g + t < nt)
+#pragma unroll
+						for (int r = 0; r < 4; r++)
+							acc[t][r] = M[(bj * 16 + (lane & 15)) * ld + (g + t) * 16 + acc_row<T>(lane, r)];
+				}
+#pragma unroll
+				for (int kk = 0; kk < 16; kk += 4)
+				{
+					const int kq = c0 + kk + (lane >> 4);
+					const T b = M[kq * ld + bj * 16 + (lane & 15)];
+#pragma unroll
+					for (int t = 0; t < 4; t++)
+					{
+						if (g + t < nt)
+						{
+							const T a = M[kq * ld + (g + t) * 16 + (lane & 15)];
+							acc[t] = mfma16(-a, b, acc[t]);
+						}
+					}
+				}
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+				{
+					if (g + t < nt)
+#pragma unroll
+						for (int r = 0; r < 4; r++)
+							M[(bj * 16 + (lane & 15)) * ld + (g + t) * 16 + acc_row<T>(lane, r)] = acc[t][r];
+				}
 			}
-			const int bi = jb + 1 + bj + rem;
-			bj += jb + 1;
-			V4 acc;
-#pragma unroll
-			for (int r = 0; r < 4; r++)
-				acc[r] = M[(bj * 16 + (lane & 15)) * ld + bi * 16 + acc_row<T>(lane, r)];
-#pragma unroll
-			for (int kk = 0; kk < 16; kk += 4)
-			{
-				const int kq = c0 + kk + (lane >> 4);
-				const T a = M[kq * ld + bi * 16 + (lane & 15)];
-				const T b = M[kq * ld + bj * 16 + (lane & 15)];
-				acc = mfma16(-a, b, acc);
-			}
-#pragma unroll
-			for (int r = 0; r < 4; r++)
-				M[(bj * 16 + (lane & 15)) * ld + bi * 16 + acc_row<T>(lane, r)] = acc[r];
 		}
 		__syncthreads();
 	}

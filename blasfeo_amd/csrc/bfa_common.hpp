@@ -87,32 +87,36 @@ __device__ __forceinline__ PM<T, PS> view(const typename MatOf<T>::type *s)
 // matrix into LDS, element (i,c) stored at lds[c*ld + i] (TRANS=false) or
 // lds[i*ld + c] (TRANS=true).  Elements with i >= Rv or c >= Cv (and, with
 // LOWER, those above the diagonal, i < c) are written as zero and never read
-// from memory.  Threads walk the covering panels in
-// memory order, so a wave reads contiguous runs of C*PS elements.
+// from memory.  The waves take the covering panels in turn and the lanes of a
+// wave walk one panel's contiguous run of C*PS elements, so a wave reads
+// contiguous memory and no index needs a runtime division.
 template <typename T, int PS, int NT, bool TRANS, bool LOWER = false>
 __device__ __forceinline__ void g2s(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, T *lds, int ld)
 {
+	constexpr int NW = NT / WAVE;
 	const int rr = r0 & (PS - 1);
 	const int np = (rr + R + PS - 1) / PS;
 	const int run = PS * C;
-	const int total = np * run;
 	const T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
-	for (int e = threadIdx.x; e < total; e += NT)
+	const int lane = threadIdx.x & (WAVE - 1);
+	for (int p = threadIdx.x / WAVE; p < np; p += NW)
 	{
-		const int p = e / run;
-		const int rem = e - p * run;
-		const int c = rem / PS;
-		const int q = rem - c * PS;
-		const int i = p * PS + q - rr;
-		if (i < 0 || i >= R)
-			continue;
-		T v = T(0);
-		if (i < Rv && c < Cv && (!LOWER || i >= c))
-			v = base[(size_t)p * PS * g.cn + (size_t)c * PS + q];
-		if (TRANS)
-			lds[i * ld + c] = v;
-		else
-			lds[c * ld + i] = v;
+		const T *src = base + (size_t)p * PS * g.cn;
+		for (int e = lane; e < run; e += WAVE)
+		{
+			const int c = (unsigned)e / PS;
+			const int q = e & (PS - 1);
+			const int i = p * PS + q - rr;
+			if (i < 0 || i >= R)
+				continue;
+			T v = T(0);
+			if (i < Rv && c < Cv && (!LOWER || i >= c))
+				v = src[e];
+			if (TRANS)
+				lds[i * ld + c] = v;
+			else
+				lds[c * ld + i] = v;
+		}
 	}
 }
 
@@ -121,23 +125,33 @@ __device__ __forceinline__ void g2s(const PM<T, PS> g, int r0, int c0, int R, in
 template <typename T, int PS, int NT, bool LOWER>
 __device__ __forceinline__ void s2g(PM<T, PS> g, int r0, int c0, int Rv, int Cv, const T *lds, int ld, int diag)
 {
+	constexpr int NW = NT / WAVE;
 	const int rr = r0 & (PS - 1);
 	const int np = (rr + Rv + PS - 1) / PS;
 	const int run = PS * Cv;
-	const int total = np * run;
 	T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
-	for (int e = threadIdx.x; e < total; e += NT)
+	const int lane = threadIdx.x & (WAVE - 1);
+	for (int p = threadIdx.x / WAVE; p < np; p += NW)
 	{
-		const int p = e / run;
-		const int rem = e - p * run;
-		const int c = rem / PS;
-		const int q = rem - c * PS;
-		const int i = p * PS + q - rr;
-		if (i < 0 || i >= Rv)
-			continue;
-		if (LOWER && i - c < diag)
-			continue;
-		base[(size_t)p * PS * g.cn + (size_t)c * PS + q] = lds[c * ld + i];
+		T *dst = base + (size_t)p * PS * g.cn;
+		// with LOWER, columns beyond the panel's last row are never written
+		int cend = run;
+		if (LOWER)
+		{
+			const int last = min(p * PS + PS - 1 - rr, Rv - 1);
+			cend = min(run, (last - diag + 1) * PS);
+		}
+		for (int e = lane; e < cend; e += WAVE)
+		{
+			const int c = (unsigned)e / PS;
+			const int q = e & (PS - 1);
+			const int i = p * PS + q - rr;
+			if (i < 0 || i >= Rv)
+				continue;
+			if (LOWER && i - c < diag)
+				continue;
+			dst[e] = lds[c * ld + i];
+		}
 	}
 }
 
@@ -164,6 +178,25 @@ __device__ __forceinline__ int wave_argmax_abs(T best, int idx)
 		}
 	}
 	return __builtin_amdgcn_readfirstlane(idx);
+}
+
+// 1/sqrt(c) for c > 0 from the hardware estimate refined by one Newton step
+// y <- y + y*(1/2 - c*y*y/2): within about one ulp of the correctly rounded
+// value, with a dependency chain of one transcendental and three FMAs (the
+// Cholesky panel's critical path).
+__device__ __forceinline__ double rsqrt_fast(double c)
+{
+	const double y = __builtin_amdgcn_rsq(c);
+	const double h = 0.5 * c * y;
+	const double r = __builtin_fma(-h, y, 0.5);
+	return __builtin_fma(y, r, y);
+}
+__device__ __forceinline__ float rsqrt_fast(float c)
+{
+	const float y = __builtin_amdgcn_rsqf(c);
+	const float h = 0.5f * c * y;
+	const float r = __builtin_fmaf(-h, y, 0.5f);
+	return __builtin_fmaf(y, r, y);
 }
 
 template <typename T> __device__ __forceinline__ T rsqrt_ref(T c);

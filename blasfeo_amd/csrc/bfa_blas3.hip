@@ -158,16 +158,169 @@ __global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::ty
 	}
 }
 
-// Cache update after trsm_rltn, in the reference's order
-// (blasfeo_hp_pm/d_blas3_lib4.c:4157-4178): invalidate D's cache, then
-// refresh A's reciprocal diagonal unless it was valid.  Run as its own launch
-// so that no workgroup of the solve reads a dA another one is writing.
-template <typename T, int PS>
-__global__ void k_trsm_rltn_dA(const typename TrsmArg<T>::type *__restrict__ args)
+// ---------------------------------------------------------------------------
+// trsm_llnn / trsm_lltn: D = alpha * A^{-1} * B or alpha * A^{-T} * B, A lower
+// m x m, B m x n (blasfeo_ref/x_blas3_ref.c:445-673 and 674-900).
+// A workgroup keeps a block of up to LNC columns of X = alpha*B for all m rows
+// in LDS (column-major, odd leading dimension) and sweeps the 16-row blocks of
+// A, top-down for llnn and bottom-up for lltn: wave 0 solves the 16x16
+// diagonal triangle of the block (one column of X per lane, the 16 rows in
+// registers, multiplying by the reciprocal diagonal as the reference does),
+// then every thread applies the solved rows to one remaining row of X (below
+// for llnn, above for lltn), reading that row's 16 elements of A once.  The
+// next diagonal block is staged into the other half of a double buffer during
+// the update, so each block costs two workgroup barriers.
+// ---------------------------------------------------------------------------
+constexpr int LNC = 16;
+template <typename T> constexpr int trsml_mmax() { return sizeof(T) == 8 ? 1024 : 2048; }
+
+template <typename T, int PS, bool TRANS>
+__device__ __forceinline__ void trsml_stage_diag(const PM<T, PS> gA, int ai, int aj, int r0, int rv, T *Ad)
+{
+	for (int e = threadIdx.x; e < 256; e += NT3)
+	{
+		const int r = e & 15, c = e >> 4;
+		T v = T(0);
+		if (r < rv && c < rv && r > c)
+			v = gA.at(ai + r0 + r, aj + r0 + c);
+		Ad[r * 17 + c] = v;
+	}
+}
+
+template <typename T, int PS, bool TRANS>
+__global__ __launch_bounds__(NT3) void k_trsm_left(const typename TrsmArg<T>::type *__restrict__ args, int mmax)
+{
+	extern __shared__ unsigned char smem_raw[];
+	__shared__ T Ad[2][16 * 17];
+	const int mpad = (mmax + 15) / 16 * 16;
+	const int LDM = mpad + 1;
+	T *Xs = reinterpret_cast<T *>(smem_raw);   // [LNC][LDM]
+	T *inv_s = Xs + (size_t)LNC * LDM;           // [mpad]
+
+	const typename TrsmArg<T>::type &a = args[blockIdx.x];
+	const int m = a.m, n = a.n;
+	if (m <= 0 || n <= 0 || m > mmax)
+		return;
+	const PM<T, PS> gA = view<T, PS>(a.sA);
+	const PM<T, PS> gB = view<T, PS>(a.sB);
+	PM<T, PS> gD = view<T, PS>(a.sD);
+	const int ai = a.ai, aj = a.aj;
+	const T alpha = a.alpha;
+	const int lane = lane_id();
+	const int w = wave_id();
+	const int mb = (m + 15) / 16;
+
+	const bool cached = (ai == 0 && aj == 0 && a.sD != a.sA && a.sA->use_dA >= m);
+	for (int j = threadIdx.x; j < m; j += NT3)
+		inv_s[j] = cached ? a.sA->dA[j] : T(1) / gA.at(ai + j, aj + j);
+
+	for (int j0 = blockIdx.y * LNC; j0 < n; j0 += gridDim.y * LNC)
+	{
+		const int nc = min(LNC, n - j0);
+		__syncthreads();
+		g2s<T, PS, NT3, false>(gB, a.bi, a.bj + j0, mb * 16, LNC, m, nc, Xs, LDM);
+		{
+			const int rb0 = TRANS ? mb - 1 : 0;
+			trsml_stage_diag<T, PS, TRANS>(gA, ai, aj, rb0 * 16, min(16, m - rb0 * 16), Ad[0]);
+		}
+		__syncthreads();
+		for (int e = threadIdx.x; e < LNC * LDM; e += NT3)
+			Xs[e] *= alpha;
+		__syncthreads();
+		for (int s = 0; s < mb; s++)
+		{
+			const int rb = TRANS ? mb - 1 - s : s;
+			const int r0 = rb * 16;
+			const int rv = min(16, m - r0);
+			const T *A0 = Ad[s & 1];
+			// diagonal triangle: lane c owns column c of the block
+			if (w == 0 && lane < nc)
+			{
+				T x[16];
+#pragma unroll
+				for (int l = 0; l < 16; l++)
+					x[l] = Xs[lane * LDM + r0 + l];
+				if (!TRANS)
+				{
+#pragma unroll
+					for (int i = 0; i < 16; i++)
+					{
+						if (i < rv)
+						{
+							T c = x[i];
+#pragma unroll
+							for (int l = 0; l < i; l++)
+								c -= A0[i * 17 + l] * x[l];
+							x[i] = c * inv_s[r0 + i];
+						}
+					}
+				}
+				else
+				{
+#pragma unroll
+					for (int i = 15; i >= 0; i--)
+					{
+						if (i < rv)
+						{
+							T c = x[i];
+#pragma unroll
+							for (int l = i + 1; l < 16; l++)
+								c -= A0[l * 17 + i] * x[l];
+							x[i] = c * inv_s[r0 + i];
+						}
+					}
+				}
+#pragma unroll
+				for (int l = 0; l < 16; l++)
+					if (l < rv)
+						Xs[lane * LDM + r0 + l] = x[l];
+			}
+			__syncthreads();
+			// stage the next diagonal block, then update the remaining rows
+			if (s + 1 < mb)
+			{
+				const int rbn = TRANS ? rb - 1 : rb + 1;
+				trsml_stage_diag<T, PS, TRANS>(gA, ai, aj, rbn * 16, min(16, m - rbn * 16), Ad[(s + 1) & 1]);
+			}
+			const int ilo = TRANS ? 0 : r0 + 16;
+			const int ihi = TRANS ? r0 : m;
+			for (int i = ilo + threadIdx.x; i < ihi; i += NT3)
+			{
+				T av[16];
+#pragma unroll
+				for (int l = 0; l < 16; l++)
+				{
+					T v = T(0);
+					if (l < rv)
+						v = TRANS ? gA.at(ai + r0 + l, aj + i) : gA.at(ai + i, aj + r0 + l);
+					av[l] = v;
+				}
+				for (int c = 0; c < nc; c++)
+				{
+					T acc = Xs[c * LDM + i];
+#pragma unroll
+					for (int l = 0; l < 16; l++)
+						acc -= av[l] * Xs[c * LDM + r0 + l];
+					Xs[c * LDM + i] = acc;
+				}
+			}
+			__syncthreads();
+		}
+		s2g<T, PS, NT3, false>(gD, a.di, a.dj + j0, m, nc, Xs, LDM, 0);
+	}
+}
+
+// Cache update after trsm_rltn / trsm_llnn / trsm_lltn, in the reference's
+// order (blasfeo_hp_pm/d_blas3_lib4.c:4157-4178; blasfeo_ref/x_blas3_ref.c:
+// 466-480): invalidate D's cache, then refresh A's reciprocal diagonal unless
+// it was valid.  Run as its own launch so that no workgroup of the solve
+// reads a dA another one is writing.  LEFT: the triangle is m x m, else n x n.
+template <typename T, int PS, bool LEFT>
+__global__ void k_trsm_dA(const typename TrsmArg<T>::type *__restrict__ args)
 {
 	const typename TrsmArg<T>::type &a = args[blockIdx.x];
-	const int n = a.n;
-	if (a.m <= 0 || n <= 0)
+	const int n = LEFT ? a.m : a.n;
+	if (a.m <= 0 || a.n <= 0)
 		return;
 	const bool zero_off = (a.ai == 0 && a.aj == 0);
 	const int use0 = (a.sD == a.sA) ? 0 : a.sA->use_dA;
@@ -248,7 +401,7 @@ static int launch_trsm_ps(int batch, const typename TrsmArg<T>::type *args, dim3
 	int rc = check_launch(name);
 	if (rc)
 		return rc;
-	hipLaunchKernelGGL((k_trsm_rltn_dA<T, PS>), dim3(batch), dim3(64), 0, st, args);
+	hipLaunchKernelGGL((k_trsm_dA<T, PS, false>), dim3(batch), dim3(64), 0, st, args);
 	return check_launch(name);
 }
 
@@ -273,6 +426,39 @@ static int launch_trsm(int ps, int batch, const typename TrsmArg<T>::type *args,
 	return launch_trsm_ps<T, 16>(batch, args, grid, shm, npad, st, name);
 }
 
+template <typename T, int PS, bool TRANS>
+static int launch_trsml_ps(int batch, const typename TrsmArg<T>::type *args, dim3 grid, size_t shm, int mmax, hipStream_t st, const char *name)
+{
+	hipLaunchKernelGGL((k_trsm_left<T, PS, TRANS>), grid, dim3(NT3), shm, st, args, mmax);
+	int rc = check_launch(name);
+	if (rc)
+		return rc;
+	hipLaunchKernelGGL((k_trsm_dA<T, PS, true>), dim3(batch), dim3(64), 0, st, args);
+	return check_launch(name);
+}
+
+template <typename T, bool TRANS>
+static int launch_trsml(int ps, int batch, const typename TrsmArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
+{
+	int rc = check_common(ps, batch, args, sizeof(T) == 4, name);
+	if (rc || batch == 0)
+		return rc;
+	if (!hint || hint->max_m <= 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: hint->max_m (upper bound of m) is required", name);
+	if (hint->max_m > trsml_mmax<T>())
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: m up to %d supported, got %d", name, trsml_mmax<T>(), hint->max_m);
+	const int mmax = hint->max_m;
+	const int mpad = (mmax + 15) / 16 * 16;
+	const size_t shm = sizeof(T) * ((size_t)LNC * (mpad + 1) + mpad);
+	const int gy = hint->max_n > 0 ? min(64, (hint->max_n + LNC - 1) / LNC) : 1;
+	dim3 grid(batch, gy);
+	if (ps == 4)
+		return launch_trsml_ps<T, 4, TRANS>(batch, args, grid, shm, mmax, st, name);
+	if (ps == 8)
+		return launch_trsml_ps<T, 8, TRANS>(batch, args, grid, shm, mmax, st, name);
+	return launch_trsml_ps<T, 16, TRANS>(batch, args, grid, shm, mmax, st, name);
+}
+
 } // namespace bfa
 
 using namespace bfa;
@@ -291,6 +477,10 @@ int blasfeo_amd_dtrmm_rlnn_batched(int ps, int batch, const blasfeo_amd_dtrsm_ar
 { return launch_trmm<double>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrmm_rlnn_batched"); }
 int blasfeo_amd_dtrsm_rltn_batched(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
 { return launch_trsm<double>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_rltn_batched"); }
+int blasfeo_amd_dtrsm_llnn_batched(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_trsml<double, false>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_llnn_batched"); }
+int blasfeo_amd_dtrsm_lltn_batched(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_trsml<double, true>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_lltn_batched"); }
 
 int blasfeo_amd_sgemm_nt_batched(int ps, int batch, const blasfeo_amd_sgemm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
 { return launch_gemm<float, OP_NT>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_sgemm_nt_batched"); }

@@ -164,17 +164,17 @@ namespace bfa {
 
 // ---------------------------------------------------------------------------
 // In-LDS right-looking blocked Cholesky of an n x n matrix held column-major
-// in LDS (M[c*ld + r], lower triangle used, n <= 128), in place: M <- L,
+// in LDS (M[c*ld + r], lower triangle used, n <= NMAX <= 128), in place: M <- L,
 // invd[j] = 1/sqrt(pivot) with the zero-pivot rule above (L_jj = c*invd[j]).
 // Rows and columns n .. ceil16(n)-1 must hold zeros on entry.
 // Per 16-column block: wave 0 factors the whole panel (diagonal block and the
-// rows below, two rows per lane) in registers, broadcasting L(jj,j) with
+// rows below, two rows per lane, one when NMAX <= 64) in registers, broadcasting L(jj,j) with
 // v_readlane, so the panel needs no workgroup barrier; then the waves apply
 // the rank-16 trailing update to the lower tiles on the matrix cores, one
 // block column of tiles at a time with independent accumulators.
 // Called by all NT threads of the workgroup; starts and ends with a barrier.
 // ---------------------------------------------------------------------------
-template <typename T, int NT = NTL>
+template <typename T, int NT = NTL, int NMAX = 128>
 __device__ void chol_lds(T *M, int ld, int n, T *invd)
 {
 	typedef typename Vec4<T>::type V4;
@@ -191,7 +191,7 @@ __device__ void chol_lds(T *M, int ld, int n, T *invd)
 		if (w == 0)
 		{
 			T x0[16], x1[16];
-			const bool r0ok = lane < nr, r1ok = lane + 64 < nr;
+			const bool r0ok = lane < nr, r1ok = NMAX > 64 && lane + 64 < nr;
 #pragma unroll
 			for (int j = 0; j < 16; j++)
 			{

@@ -1,0 +1,539 @@
+// bfa_posv.hip -- batched SPD solve A X = B for gfx950: FP32 Cholesky factor
+// resident in LDS, FP64 iterative refinement, FP64 fallback on the device.
+//
+// One workgroup of 8 waves per instance, n <= 256.  The lower triangle of A
+// is converted to FP32 into LDS as packed 16x16 tiles (tile (I,J), I >= J, at
+// index I(I+1)/2 + J, column-major inside the tile): 136 tiles = 139,264 B
+// for n = 256, where the FP64 triangle (272 KB) would not fit.  The factor
+// never leaves LDS:
+//   per tile column K: wave 0 factors the diagonal tile in FP64 registers
+//   (v_readlane broadcasts, reciprocal square roots) and inverts it in FP64;
+//   the FP32 inverse Z_K replaces the tile; the panel L(I,K) = A(I,K) Z_K^T
+//   and the trailing update A(I,J) -= L(I,K) L(J,K)^T run on the FP32 matrix
+//   cores (v_mfma_f32_16x16x4f32).
+// Solves with the factor apply Z_K to the 16-row blocks, so each block step
+// is a small matrix-vector product instead of a 16-step substitution.
+// Refinement: r = b - A x is formed in FP64 from the FP64 A in global memory
+// (the symmetric product from the stored lower triangle), the correction
+// solves in FP32 and is accumulated into the FP64 x held in LDS, until the
+// normwise backward error max_c |r_c|/(|A| |x_c| + |b_c|) (infinity norms)
+// is <= tol.  Instances whose FP32 factor meets a non-positive pivot or whose
+// refinement does not converge are solved again by the FP64 kernels
+// (potrf_l + trsm_llnn + trsm_lltn) in the same stream.
+#include "bfa_chol_core.hpp"
+#include "bfa_internal.hpp"
+
+namespace bfa {
+
+constexpr int PV_NT = 512;
+constexpr int PV_NW = PV_NT / WAVE;
+constexpr int PV_NMAX = 256;
+constexpr int PV_R = 4;  // right-hand sides per refinement pass
+
+__device__ __forceinline__ int tix(int I, int J) { return I * (I + 1) / 2 + J; }
+
+struct PosvLds
+{
+	float *F;      // packed tiles
+	double *X;     // [n16][4] current solution (FP64)
+	float *R;      // [n16][4] right-hand side / residual of the FP32 solves
+	double *S;     // [n16][5] scratch: residual partial sums; factor: L_KK, 1/l_jj
+	double *red;   // [PV_NW][12] reductions
+	int *flag;     // [4]
+};
+
+__host__ __device__ inline size_t posv_lds_bytes(int nb)
+{
+	const size_t ntile = (size_t)nb * (nb + 1) / 2;
+	return ntile * 256 * sizeof(float) + (size_t)nb * 16 * PV_R * (sizeof(double) + sizeof(float))
+		+ (size_t)nb * 16 * 5 * sizeof(double) + PV_NW * 12 * sizeof(double) + 4 * sizeof(int);
+}
+
+__device__ inline PosvLds posv_carve(unsigned char *base, int nb)
+{
+	PosvLds L;
+	const size_t ntile = (size_t)nb * (nb + 1) / 2;
+	L.F = reinterpret_cast<float *>(base);
+	unsigned char *p = base + ntile * 256 * sizeof(float);
+	L.X = reinterpret_cast<double *>(p);
+	p += (size_t)nb * 16 * PV_R * sizeof(double);
+	L.S = reinterpret_cast<double *>(p);
+	p += (size_t)nb * 16 * 5 * sizeof(double);
+	L.red = reinterpret_cast<double *>(p);
+	p += PV_NW * 12 * sizeof(double);
+	L.R = reinterpret_cast<float *>(p);
+	p += (size_t)nb * 16 * PV_R * sizeof(float);
+	L.flag = reinterpret_cast<int *>(p);
+	return L;
+}
+
+// Block-wide maximum of NV values per thread (every thread gets the result).
+template <int NV>
+__device__ __forceinline__ void block_max(double *v, double *red)
+{
+	const int lane = lane_id(), w = wave_id();
+#pragma unroll
+	for (int q = 0; q < NV; q++)
+		for (int m = 1; m < WAVE; m <<= 1)
+			v[q] = fmax(v[q], __shfl_xor(v[q], m, WAVE));
+	__syncthreads();
+	if (lane == 0)
+#pragma unroll
+		for (int q = 0; q < NV; q++)
+			red[w * 12 + q] = v[q];
+	__syncthreads();
+#pragma unroll
+	for (int q = 0; q < NV; q++)
+	{
+		double r = red[q];
+		for (int ww = 1; ww < PV_NW; ww++)
+			r = fmax(r, red[ww * 12 + q]);
+		v[q] = r;
+	}
+}
+
+// In-LDS Cholesky of the nb x nb tile matrix.  Returns (uniformly) false if a
+// pivot was not positive.
+__device__ bool posv_factor(const PosvLds &L, int nb)
+{
+	typedef f4_t V4;
+	const int lane = lane_id();
+	const int w = wave_id();
+	if (threadIdx.x == 0)
+		L.flag[0] = 0;
+	__syncthreads();
+	for (int K = 0; K < nb; K++)
+	{
+		float *FKK = L.F + tix(K, K) * 256;
+		if (w == 0)
+		{
+			// lanes 0..15 hold row `lane` of the diagonal tile
+			double x[16];
+#pragma unroll
+			for (int j = 0; j < 16; j++)
+				x[j] = lane < 16 ? (double)FKK[j * 16 + lane] : 0.0;
+			double iv[16];
+			bool bad = false;
+#pragma unroll
+			for (int j = 0; j < 16; j++)
+			{
+				const double c = readlane(x[j], j);
+				const bool ok = c > 0.0;
+				bad = bad || !ok;
+				const double inv = ok ? rsqrt_fast(c) : 0.0;
+				iv[j] = inv;
+				if (lane > j)
+					x[j] *= inv;
+				else if (lane == j)
+					x[j] = c * inv;
+#pragma unroll
+				for (int jj = j + 1; jj < 16; jj++)
+				{
+					const double l = readlane(x[j], jj);
+					if (lane >= jj)
+						x[jj] -= x[j] * l;
+				}
+			}
+			// Z = L_KK^{-1}: lane c computes column c by forward substitution
+			double z[16];
+#pragma unroll
+			for (int i = 0; i < 16; i++)
+			{
+				double s = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+				for (int k = 0; k < i; k++)
+					s -= readlane(x[k], i) * z[k];
+				z[i] = s * iv[i];
+			}
+			if (lane < 16)
+#pragma unroll
+				for (int i = 0; i < 16; i++)
+					FKK[lane * 16 + i] = (float)z[i];
+			if (bad && lane == 0)
+				L.flag[0] = 1;
+		}
+		__syncthreads();
+		// panel: L(I,K) = A(I,K) Z^T
+		for (int I = K + 1 + w; I < nb; I += PV_NW)
+		{
+			float *FIK = L.F + tix(I, K) * 256;
+			V4 acc = V4{0, 0, 0, 0};
+#pragma unroll
+			for (int kk = 0; kk < 16; kk += 4)
+			{
+				const int o = (kk + (lane >> 4)) * 16 + (lane & 15);
+				acc = mfma16(FIK[o], FKK[o], acc);
+			}
+			*reinterpret_cast<V4 *>(FIK + (lane & 15) * 16 + 4 * (lane >> 4)) = acc;
+		}
+		__syncthreads();
+		// trailing update: tasks (J, group of up to 4 tiles I >= J)
+		{
+			int ntask = 0;
+			for (int J = K + 1; J < nb; J++)
+				ntask += (nb - J + 3) / 4;
+			for (int task = w; task < ntask; task += PV_NW)
+			{
+				int J = K + 1, rem = task;
+				while (rem >= (nb - J + 3) / 4)
+				{
+					rem -= (nb - J + 3) / 4;
+					J++;
+				}
+				const int g = J + 4 * rem;
+				const float *FJK = L.F + tix(J, K) * 256;
+				V4 acc[4];
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+					if (g + t < nb)
+						acc[t] = *reinterpret_cast<const V4 *>(L.F + tix(g + t, J) * 256 + (lane & 15) * 16 + 4 * (lane >> 4));
+#pragma unroll
+				for (int kk = 0; kk < 16; kk += 4)
+				{
+					const int o = (kk + (lane >> 4)) * 16 + (lane & 15);
+					const float b = FJK[o];
+#pragma unroll
+					for (int t = 0; t < 4; t++)
+						if (g + t < nb)
+							acc[t] = mfma16(-L.F[tix(g + t, K) * 256 + o], b, acc[t]);
+				}
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+					if (g + t < nb)
+						*reinterpret_cast<V4 *>(L.F + tix(g + t, J) * 256 + (lane & 15) * 16 + 4 * (lane >> 4)) = acc[t];
+			}
+		}
+		__syncthreads();
+	}
+	return L.flag[0] == 0;
+}
+
+// R <- (L L^T)^{-1} R in FP32 with the factor in LDS (diagonal tiles hold
+// Z_K = L_KK^{-1}).  Starts and ends with a barrier.
+__device__ void posv_solve(const PosvLds &L, int nb)
+{
+	const int t = threadIdx.x;
+	float *R = L.R;
+	__syncthreads();
+	// forward: y_K = Z_K r_K ; r_I -= L(I,K) y_K
+	for (int K = 0; K < nb; K++)
+	{
+		const float *Z = L.F + tix(K, K) * 256;
+		if (t < 64)
+		{
+			const int i = t & 15, c = t >> 4;
+			float s = 0.0f;
+#pragma unroll
+			for (int k = 0; k < 16; k++)
+				s += Z[k * 16 + i] * R[(16 * K + k) * PV_R + c];
+			R[(16 * K + i) * PV_R + c] = s;
+		}
+		__syncthreads();
+		const int np = (nb - K - 1) * 16 * PV_R;
+		for (int p = t; p < np; p += PV_NT)
+		{
+			const int c = p & 3, ir = 16 * (K + 1) + (p >> 2);
+			const float *LI = L.F + tix(ir >> 4, K) * 256 + (ir & 15);
+			float s = 0.0f;
+#pragma unroll
+			for (int k = 0; k < 16; k++)
+				s += LI[k * 16] * R[(16 * K + k) * PV_R + c];
+			R[ir * PV_R + c] -= s;
+		}
+		__syncthreads();
+	}
+	// backward: x_K = Z_K^T y_K ; y_J -= L(K,J)^T x_K
+	for (int K = nb - 1; K >= 0; K--)
+	{
+		const float *Z = L.F + tix(K, K) * 256;
+		if (t < 64)
+		{
+			const int i = t & 15, c = t >> 4;
+			float s = 0.0f;
+#pragma unroll
+			for (int k = 0; k < 16; k++)
+				s += Z[i * 16 + k] * R[(16 * K + k) * PV_R + c];
+			R[(16 * K + i) * PV_R + c] = s;
+		}
+		__syncthreads();
+		const int np = K * 16 * PV_R;
+		for (int p = t; p < np; p += PV_NT)
+		{
+			const int c = p & 3, ir = p >> 2;
+			const float *LK = L.F + tix(K, ir >> 4) * 256 + (ir & 15) * 16;
+			float s = 0.0f;
+#pragma unroll
+			for (int k = 0; k < 16; k++)
+				s += LK[k] * R[(16 * K + k) * PV_R + c];
+			R[ir * PV_R + c] -= s;
+		}
+		__syncthreads();
+	}
+}
+
+template <int PS>
+__global__ __launch_bounds__(PV_NT) void k_posv_mixed(const blasfeo_amd_dposv_arg *__restrict__ args, int nmax,
+	double tol, int max_iter, int *__restrict__ path_out)
+{
+	extern __shared__ unsigned char smem_raw[];
+	const blasfeo_amd_dposv_arg &a = args[blockIdx.x];
+	const int n = a.n, nrhs = a.nrhs;
+	const int t = threadIdx.x;
+	const int lane = lane_id(), w = wave_id();
+	blasfeo_amd_solve_info *info = a.info;
+	if (n <= 0 || nrhs <= 0)
+	{
+		if (t == 0)
+		{
+			path_out[blockIdx.x] = 0;
+			if (info)
+				*info = blasfeo_amd_solve_info{0, 0, 0.0};
+		}
+		return;
+	}
+	if (n > nmax)
+	{
+		if (t == 0)
+		{
+			path_out[blockIdx.x] = 1;
+			if (info)
+				*info = blasfeo_amd_solve_info{0, 1, -1.0};
+		}
+		return;
+	}
+	const int nb = (n + 15) / 16;
+	const int n16 = nb * 16;
+	const PosvLds L = posv_carve(smem_raw, nb);
+	const PM<double, PS> gA = view<double, PS>(a.sA);
+	const PM<double, PS> gB = view<double, PS>(a.sB);
+	PM<double, PS> gX = view<double, PS>(a.sX);
+	const int ai = a.ai, aj = a.aj;
+
+	// A (lower triangle, FP64) -> FP32 tiles; identity beyond n
+	{
+		int tt = 0;
+		for (int I = 0; I < nb; I++)
+			for (int J = 0; J <= I; J++, tt++)
+			{
+				if (tt % PV_NW != w)
+					continue;
+				float *FT = L.F + tt * 256;
+				const int i = lane & 15;
+				const int r = I * 16 + i;
+#pragma unroll
+				for (int k = 0; k < 4; k++)
+				{
+					const int j = 4 * k + (lane >> 4);
+					const int c = J * 16 + j;
+					double v = (r == c) ? 1.0 : 0.0;
+					if (r < n && c < n && r >= c)
+						v = gA.at(ai + r, aj + c);
+					FT[j * 16 + i] = (float)v;
+				}
+			}
+	}
+	const bool ok = posv_factor(L, nb);
+
+	int iters = 0;
+	double berr = 0.0;
+	bool conv = ok;
+	for (int c0 = 0; conv && c0 < nrhs; c0 += PV_R)
+	{
+		const int nr = min(PV_R, nrhs - c0);
+		// b -> R (FP32), x = 0, |b|_inf
+		double bmax[PV_R] = {0, 0, 0, 0};
+		for (int p = t; p < n16 * PV_R; p += PV_NT)
+		{
+			const int c = p & 3, i = p >> 2;
+			double v = 0.0;
+			if (i < n && c < nr)
+				v = gB.at(a.bi + i, a.bj + c0 + c);
+			L.R[p] = (float)v;
+			L.X[p] = 0.0;
+#pragma unroll
+			for (int q = 0; q < PV_R; q++)
+				if (q == c)
+					bmax[q] = fmax(bmax[q], fabs(v));
+		}
+		block_max<PV_R>(bmax, L.red);
+		double anorm = 0.0;
+		double eta_prev = 1e300;
+		int it = 0;
+		for (;;)
+		{
+			posv_solve(L, nb);
+			for (int p = t; p < n16 * PV_R; p += PV_NT)
+				L.X[p] += (double)L.R[p];
+			__syncthreads();
+			// r = b - A x in FP64; half h of the threads takes columns [jlo, jhi)
+			const int h = t >> 8, i = t & 255;
+			const int jm = (n + 1) >> 1;
+			const int jlo = h ? jm : 0, jhi = h ? n : jm;
+			double acc[PV_R] = {0, 0, 0, 0};
+			double sabs = 0.0;
+			if (i < n)
+			{
+				for (int j = jlo; j < jhi; j++)
+				{
+					const double v = (j <= i) ? gA.at(ai + i, aj + j) : gA.at(ai + j, aj + i);
+					const double *xj = L.X + j * PV_R;
+#pragma unroll
+					for (int q = 0; q < PV_R; q++)
+						acc[q] += v * xj[q];
+					sabs += fabs(v);
+				}
+			}
+			if (h == 1 && i < n)
+			{
+#pragma unroll
+				for (int q = 0; q < PV_R; q++)
+					L.S[i * 5 + q] = acc[q];
+				L.S[i * 5 + 4] = sabs;
+			}
+			__syncthreads();
+			double v8[2 * PV_R + 1];
+#pragma unroll
+			for (int q = 0; q < 2 * PV_R + 1; q++)
+				v8[q] = 0.0;
+			if (h == 0 && i < n)
+			{
+#pragma unroll
+				for (int q = 0; q < PV_R; q++)
+				{
+					const double bq = q < nr ? gB.at(a.bi + i, a.bj + c0 + q) : 0.0;
+					const double rq = bq - (acc[q] + L.S[i * 5 + q]);
+					L.R[i * PV_R + q] = (float)rq;
+					v8[q] = fabs(rq);
+					v8[PV_R + q] = fabs(L.X[i * PV_R + q]);
+				}
+				v8[2 * PV_R] = sabs + L.S[i * 5 + 4];
+			}
+			if (h == 0 && i >= n && i < n16)
+#pragma unroll
+				for (int q = 0; q < PV_R; q++)
+					L.R[i * PV_R + q] = 0.0f;
+			block_max<2 * PV_R + 1>(v8, L.red);
+			if (it == 0)
+				anorm = v8[2 * PV_R];
+			double eta = 0.0;
+#pragma unroll
+			for (int q = 0; q < PV_R; q++)
+			{
+				if (q < nr)
+				{
+					const double den = anorm * v8[PV_R + q] + bmax[q];
+					const double e = den > 0.0 ? v8[q] / den : (v8[q] > 0.0 ? 1e300 : 0.0);
+					eta = fmax(eta, e);
+				}
+			}
+			if (!(eta <= tol))
+			{
+				// not converged: stop on the iteration limit, on growth, or on NaN
+				if (it >= max_iter || !(eta < eta_prev))
+				{
+					conv = false;
+					berr = fmax(berr, eta);
+					iters = max(iters, it);
+					break;
+				}
+				eta_prev = eta;
+				it++;
+				continue;
+			}
+			berr = fmax(berr, eta);
+			iters = max(iters, it);
+			break;
+		}
+		if (conv)
+		{
+			__syncthreads();
+			for (int p = t; p < n * PV_R; p += PV_NT)
+			{
+				const int c = p & 3, i = p >> 2;
+				if (c < nr)
+					gX.at(a.xi + i, a.xj + c0 + c) = L.X[p];
+			}
+		}
+	}
+	if (t == 0)
+	{
+		path_out[blockIdx.x] = conv ? 0 : 1;
+		if (info)
+			*info = conv ? blasfeo_amd_solve_info{iters, 0, berr} : blasfeo_amd_solve_info{iters, 1, -1.0};
+	}
+	if (t == 0)
+		a.sX->use_dA = 0;
+}
+
+// Records for the FP64 re-solve of the instances that did not converge:
+// potrf_l(A -> L), trsm_llnn(L, B -> X), trsm_lltn(L, X -> X); size 0 (a
+// no-op) for the others.
+__global__ void k_posv_fallback_args(const blasfeo_amd_dposv_arg *__restrict__ args, const int *__restrict__ path,
+	int batch, blasfeo_amd_dfact_arg *fa, blasfeo_amd_dtrsm_arg *t1, blasfeo_amd_dtrsm_arg *t2)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= batch)
+		return;
+	const blasfeo_amd_dposv_arg &a = args[b];
+	const int n = (path[b] == 1 && a.nrhs > 0) ? a.n : 0;
+	fa[b] = blasfeo_amd_dfact_arg{n, n, a.ai, a.aj, a.li, a.lj, a.sA, a.sL, nullptr};
+	t1[b] = blasfeo_amd_dtrsm_arg{n, a.nrhs, a.li, a.lj, a.bi, a.bj, a.xi, a.xj, 1.0, a.sL, a.sB, a.sX};
+	t2[b] = blasfeo_amd_dtrsm_arg{n, a.nrhs, a.li, a.lj, a.xi, a.xj, a.xi, a.xj, 1.0, a.sL, a.sX, a.sX};
+}
+
+} // namespace bfa
+
+using namespace bfa;
+
+extern "C" int blasfeo_amd_dposv_mixed_batched(int ps, int batch, const blasfeo_amd_dposv_arg *args, double tol,
+	int max_iter, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{
+	const char *name = "blasfeo_amd_dposv_mixed_batched";
+	hipStream_t st = (hipStream_t)s;
+	if (batch < 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: batch=%d < 0", name, batch);
+	if (batch > 0 && !args)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: args is NULL", name);
+	if (ps != 4 && ps != 8)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: unsupported panel size %d", name, ps);
+	if (!(tol > 0.0) || max_iter < 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: tol must be > 0 and max_iter >= 0", name);
+	if (!hint || hint->max_m <= 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: hint->max_m (upper bound of n) is required", name);
+	if (hint->max_m > fact_mmax<double>())
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: n up to %d supported, got %d", name, fact_mmax<double>(), hint->max_m);
+	if (batch == 0)
+		return BLASFEO_AMD_OK;
+	const int nmax = min(hint->max_m, PV_NMAX);
+	const int nb = (nmax + 15) / 16;
+	const size_t shm = posv_lds_bytes(nb);
+	// workspace: per-instance path, then the fallback records
+	const size_t off_fa = ((size_t)batch * sizeof(int) + 255) / 256 * 256;
+	const size_t off_t1 = off_fa + ((size_t)batch * sizeof(blasfeo_amd_dfact_arg) + 255) / 256 * 256;
+	const size_t off_t2 = off_t1 + ((size_t)batch * sizeof(blasfeo_amd_dtrsm_arg) + 255) / 256 * 256;
+	const size_t bytes = off_t2 + (size_t)batch * sizeof(blasfeo_amd_dtrsm_arg);
+	unsigned char *ws = static_cast<unsigned char *>(workspace(bytes, st));
+	if (!ws)
+		return set_error(BLASFEO_AMD_ENOMEM, "%s: workspace of %zu bytes not available", name, bytes);
+	int *path = reinterpret_cast<int *>(ws);
+	auto *fa = reinterpret_cast<blasfeo_amd_dfact_arg *>(ws + off_fa);
+	auto *t1 = reinterpret_cast<blasfeo_amd_dtrsm_arg *>(ws + off_t1);
+	auto *t2 = reinterpret_cast<blasfeo_amd_dtrsm_arg *>(ws + off_t2);
+	if (ps == 4)
+		hipLaunchKernelGGL((k_posv_mixed<4>), dim3(batch), dim3(PV_NT), shm, st, args, nmax, tol, max_iter, path);
+	else
+		hipLaunchKernelGGL((k_posv_mixed<8>), dim3(batch), dim3(PV_NT), shm, st, args, nmax, tol, max_iter, path);
+	int rc = check_launch(name);
+	if (rc)
+		return rc;
+	hipLaunchKernelGGL(k_posv_fallback_args, dim3((batch + 255) / 256), dim3(256), 0, st, args, path, batch, fa, t1, t2);
+	if ((rc = check_launch(name)))
+		return rc;
+	blasfeo_amd_hint h2 = {hint->max_m, hint->max_m, 0};
+	if ((rc = blasfeo_amd_dpotrf_l_batched(ps, batch, fa, &h2, s)))
+		return rc;
+	blasfeo_amd_hint h3 = {hint->max_m, hint->max_n > 0 ? hint->max_n : 64, 0};
+	if ((rc = blasfeo_amd_dtrsm_llnn_batched(ps, batch, t1, &h3, s)))
+		return rc;
+	return blasfeo_amd_dtrsm_lltn_batched(ps, batch, t2, &h3, s);
+}

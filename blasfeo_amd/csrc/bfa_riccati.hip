@@ -50,55 +50,19 @@ __host__ __device__ __forceinline__ int ric_footprint(int nkmax, int nxmax)
 // column-major with LDM = nkp + 1, so 16 lanes reading one row of a column
 // block hit distinct banks.
 // ---------------------------------------------------------------------------
-template <int PS, int NT>
-__device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
+// The two matrix-core stages of a Riccati step, kept out of line: inlined
+// into the stage loop, their address arithmetic is hoisted across the whole
+// sweep and the one-wave kernel needs 248 VGPRs + 64 AGPRs (one wave per
+// SIMD); out of line it needs 102 + 32 and three instances share a SIMD.
+//
+// W <- W * Lxx (trmm_rlnn), Lxx = L[k+1](u1:u1+n1, u1:u1+n1), held in M.
+template <int NT>
+__device__ __attribute__((noinline)) void ric_trmm_w(const double *M, double *W, int LDM, int nkt, int n1, int n1t, int u1)
 {
 	typedef d4_t V4;
 	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
 	const int w = wave_id();
-	const int N = a.N;
-	const int *nx = a.nx;
-	const int *nu = a.nu;
-	blasfeo_dmat *L = a.L;
-	blasfeo_dmat *RSQ = a.RSQrq;
-	blasfeo_dmat *BAbt = a.BAbt;
-	int nkmax = nx[N], nxmax = nx[N];
-	for (int k = 0; k < N; k++)
-	{
-		nkmax = max(nkmax, nu[k] + nx[k]);
-		nxmax = max(nxmax, nx[k]);
-	}
-	const int nkp = (nkmax + 15) / 16 * 16;
-	const int nxp = (nxmax + 15) / 16 * 16;
-	const int LDM = nkp + 1;
-	double *M = smem;
-	double *W = M + (size_t)nkp * LDM;
-	double *invd = W + (size_t)nxp * LDM;
-
-	// last stage: L[N] = chol(Q_N)
-	{
-		const int m = nx[N];
-		const int mt = (m + 15) / 16;
-		g2s<double, PS, NT, false, true>(view<double, PS>(&RSQ[N]), 0, 0, mt * 16, mt * 16, m, m, M, LDM);
-		chol_lds<double, NT>(M, LDM, m, invd);
-		s2g<double, PS, NT, true>(view<double, PS>(&L[N]), 0, 0, m, m, M, LDM, 0);
-		for (int j = threadIdx.x; j < m; j += NT)
-			L[N].dA[j] = invd[j];
-		if (threadIdx.x == 0)
-			L[N].use_dA = m;
-	}
-	for (int s = 0; s < N; s++)
-	{
-		const int k = N - 1 - s;
-		const int nk = nu[k] + nx[k];
-		const int n1 = nx[k + 1];
-		const int u1 = nu[k + 1];
-		const int nkt = (nk + 15) / 16;
-		const int n1t = (n1 + 15) / 16;
-		// stage BAbt[k] (nk x n1) into W; M still holds L[k+1]
-		g2s<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W, LDM);
-		__syncthreads();
 		// W <- W * Lxx, Lxx = L[k+1](u1:u1+n1, u1:u1+n1) lower; 16-row strips per wave,
 		// the strip's 16 x 64 result kept in four accumulators
 		for (int rt = w; rt < nkt; rt += NW)
@@ -131,10 +95,16 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 					for (int r = 0; r < 4; r++)
 						W[(ct * 16 + (lane & 15)) * LDM + rt * 16 + acc_row<double>(lane, r)] = acc[ct][r];
 		}
-		__syncthreads();
-		// M <- lower(RSQ[k]) (L[k+1] is no longer needed)
-		g2s<double, PS, NT, false, true>(view<double, PS>(&RSQ[k]), 0, 0, nkt * 16, nkt * 16, nk, nk, M, LDM);
-		__syncthreads();
+}
+
+// M += W W^T on the lower tiles (syrk_ln).
+template <int NT>
+__device__ __attribute__((noinline)) void ric_syrk_m(double *M, const double *W, int LDM, int nkt, int n1)
+{
+	typedef d4_t V4;
+	constexpr int NW = NT / WAVE;
+	const int lane = lane_id();
+	const int w = wave_id();
 		// M += W W^T on the lower tiles: (block column bj, group of up to 4 tiles
 		// bi = g..g+3) per wave, the B operand shared by the group
 		{
@@ -183,8 +153,65 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 				}
 			}
 		}
+}
+
+template <int PS, int NT>
+__device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
+{
+	typedef d4_t V4;
+	constexpr int NW = NT / WAVE;
+	const int lane = lane_id();
+	const int w = wave_id();
+	const int N = a.N;
+	const int *nx = a.nx;
+	const int *nu = a.nu;
+	blasfeo_dmat *L = a.L;
+	blasfeo_dmat *RSQ = a.RSQrq;
+	blasfeo_dmat *BAbt = a.BAbt;
+	int nkmax = nx[N], nxmax = nx[N];
+	for (int k = 0; k < N; k++)
+	{
+		nkmax = max(nkmax, nu[k] + nx[k]);
+		nxmax = max(nxmax, nx[k]);
+	}
+	const int nkp = (nkmax + 15) / 16 * 16;
+	const int nxp = (nxmax + 15) / 16 * 16;
+	const int LDM = nkp + 1;
+	double *M = smem;
+	double *W = M + (size_t)nkp * LDM;
+	double *invd = W + (size_t)nxp * LDM;
+
+	// last stage: L[N] = chol(Q_N)
+	{
+		const int m = nx[N];
+		const int mt = (m + 15) / 16;
+		g2s<double, PS, NT, false, true>(view<double, PS>(&RSQ[N]), 0, 0, mt * 16, mt * 16, m, m, M, LDM);
+		chol_lds<double, NT, (NT == WAVE ? 64 : 128)>(M, LDM, m, invd);
+		s2g<double, PS, NT, true>(view<double, PS>(&L[N]), 0, 0, m, m, M, LDM, 0);
+		for (int j = threadIdx.x; j < m; j += NT)
+			L[N].dA[j] = invd[j];
+		if (threadIdx.x == 0)
+			L[N].use_dA = m;
+	}
+	for (int s = 0; s < N; s++)
+	{
+		const int k = N - 1 - s;
+		const int nk = nu[k] + nx[k];
+		const int n1 = nx[k + 1];
+		const int u1 = nu[k + 1];
+		const int nkt = (nk + 15) / 16;
+		const int n1t = (n1 + 15) / 16;
+		// stage BAbt[k] (nk x n1) into W; M still holds L[k+1]
+		g2s<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W, LDM);
+		__syncthreads();
+		ric_trmm_w<NT>(M, W, LDM, nkt, n1, n1t, u1);
+		__syncthreads();
+		// M <- lower(RSQ[k]) (L[k+1] is no longer needed)
+		g2s<double, PS, NT, false, true>(view<double, PS>(&RSQ[k]), 0, 0, nkt * 16, nkt * 16, nk, nk, M, LDM);
+		__syncthreads();
+		ric_syrk_m<NT>(M, W, LDM, nkt, n1);
 		// L[k] = chol(M)
-		chol_lds<double, NT>(M, LDM, nk, invd);
+		chol_lds<double, NT, (NT == WAVE ? 64 : 128)>(M, LDM, nk, invd);
 		s2g<double, PS, NT, true>(view<double, PS>(&L[k]), 0, 0, nk, nk, M, LDM, 0);
 		for (int j = threadIdx.x; j < nk; j += NT)
 			L[k].dA[j] = invd[j];
@@ -308,7 +335,8 @@ __global__ __launch_bounds__(NT) void k_ric_class(const blasfeo_amd_riccati_arg 
 }
 
 // one wave per instance: registers capped at 168 so that three instances
-// share each SIMD (the sweep needs no more without spilling)
+// share each SIMD.  Only classes of at most 40960 B use it, whose stages have
+// nu+nx <= 48, so the Cholesky keeps one panel row per lane (n <= 64).
 template <int PS>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3))) void k_ric_class1(
 	const blasfeo_amd_riccati_arg *__restrict__ args, const int *__restrict__ count, const int *__restrict__ list)

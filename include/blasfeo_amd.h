@@ -273,6 +273,12 @@ int blasfeo_amd_dsyrk_ln_batched(int ps, int batch, const struct blasfeo_amd_dge
 int blasfeo_amd_dsyrk_ln_mn_batched(int ps, int batch, const struct blasfeo_amd_dgemm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 /* replaces batch x blasfeo_dtrsm_rltn (blasfeo_hp_pm/d_blas3_lib4.c:7306,4123) */
 int blasfeo_amd_dtrsm_rltn_batched(int ps, int batch, const struct blasfeo_amd_dtrsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dtrsm_llnn (blasfeo_hp_pm/d_blas3_lib4.c:7236,3481;
+ * blasfeo_ref/x_blas3_ref.c:445): D = alpha * A^{-1} * B, A lower m x m, B m x n */
+int blasfeo_amd_dtrsm_llnn_batched(int ps, int batch, const struct blasfeo_amd_dtrsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dtrsm_lltn (blasfeo_hp_pm/d_blas3_lib4.c:3782;
+ * blasfeo_ref/x_blas3_ref.c:674): D = alpha * A^{-T} * B, A lower m x m, B m x n */
+int blasfeo_amd_dtrsm_lltn_batched(int ps, int batch, const struct blasfeo_amd_dtrsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 /* replaces batch x blasfeo_dtrmm_rlnn (blasfeo_hp_pm/d_blas3_lib4.c:4893) */
 int blasfeo_amd_dtrmm_rlnn_batched(int ps, int batch, const struct blasfeo_amd_dtrsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 /* replaces batch x blasfeo_dpotrf_l (blasfeo_hp_pm/d_lapack_lib4.c:4269,918) */
@@ -318,6 +324,47 @@ struct blasfeo_amd_riccati_arg
  * one launch (the per-stage trmm_rlnn + syrk_dpotrf_ln chain of
  * examples/example_d_riccati_recursion.c:120-138). */
 int blasfeo_amd_d_back_ric_trf_batched(int ps, int batch, const struct blasfeo_amd_riccati_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Cholesky solve A X = B with A symmetric positive definite                */
+/* ------------------------------------------------------------------------ */
+
+/* Outcome of one mixed-precision solve.
+ * iters: refinement corrections applied after the first solve;
+ * path:  0 = FP32 factor + FP64 refinement converged,
+ *        1 = solved by the FP64 factorisation (the FP32 factor broke down or
+ *            the refinement did not reach tol within max_iter);
+ * berr:  normwise backward error max_c |b_c - A x_c|_inf / (|A|_inf |x_c|_inf + |b_c|_inf)
+ *        of the returned solution when path = 0 (-1 when path = 1). */
+struct blasfeo_amd_solve_info
+	{
+	int iters;
+	int path;
+	double berr;
+	};
+
+/* One solve: n x n SPD A (lower triangle read) at (ai,aj), right-hand sides
+ * B (n x nrhs) at (bi,bj), solution X (n x nrhs) at (xi,xj).  sL (n x n at
+ * (li,lj)) receives the FP64 Cholesky factor of instances that take path 1
+ * (it may alias sA; then A is overwritten for those instances only).
+ * info may be NULL. */
+struct blasfeo_amd_dposv_arg
+	{
+	int n, nrhs;
+	int ai, aj, bi, bj, xi, xj, li, lj;
+	struct blasfeo_dmat *sA, *sB, *sX, *sL;
+	struct blasfeo_amd_solve_info *info;
+	};
+
+/* No single counterpart in the reference: it computes what the sequence
+ * blasfeo_dpotrf_l + blasfeo_dtrsm_llnn + blasfeo_dtrsm_lltn
+ * (blasfeo_hp_pm/d_lapack_lib4.c:4269; d_blas3_lib4.c:7236,3782) computes, to
+ * normwise backward error <= tol.  The factor is formed in FP32 and kept in
+ * LDS (FP32 matrix-core trailing updates, FP64 16x16 diagonal blocks), the
+ * residuals b - A x are formed in FP64 from the FP64 A, and corrections are
+ * applied until berr <= tol (at most max_iter).  Instances that do not
+ * converge are re-solved in FP64 into sL on the device.  n <= 256. */
+int blasfeo_amd_dposv_mixed_batched(int ps, int batch, const struct blasfeo_amd_dposv_arg *args, double tol, int max_iter, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 
 #ifdef __cplusplus
 }

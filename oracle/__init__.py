@@ -22,6 +22,7 @@ _SIGS = {
     "gemm_nt": "iiiRpiipiiRpiipii", "gemm_nn": "iiiRpiipiiRpiipii",
     "syrk_ln": "iiRpiipiiRpiipii", "syrk_ln_mn": "iiiRpiipiiRpiipii",
     "trsm_rltn": "iiRpiipiipii", "trmm_rlnn": "iiRpiipiipii",
+    "trsm_llnn": "iiRpiipiipii", "trsm_lltn": "iiRpiipiipii",
     "potrf_l": "ipiipii", "potrf_l_mn": "iipiipii",
     "syrk_potrf_ln": "iipiipiipiipii", "syrk_potrf_ln_mn": "iiipiipiipiipii",
     "getrf_rp": "iipiipiip", "ric_trf": "ippppp",
@@ -45,7 +46,7 @@ def lib():
                 f.argtypes = [_I] + [{"i": _I, "p": _V, "R": real}[c] for c in sig]
                 f.restype = None
         for name in ("dgemm_nt", "dgemm_nn", "dsyrk_ln", "dpotrf_l", "dgetrf_rp",
-                     "sgetrf_rp", "dric_trf"):
+                     "sgetrf_rp", "dric_trf", "dposv"):
             f = getattr(L, "or_%s_batch" % name)
             f.argtypes = [_I, _I, _V]
             f.restype = None

@@ -98,6 +98,19 @@ void or_dric_trf_batch(int ps, int batch, const struct blasfeo_amd_riccati_arg *
 		or_dric_trf(ps, a[i].N, a[i].nx, a[i].nu, a[i].BAbt, a[i].RSQrq, a[i].L, a[i].work);
 	}
 
+/* The FP64 Cholesky solve a reference caller writes:
+ * blasfeo_dpotrf_l(A -> L); blasfeo_dtrsm_llnn(L, B -> X); blasfeo_dtrsm_lltn(L, X -> X). */
+void or_dposv_batch(int ps, int batch, const struct blasfeo_amd_dposv_arg *a)
+	{
+#pragma omp parallel for schedule(dynamic, 4)
+	for(int i = 0; i < batch; i++)
+		{
+		or_dpotrf_l(ps, a[i].n, a[i].sA, a[i].ai, a[i].aj, a[i].sL, a[i].li, a[i].lj);
+		or_dtrsm_llnn(ps, a[i].n, a[i].nrhs, 1.0, a[i].sL, a[i].li, a[i].lj, a[i].sB, a[i].bi, a[i].bj, a[i].sX, a[i].xi, a[i].xj);
+		or_dtrsm_lltn(ps, a[i].n, a[i].nrhs, 1.0, a[i].sL, a[i].li, a[i].lj, a[i].sX, a[i].xi, a[i].xj, a[i].sX, a[i].xi, a[i].xj);
+		}
+	}
+
 int or_num_threads(void)
 	{
 	int t = 1;

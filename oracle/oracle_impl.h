@@ -109,6 +109,70 @@ void PFX(trsm_rltn)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, M
 		}
 	}
 
+/* dA / use_dA handling shared by the left-side solves (blasfeo_ref/x_blas3_ref.c:472-486, 701-715) */
+static REAL *PFX(trsm_left_diag)(int ps, int m, MAT *sA, int ai, int aj, MAT *sD)
+	{
+	sD->use_dA = 0;
+	REAL *dA = sA->dA;
+	if(ai == 0 && aj == 0)
+		{
+		if(sA->use_dA < m)
+			{
+			for(int i = 0; i < m; i++)
+				dA[i] = (REAL)1.0 / EL(sA, i, i);
+			sA->use_dA = m;
+			}
+		}
+	else
+		{
+		for(int i = 0; i < m; i++)
+			dA[i] = (REAL)1.0 / EL(sA, ai+i, aj+i);
+		sA->use_dA = 0;
+		}
+	return dA;
+	}
+
+/* D <= alpha * A^{-1} * B, A lower m x m ; blasfeo_ref/x_blas3_ref.c:445-673.
+ * Every element accumulates its k terms in ascending order (the 2x2 blocks of
+ * the reference subtract the in-block term last, which is also ascending). */
+void PFX(trsm_llnn)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	REAL *dA = PFX(trsm_left_diag)(ps, m, sA, ai, aj, sD);
+	for(int j = 0; j < n; j++)
+		for(int i = 0; i < m; i++)
+			{
+			REAL c = alpha * EL(sB, bi+i, bj+j);
+			for(int l = 0; l < i; l++)
+				c -= EL(sA, ai+i, aj+l) * EL(sD, di+l, dj+j);
+			EL(sD, di+i, dj+j) = c * dA[i];
+			}
+	}
+
+/* D <= alpha * A^{-T} * B, A lower m x m ; blasfeo_ref/x_blas3_ref.c:674-900.
+ * Rows are solved bottom-up in pairs (id, id+1), id = m-2, m-4, ...: the
+ * upper row of a pair accumulates k = id+2 .. m-1 ascending and subtracts the
+ * in-pair term k = id+1 last; the lower row of a pair, and the single row 0
+ * left over when m is odd, accumulate k ascending from the row below. */
+void PFX(trsm_lltn)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	REAL *dA = PFX(trsm_left_diag)(ps, m, sA, ai, aj, sD);
+	for(int j = 0; j < n; j++)
+		for(int i = m - 1; i >= 0; i--)
+			{
+			REAL c = alpha * EL(sB, bi+i, bj+j);
+			const int pair_top = ((m - i) % 2 == 0);
+			for(int l = pair_top ? i + 2 : i + 1; l < m; l++)
+				c -= EL(sA, ai+l, aj+i) * EL(sD, di+l, dj+j);
+			if(pair_top)
+				c -= EL(sA, ai+i+1, aj+i) * EL(sD, di+i+1, dj+j);
+			EL(sD, di+i, dj+j) = c * dA[i];
+			}
+	}
+
 /* D <= alpha * B * A, A lower n x n ; blasfeo_ref/x_blas3_ref.c:3218-3330 */
 void PFX(trmm_rlnn)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sD, int di, int dj)
 	{

@@ -41,6 +41,8 @@ def test_struct_layouts_match_c(tmp_path):
         "blasfeo_amd_dfact_arg": (bfa.DFACT_ARG, ["dj", "sC", "sD", "ipiv"]),
         "blasfeo_amd_dsyrk_dpotrf_arg": (bfa.DSYRK_DPOTRF_ARG, ["dj", "sA", "sD"]),
         "blasfeo_amd_riccati_arg": (bfa.RICCATI_ARG, ["nx", "nu", "BAbt", "RSQrq", "L", "work"]),
+        "blasfeo_amd_dposv_arg": (bfa.DPOSV_ARG, ["n", "nrhs", "ai", "lj", "sA", "sB", "sX", "sL", "info"]),
+        "blasfeo_amd_solve_info": (bfa.SOLVE_INFO, ["iters", "path", "berr"]),
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HDR, "int main(void){"]
     for s, (_, fields) in checks.items():
@@ -141,3 +143,14 @@ def test_argument_errors_without_gpu():
     h = bfa.HINT(5000, 0, 0)
     assert L.blasfeo_amd_dpotrf_l_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
     assert L.blasfeo_amd_d_back_ric_trf_batched(4, 1, ctypes.c_void_p(8), None, None) == -1
+    # left-side solves need the bound of m; the mixed solve validates tol, max_iter and the hint
+    assert L.blasfeo_amd_dtrsm_llnn_batched(4, 1, ctypes.c_void_p(8), None, None) == -1
+    h = bfa.HINT(3000, 4, 0)
+    assert L.blasfeo_amd_dtrsm_lltn_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
+    ok = bfa.HINT(256, 4, 0)
+    f = L.blasfeo_amd_dposv_mixed_batched
+    assert f(4, 1, ctypes.c_void_p(8), 0.0, 10, ctypes.byref(ok), None) == -1
+    assert f(4, 1, ctypes.c_void_p(8), 1e-14, -1, ctypes.byref(ok), None) == -1
+    assert f(4, 1, ctypes.c_void_p(8), 1e-14, 10, None, None) == -1
+    assert f(6, 1, ctypes.c_void_p(8), 1e-14, 10, ctypes.byref(ok), None) == -1
+    assert f(4, 0, None, 1e-14, 10, ctypes.byref(ok), None) == 0

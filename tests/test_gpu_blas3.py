@@ -134,3 +134,42 @@ def test_trxm(gpu, prec, ps, op):
     g, gd, rr, rd = run_both(b, buf, argdt, recs, name, ocall, (160, 96, 96))
     compare_buffers(b, g, rr, "%s ps=%d" % (name, ps))
     assert (gd["use_dA"] == rd["use_dA"]).all()
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+@pytest.mark.parametrize("op", ["trsm_llnn", "trsm_lltn"])
+def test_trsm_left(gpu, ps, op):
+    """D = alpha A^{-1} B and D = alpha A^{-T} B (A lower m x m), with offsets,
+    the reciprocal-diagonal cache of A (valid, stale and offset cases), the
+    right-hand sides spread over several column blocks, and in-place D = B."""
+    rng = np.random.default_rng(400 + ps)
+    b = Batch("d", ps, seed=ps + 11)
+    recs = []
+    for r in range(18):
+        m = int(rng.integers(1, 260)) if r else 256
+        n = int(rng.integers(1, 40)) if r else 4
+        ai, bi, di = (int(x) for x in rng.integers(0, 2 * ps, 3))
+        aj, bj, dj = (int(x) for x in rng.integers(0, 3, 3))
+        if r % 3 == 0:
+            ai = aj = 0
+        Ad = np.tril(rng.standard_normal((ai + m, aj + m)) * 0.1)
+        for t in range(m):
+            Ad[ai + t, aj + t] = 1.0 + rng.uniform(0.5, 1.5)
+        A = b.mat(ai + m, aj + m, "data", Ad)
+        B = b.mat(bi + m, bj + n)
+        if r % 5 == 4:
+            D, di, dj = B, bi, bj
+        else:
+            D = b.mat(di + m, dj + n, "const", -1.0)
+        recs.append(dict(m=m, n=n, ai=ai, aj=aj, bi=bi, bj=bj, di=di, dj=dj,
+                         alpha=float(rng.choice([1.0, -0.5, 2.0])), sA=A, sB=B, sD=D))
+    buf = b.host()
+    f = oracle.fn("d", op)
+
+    def ocall(h, r):
+        f(ps, r["m"], r["n"], r["alpha"], h.desc_addr(r["sA"]), r["ai"], r["aj"],
+          h.desc_addr(r["sB"]), r["bi"], r["bj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    g, gd, rr, rd = run_both(b, buf, bfa.DTRSM_ARG, recs, "d" + op, ocall, (260, 40, 0))
+    compare_buffers(b, g, rr, "d%s ps=%d" % (op, ps))
+    assert (gd["use_dA"] == rd["use_dA"]).all()

@@ -129,6 +129,61 @@ def test_trsm_uses_cached_dA():
 
 
 @pytest.mark.parametrize("ps", [4, 8])
+@pytest.mark.parametrize("op", ["trsm_llnn", "trsm_lltn"])
+def test_trsm_left_exact_identity_residual(ps, op):
+    """Left-side solves: A = I gives alpha*B exactly; a unit-lower integer A
+    with B = A X (or A^T X) for integer X is solved exactly (every partial
+    sum is an integer), which fails for a wrong index, a dropped term or a
+    transposed operand; a random well-conditioned A meets the residual
+    identity; offsets leave the cache invalid (use_dA = 0)."""
+    rng = np.random.default_rng(11 + ps)
+    f = oracle.fn("d", op)
+    m, n = 19, 5
+    B = rng.standard_normal((m, n))
+    lay, hb = _hb([np.eye(m), B, np.zeros((m, n))], ps)
+    f(ps, m, n, -2.0, hb.desc_addr(0), 0, 0, hb.desc_addr(1), 0, 0, hb.desc_addr(2), 0, 0)
+    np.testing.assert_array_equal(_get(lay, hb, 2), -2.0 * B)
+    assert hb.desc[0]["use_dA"] == m
+    A = np.tril(rng.integers(-3, 4, (m, m)).astype(float), -1) + np.eye(m)
+    X = rng.integers(-5, 6, (m, n)).astype(float)
+    Bx = (A if op == "trsm_llnn" else A.T) @ X
+    lay, hb = _hb([A, Bx, np.zeros((m, n))], ps)
+    f(ps, m, n, 1.0, hb.desc_addr(0), 0, 0, hb.desc_addr(1), 0, 0, hb.desc_addr(2), 0, 0)
+    np.testing.assert_array_equal(_get(lay, hb, 2), X)
+    Lw = rng.standard_normal((m + 3, m + 2))  # only the lower triangle of the block is read
+    Lw[3:, 2:] = np.tril(Lw[3:, 2:]) * 0.3 + 2 * np.eye(m)
+    lay, hb = _hb([Lw, B, np.zeros((m, n))], ps)
+    f(ps, m, n, 0.5, hb.desc_addr(0), 3, 2, hb.desc_addr(1), 0, 0, hb.desc_addr(2), 0, 0)
+    T = np.tril(Lw[3:, 2:])
+    Y = _get(lay, hb, 2)
+    np.testing.assert_allclose((T if op == "trsm_llnn" else T.T) @ Y, 0.5 * B, rtol=0, atol=1e-13)
+    assert hb.desc[0]["use_dA"] == 0
+    np.testing.assert_array_equal(lay.dview(hb.buf, 0)[:m], 1.0 / np.diag(T))
+
+
+def test_trsm_lltn_summation_order():
+    """The reference's trsm_lltn subtracts the in-pair term last for the upper
+    row of each bottom-up pair (blasfeo_ref/x_blas3_ref.c:730-744).  With
+    values chosen so that this order changes the rounded result, the oracle
+    matches the reference's order and not plain ascending order."""
+    # m = 4: pairs (2,3) then (0,1); row 0 subtracts k = 2, 3 first and k = 1 last
+    m = 4
+    A = np.eye(m)
+    A[1, 0] = 1.0
+    A[2, 0] = 1.0
+    A[3, 0] = 1.0
+    b = np.array([[1.0], [1e16], [-1e16], [1.0]])
+    lay, hb = _hb([A, b, np.zeros((m, 1))])
+    oracle.fn("d", "trsm_lltn")(4, m, 1, 1.0, hb.desc_addr(0), 0, 0, hb.desc_addr(1), 0, 0, hb.desc_addr(2), 0, 0)
+    x = _get(lay, hb, 2)[:, 0]
+    # reference order for row 0: ((1 - x2) - x3) - x1 = ((1 + 1e16) - 1) - 1e16
+    ref = ((1.0 - x[2]) - x[3]) - x[1]
+    asc = ((1.0 - x[1]) - x[2]) - x[3]
+    assert ref != asc
+    assert x[0] == ref
+
+
+@pytest.mark.parametrize("ps", [4, 8])
 def test_potrf_closed_forms_and_backward_error(ps):
     rng = np.random.default_rng(4)
     d = rng.uniform(1, 5, 10)

@@ -148,7 +148,7 @@ def main():
     ap.add_argument("--cpu-max-instances", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="riccati",
-                    choices=("riccati", "potrf32", "gemm_syrk", "getrf", "sgetrf"),
+                    choices=("riccati", "potrf32", "gemm_syrk", "getrf", "sgetrf", "posv256"),
                     help="riccati (default, the headline); the others measure the remaining "
                          "batched configurations")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -177,6 +177,8 @@ def main():
         log("copied %d bytes x3" % n)
         return
 
+    if args.workload == "posv256":
+        return run_posv(args, torch, bfa)
     if args.workload != "riccati":
         return run_other(args, torch, bfa)
     return run_riccati(args, torch, bfa)
@@ -458,6 +460,163 @@ def run_other(args, torch, bfa):
            "roofline": {"bound": bound, "achieved": achieved, "peak": peak, "unit": punit,
                         "frac": achieved / peak, "traffic": None, "algorithmic_bytes": algo,
                         "flops_per_launch": flops, "kernel": kernel},
+           "cpu_baseline": cpu, "checks": checks}
+    print(json.dumps(out), flush=True)
+
+
+def _cn(ps, n):
+    pld = 8 if ps == 8 else 4
+    return (n + pld - 1) // pld * pld
+
+
+def _pm_pack(torch, A, ps):
+    """(batch, m, n) dense -> (batch, m*cn) panel-major FP64 payload (m a multiple of ps)."""
+    b, m, n = A.shape
+    cn = _cn(ps, n)
+    if cn != n:
+        A = torch.nn.functional.pad(A, (0, cn - n))
+    return A.reshape(b, m // ps, ps, cn).permute(0, 1, 3, 2).reshape(b, m * cn)
+
+
+def _pm_unpack(torch, flat, ps, m, n):
+    b = flat.shape[0]
+    cn = _cn(ps, n)
+    return flat[:, : m * cn].reshape(b, m // ps, cn, ps).permute(0, 1, 3, 2).reshape(b, m, cn)[:, :, :n]
+
+
+def _posv_berr(torch, A, X, B):
+    """Normwise backward error per instance: max_c |b_c - A x_c|_inf / (|A|_inf |x_c|_inf + |b_c|_inf)."""
+    R = B - torch.bmm(A, X.contiguous())
+    an = A.abs().sum(2).amax(1, keepdim=True)
+    den = an * X.abs().amax(1) + B.abs().amax(1)
+    return (R.abs().amax(1) / den).amax(1)
+
+
+def run_posv(args, torch, bfa):
+    """2048 SPD solves, n = 256, 4 right-hand sides: the mixed-precision entry
+    (FP32 factor in LDS + FP64 refinement) against the plain FP64 sequence
+    dpotrf_l + dtrsm_llnn + dtrsm_lltn on the same inputs, both to the
+    backward error each reaches (the mixed one refines to tol = 1e-14)."""
+    from blasfeo_amd.device import to_device
+    import oracle
+    ps = args.ps
+    batch, n, nrhs = 2048, 256, 4
+    tol, max_iter = 1e-14, 10
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(args.seed)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    dev = "cuda"
+    f64 = torch.float64
+    # A = Q diag(lambda) Q^T, Q Haar (QR of N(0,1) with the sign fix), lambda
+    # geometric from 1 to 1/cond, cond = 10^U[2,8]; B ~ N(0,1)
+    A = torch.empty((batch, n, n), dtype=f64, device=dev)
+    for b0 in range(0, batch, 256):
+        G = torch.randn((256, n, n), dtype=f64, device=dev, generator=gen)
+        Q, R = torch.linalg.qr(G)
+        Q = Q * torch.sign(torch.diagonal(R, dim1=1, dim2=2)).unsqueeze(1)
+        lc = torch.rand(256, dtype=f64, device=dev, generator=gen) * 6 + 2
+        lam = torch.pow(10.0, -lc[:, None] * torch.arange(n, dtype=f64, device=dev)[None, :] / (n - 1))
+        Ab = torch.bmm(Q * lam[:, None, :], Q.transpose(1, 2).contiguous())
+        A[b0:b0 + 256] = (Ab + Ab.transpose(1, 2)) / 2
+        del G, Q, R, Ab
+    Bd = torch.randn((batch, n, nrhs), dtype=f64, device=dev, generator=gen)
+    log("posv256: inputs generated")
+
+    def pm_buffer(rows, cols, dense=None):
+        d, off, nb = bfa.bulk_layout(np.float64, ps, [rows] * batch, [cols] * batch, 0)
+        buf = torch.zeros(max(nb, 8) // 8, dtype=f64, device=dev)
+        stride = int(off[1]) // 8 if batch > 1 else nb // 8
+        if dense is not None:
+            buf.view(batch, stride)[:, : rows * _cn(ps, cols)] = _pm_pack(torch, dense, ps)
+        desc = to_device(bfa.bulk_layout(np.float64, ps, [rows] * batch, [cols] * batch, buf.data_ptr())[0])
+        return buf, desc, stride
+
+    bufA, dA, sA = pm_buffer(n, n, A)
+    bufB, dB, sB = pm_buffer(n, nrhs, Bd)
+    bufX, dX, sX = pm_buffer(n, nrhs)
+    bufL, dL, _ = pm_buffer(n, n)
+    bufX2, dX2, _ = pm_buffer(n, nrhs)
+    bufL2, dL2, _ = pm_buffer(n, n)
+    idx = np.arange(batch, dtype=np.uint64) * bfa.DMAT.itemsize
+    # FP64 sequence
+    fa = _records(bfa.DFACT_ARG, batch)
+    fa["m"], fa["n"] = n, n
+    fa["sC"], fa["sD"] = dA.data_ptr() + idx, dL.data_ptr() + idx
+    t1 = _records(bfa.DTRSM_ARG, batch)
+    t1["m"], t1["n"], t1["alpha"] = n, nrhs, 1.0
+    t1["sA"], t1["sB"], t1["sD"] = dL.data_ptr() + idx, dB.data_ptr() + idx, dX.data_ptr() + idx
+    t2 = t1.copy()
+    t2["sB"] = t2["sD"]
+    dfa, dt1, dt2 = to_device(fa), to_device(t1), to_device(t2)
+    # mixed entry
+    info = torch.zeros(batch * bfa.SOLVE_INFO.itemsize, dtype=torch.uint8, device=dev)
+    pr = _records(bfa.DPOSV_ARG, batch)
+    pr["n"], pr["nrhs"] = n, nrhs
+    pr["sA"], pr["sB"] = dA.data_ptr() + idx, dB.data_ptr() + idx
+    pr["sX"], pr["sL"] = dX2.data_ptr() + idx, dL2.data_ptr() + idx
+    pr["info"] = info.data_ptr() + np.arange(batch, dtype=np.uint64) * bfa.SOLVE_INFO.itemsize
+    dpr = to_device(pr)
+    hint = (n, nrhs, 0)
+
+    def fp64():
+        bfa.call("dpotrf_l", ps, batch, dfa.data_ptr(), (n, n, 0), sh)
+        bfa.call("dtrsm_llnn", ps, batch, dt1.data_ptr(), hint, sh)
+        bfa.call("dtrsm_lltn", ps, batch, dt2.data_ptr(), hint, sh)
+
+    def mixed():
+        bfa.posv_mixed(ps, batch, dpr.data_ptr(), tol, max_iter, hint, sh)
+
+    ms64 = time_steps(torch, stream, fp64, args.steps, args.warmup)
+    log("posv256 fp64 sequence: %.3f ms/step" % ms64)
+    msmx = time_steps(torch, stream, mixed, args.steps, args.warmup)
+    log("posv256 mixed: %.3f ms/step" % msmx)
+    X64 = _pm_unpack(torch, bufX.view(batch, sX), ps, n, nrhs)
+    Xmx = _pm_unpack(torch, bufX2.view(batch, sX), ps, n, nrhs)
+    be64 = _posv_berr(torch, A, X64, Bd)
+    bemx = _posv_berr(torch, A, Xmx, Bd)
+    inf = info.cpu().numpy().view(bfa.SOLVE_INFO)
+    paths = np.bincount(inf["path"], minlength=2)
+    iters = inf["iters"][inf["path"] == 0]
+    checks = {"berr_max_mixed": float(bemx.max()), "berr_max_fp64": float(be64.max()),
+              "berr_target": tol, "mixed_path_counts": {"fp32_refined": int(paths[0]), "fp64_fallback": int(paths[1])},
+              "refinement_iters_mean": float(iters.mean()) if iters.size else None,
+              "refinement_iters_max": int(iters.max()) if iters.size else None,
+              "pass": bool(bemx.max() <= 1.5e-14 and be64.max() <= 1.5e-14)}
+    log("posv256 checks: %s" % json.dumps(checks))
+    flops_fact = batch * n ** 3 / 3.0
+    value = batch / (msmx * 1e-3)
+    v64 = batch / (ms64 * 1e-3)
+    achieved = flops_fact / (msmx * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        ns = min(batch, 64)
+        hA = bufA.view(batch, sA)[:ns].cpu().numpy().copy()
+        hB = bufB.view(batch, sB)[:ns].cpu().numpy().copy()
+        hX = np.zeros_like(hB)
+        hL = np.zeros_like(hA)
+        def hdesc(arr, rows, cols):
+            base = arr.ctypes.data
+            return bfa.bulk_layout(np.float64, ps, [rows] * ns, [cols] * ns, base)[0]
+        DA, DB, DX, DL = hdesc(hA, n, n), hdesc(hB, n, nrhs), hdesc(hX, n, nrhs), hdesc(hL, n, n)
+        hi = np.arange(ns, dtype=np.uint64) * bfa.DMAT.itemsize
+        hr = _records(bfa.DPOSV_ARG, ns)
+        hr["n"], hr["nrhs"] = n, nrhs
+        hr["sA"], hr["sB"], hr["sX"], hr["sL"] = (DA.ctypes.data + hi, DB.ctypes.data + hi,
+                                                  DX.ctypes.data + hi, DL.ctypes.data + hi)
+        cpu = cpu_time(lambda: oracle.lib().or_dposv_batch(ps, ns, hr.ctypes.data), ns, args.cpu_seconds,
+                       "solves/s", "the first %d instances (potrf_l + trsm_llnn + trsm_lltn, FP64)" % ns)
+    wl = {"workload": "dposv_mixed_batched vs dpotrf_l+dtrsm_llnn+dtrsm_lltn", "batch": batch, "n": n,
+          "nrhs": nrhs, "ps": ps, "cond": "10^U[2,8]", "tol": tol, "max_iter": max_iter}
+    out = {"metric": METRIC, "value": value, "unit": "solves/s (mixed precision, berr <= 1e-14)",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": msmx,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+           "data": "synthetic, seeded (%d)" % args.seed, "config": wl,
+           "fp64_solves_per_s": v64, "fp64_ms_per_step": ms64, "mixed_over_fp64": value / v64,
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
+                        "frac": achieved / 157.3, "traffic": None,
+                        "flops_per_launch": flops_fact, "kernel": "k_posv_mixed<%d>" % ps,
+                        "note": "factorisation flops n^3/3 per instance over the whole mixed step, against the FP32 matrix-core peak"},
            "cpu_baseline": cpu, "checks": checks}
     print(json.dumps(out), flush=True)
 

@@ -73,6 +73,236 @@ __global__ __launch_bounds__(NT) void k_potrf_lds(const typename FactArg<T>::typ
 		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? m : 0;
 }
 
+// ---------------------------------------------------------------------------
+// potrf_l for 128 < n <= 256: left-looking over 64-column block panels held in
+// LDS as 16x16 tiles (col-major inside a tile, tile (r, jj) of the panel at
+// (4r + jj) * 256), one 8-wave workgroup per instance.
+//   1. the panel's lower part is staged from C (identity beyond n);
+//   2. the earlier block columns are applied on the matrix cores,
+//      P(r, jj) -= L(rows r, 0:c0) L(rows jj, 0:c0)^T, with L read from D
+//      (one wave per tile row, the A operand shared by the row's 4 tiles);
+//   3. the panel is factored right-looking in LDS: wave 0 factors the
+//      diagonal tile in registers (as chol_lds), every thread then solves one
+//      row below it against L_jj with the reciprocal diagonal, and the
+//      panel's remaining tile columns get a rank-16 matrix-core update;
+//   4. L and the reciprocal diagonal are written to D.
+// Compared with the 16-column left-looking form this reads the earlier
+// columns once per 64 columns instead of once per 16.
+// ---------------------------------------------------------------------------
+constexpr int PT_NT = 512;
+constexpr int PT_NW = PT_NT / WAVE;
+constexpr int PT_NMAX = 256;
+
+template <typename T>
+__host__ __device__ inline size_t potrf_tiles_lds_bytes()
+{
+	return sizeof(T) * (64 * 256 + 16 * 17 + 16);
+}
+
+template <typename T, int PS>
+__global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>::type *__restrict__ args, int nmax)
+{
+	typedef typename Vec4<T>::type V4;
+	extern __shared__ unsigned char smem_raw[];
+	T *P = reinterpret_cast<T *>(smem_raw);
+	T *L11 = P + 64 * 256;
+	T *invs = L11 + 16 * 17;
+	const typename FactArg<T>::type &a = args[blockIdx.x];
+	const int n = a.m;
+	if (n <= 0 || n > nmax)
+		return;
+	const int lane = lane_id();
+	const int w = wave_id();
+	const int nb = (n + 15) / 16;
+	const PM<T, PS> gC = view<T, PS>(a.sC);
+	PM<T, PS> gD = view<T, PS>(a.sD);
+	const int ci = a.ci, cj = a.cj, di = a.di, dj = a.dj;
+	T *dD = a.sD->dA;
+
+	for (int J0 = 0; J0 < nb; J0 += 4)
+	{
+		const int R = nb - J0;           // tile rows of the panel
+		const int C = min(4, R);         // tile columns of the panel
+		const int c0 = 16 * J0;          // first column of the panel
+		__syncthreads();
+		// 1. stage the panel (lower part; identity beyond n)
+		for (int t = w; t < R * 4; t += PT_NW)
+		{
+			const int r = t >> 2, jj = t & 3;
+			if (jj >= C || r < jj)
+				continue;
+			T *PT = P + t * 256;
+			const int i = lane & 15;
+			const int gr = c0 + 16 * r + i;
+#pragma unroll
+			for (int k = 0; k < 4; k++)
+			{
+				const int j = 4 * k + (lane >> 4);
+				const int gc = c0 + 16 * jj + j;
+				T v = (gr == gc) ? T(1) : T(0);
+				if (gr < n && gc < n && gr >= gc)
+					v = gC.at(ci + gr, cj + gc);
+				PT[j * 16 + i] = v;
+			}
+		}
+		__syncthreads();
+		// 2. apply the earlier block columns (L from D)
+		if (J0 > 0)
+		{
+			for (int r = w; r < R; r += PT_NW)
+			{
+				V4 acc[4];
+#pragma unroll
+				for (int jj = 0; jj < 4; jj++)
+				{
+					acc[jj] = V4{0, 0, 0, 0};
+					if (jj < C && jj <= r)
+#pragma unroll
+						for (int q = 0; q < 4; q++)
+							acc[jj][q] = P[(r * 4 + jj) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)];
+				}
+				const int ar = c0 + 16 * r + (lane & 15);
+				const bool aok = ar < n;
+
+				for (int kk = 0; kk < c0; kk += 4)
+				{
+					const int kq = kk + (lane >> 4);
+					const T av = aok ? gD.at(di + ar, dj + kq) : T(0);
+#pragma unroll
+					for (int jj = 0; jj < 4; jj++)
+					{
+						if (jj < C && jj <= r)
+						{
+							const int br = c0 + 16 * jj + (lane & 15);
+							const T bv = br < n ? gD.at(di + br, dj + kq) : T(0);
+							acc[jj] = mfma16(-av, bv, acc[jj]);
+						}
+					}
+				}
+#pragma unroll
+				for (int jj = 0; jj < 4; jj++)
+					if (jj < C && jj <= r)
+#pragma unroll
+						for (int q = 0; q < 4; q++)
+							P[(r * 4 + jj) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)] = acc[jj][q];
+			}
+			__syncthreads();
+		}
+		// 3. factor the panel
+		for (int jj = 0; jj < C; jj++)
+		{
+			T *D = P + (jj * 4 + jj) * 256;
+			if (w == 0)
+			{
+				T x[16];
+#pragma unroll
+				for (int j = 0; j < 16; j++)
+					x[j] = lane < 16 ? D[j * 16 + lane] : T(0);
+#pragma unroll
+				for (int j = 0; j < 16; j++)
+				{
+					const T c = readlane(x[j], j);
+					const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
+					if (lane > j)
+						x[j] *= inv;
+					else if (lane == j)
+						x[j] = c * inv;
+					if (lane == 0)
+						invs[j] = inv;
+#pragma unroll
+					for (int jn = j + 1; jn < 16; jn++)
+					{
+						const T l = readlane(x[j], jn);
+						if (lane >= jn)
+							x[jn] -= x[j] * l;
+					}
+				}
+				if (lane < 16)
+#pragma unroll
+					for (int j = 0; j < 16; j++)
+					{
+						D[j * 16 + lane] = x[j];
+						L11[lane * 17 + j] = x[j];
+					}
+			}
+			__syncthreads();
+			// rows below the diagonal tile: x_j = (x_j - sum_{l<j} x_l L_jl) * inv_j
+			{
+				const int nrow = (R - jj - 1) * 16;
+				for (int q = threadIdx.x; q < nrow; q += PT_NT)
+				{
+					asm volatile("" ::: "memory");
+					const int r = jj + 1 + (q >> 4), i = q & 15;
+					T *row = P + (r * 4 + jj) * 256 + i;
+					T x[16];
+#pragma unroll
+					for (int j = 0; j < 16; j++)
+						x[j] = row[j * 16];
+#pragma unroll
+					for (int j = 0; j < 16; j++)
+					{
+						T c = x[j];
+#pragma unroll
+						for (int l = 0; l < j; l++)
+							c -= x[l] * L11[j * 17 + l];
+						x[j] = c * invs[j];
+					}
+#pragma unroll
+					for (int j = 0; j < 16; j++)
+						row[j * 16] = x[j];
+				}
+			}
+			__syncthreads();
+			// rank-16 update of the panel's later tile columns
+			{
+				const int ntask = (C - jj - 1) * R;
+				for (int t = w; t < ntask; t += PT_NW)
+				{
+					const int j2 = jj + 1 + t / R, r = t % R;
+					if (r < j2)
+						continue;
+					V4 acc;
+#pragma unroll
+					for (int q = 0; q < 4; q++)
+						acc[q] = P[(r * 4 + j2) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)];
+#pragma unroll
+					for (int kk = 0; kk < 16; kk += 4)
+					{
+						const int o = (kk + (lane >> 4)) * 16 + (lane & 15);
+						acc = mfma16(-P[(r * 4 + jj) * 256 + o], P[(j2 * 4 + jj) * 256 + o], acc);
+					}
+#pragma unroll
+					for (int q = 0; q < 4; q++)
+						P[(r * 4 + j2) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)] = acc[q];
+				}
+			}
+			if (threadIdx.x < 16 && c0 + 16 * jj + threadIdx.x < n)
+				dD[c0 + 16 * jj + threadIdx.x] = invs[threadIdx.x];
+			__syncthreads();
+		}
+		// 4. write L (lower part, rows and columns < n)
+		for (int t = w; t < R * 4; t += PT_NW)
+		{
+			const int r = t >> 2, jj = t & 3;
+			if (jj >= C || r < jj)
+				continue;
+			const T *PT = P + t * 256;
+			const int i = lane & 15;
+			const int gr = c0 + 16 * r + i;
+#pragma unroll
+			for (int k = 0; k < 4; k++)
+			{
+				const int j = 4 * k + (lane >> 4);
+				const int gc = c0 + 16 * jj + j;
+				if (gr < n && gc < n && gr >= gc)
+					gD.at(di + gr, dj + gc) = PT[j * 16 + i];
+			}
+		}
+	}
+	if (threadIdx.x == 0)
+		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? n : 0;
+}
+
 template <typename T> struct SPArg;
 template <> struct SPArg<double> { typedef blasfeo_amd_dsyrk_dpotrf_arg type; };
 
@@ -338,6 +568,17 @@ static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args
 			else
 				hipLaunchKernelGGL((k_potrf_lds<T, 16, NTL>), dim3(batch), dim3(NTL), shm2, st, args, np);
 		}
+		return check_launch(name);
+	}
+	if (MODE == 0 && mmax <= PT_NMAX)
+	{
+		const size_t shm3 = potrf_tiles_lds_bytes<T>();
+		if (ps == 4)
+			hipLaunchKernelGGL((k_potrf_tiles<T, 4>), dim3(batch), dim3(PT_NT), shm3, st, args, mmax);
+		else if (ps == 8)
+			hipLaunchKernelGGL((k_potrf_tiles<T, 8>), dim3(batch), dim3(PT_NT), shm3, st, args, mmax);
+		else
+			hipLaunchKernelGGL((k_potrf_tiles<T, 16>), dim3(batch), dim3(PT_NT), shm3, st, args, mmax);
 		return check_launch(name);
 	}
 	const size_t shm = sizeof(T) * PW * ((mmax + 15) / 16 * 16 + 1);

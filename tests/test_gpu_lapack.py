@@ -87,6 +87,43 @@ def test_potrf(gpu, prec, ps, op):
     assert (gd["use_dA"] == rd["use_dA"]).all()
 
 
+@pytest.mark.parametrize("prec", ["d", "s"])
+@pytest.mark.parametrize("ps", [4, 8])
+def test_potrf_tiles(gpu, prec, ps):
+    """potrf_l with a size bound in (128, 256]: 64-column block panels factored
+    in LDS as 16x16 tiles, earlier block columns applied from D; sizes around
+    the tile and panel edges, offsets, in place, a non-SPD pivot."""
+    rng = np.random.default_rng(370 + ps)
+    b = Batch(prec, ps, seed=9)
+    recs = []
+    sizes = [1, 16, 17, 63, 64, 65, 127, 128, 129, 143, 192, 200, 255, 256]
+    for r, n in enumerate(sizes):
+        ci, cj = (0, 0) if r % 2 == 0 else (int(rng.integers(0, 2 * ps)), int(rng.integers(0, 3)))
+        di, dj = (0, 0) if r % 3 == 0 else (int(rng.integers(0, 2 * ps)), int(rng.integers(0, 3)))
+        S = np.zeros((ci + n, cj + n))
+        M = _spd(rng, n)
+        if n == 143:
+            M[100, 100] = -1.0  # pivot 100 fails: zero column, dA = 0
+        S[ci:, cj:] = M
+        C = b.mat(ci + n, cj + n, "data", S)
+        if n == 200:
+            D, di, dj = C, ci, cj
+        else:
+            D = b.mat(di + n, dj + n, "const", -1.0)
+        recs.append(dict(m=n, n=n, ci=ci, cj=cj, di=di, dj=dj, sC=C, sD=D, ipiv=0))
+    buf = b.host()
+    f = oracle.fn(prec, "potrf_l")
+
+    def ocall(h, r):
+        f(ps, r["m"], h.desc_addr(r["sC"]), r["ci"], r["cj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    tol = 1e-11 if prec == "d" else 2e-3
+    argdt = bfa.DFACT_ARG
+    g, gd, rr, rd = run_both(b, buf, argdt, recs, prec + "potrf_l", ocall, (256, 256, 0))
+    compare_buffers(b, g, rr, "%spotrf_l tiles ps=%d" % (prec, ps), tol=tol)
+    assert (gd["use_dA"] == rd["use_dA"]).all()
+
+
 @pytest.mark.parametrize("ps", [4, 8])
 def test_potrf_inplace_and_failures(gpu, ps):
     """In-place factorisation, non-SPD pivots (zero column + dA = 0) and NaN."""

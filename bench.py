@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-max-instances", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="riccati: time replays of the batched sweep captured in a HIP graph")
     ap.add_argument("--workload", default="riccati",
                     choices=("riccati", "potrf32", "gemm_syrk", "getrf", "sgetrf", "posv256"),
                     help="riccati (default, the headline); the others measure the remaining "
@@ -204,15 +206,21 @@ def run_riccati(args, torch, bfa):
             dev.run(sh)
     torch.cuda.synchronize()
 
-    log("timing %d steps" % args.steps)
+    step = lambda: dev.run(sh)
+    if args.graph:
+        graph = dev.capture(stream)
+        torch.cuda.synchronize()
+        step = graph.replay
+    log("timing %d steps%s" % (args.steps, " (graph replays)" if args.graph else ""))
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        dev.run(sh)
-    ev1.record(stream)
+    with torch.cuda.stream(stream):
+        ev0.record(stream)
+        for _ in range(args.steps):
+            step()
+        ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = ev0.elapsed_time(ev1) / args.steps
@@ -252,7 +260,8 @@ def run_riccati(args, torch, bfa):
         "data": "synthetic, seeded (numpy PCG64 seed %d): A = I + 0.1 N(0,1), B ~ N(0,1), Q = R = I" % args.seed,
         "config": {"workload": "riccati_trf_batched", "batch": args.batch, "horizon": args.horizon,
                    "nx": "U{8..64}", "nu": "U{2..16}", "ps": args.ps, "precision": "fp64",
-                   "parallelism": "one workgroup per instance, 1 GPU"},
+                   "parallelism": "one workgroup per instance, 1 GPU",
+                   "launch": "hip graph replay" if args.graph else "eager"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "algorithmic_bytes": algo_bytes, "flops_per_launch": flops,

@@ -123,6 +123,17 @@ class DeviceRiccati:
         call("d_back_ric_trf", self.rb.ps, self.rb.batch, self.args.data_ptr(),
              (self.rb.max_nk, self.rb.max_nx, 0), stream)
 
+    def capture(self, stream):
+        """The batched sweep (plan and class launches) captured into a CUDA/HIP
+        graph on ``stream``.  Run it once eagerly on the same stream first: the
+        library allocates its per-stream workspace on first use, which a
+        capture cannot record."""
+        import torch
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            self.run(stream.cuda_stream)
+        return g
+
     def L(self, i, k):
         buf = self.outbuf.cpu().numpy()
         return self.rb.out.get(buf, self.rb.i_L[i][k])

@@ -209,63 +209,92 @@ __device__ bool posv_factor(const PosvLds &L, int nb)
 }
 
 // R <- (L L^T)^{-1} R in FP32 with the factor in LDS (diagonal tiles hold
-// Z_K = L_KK^{-1}).  Starts and ends with a barrier.
+// Z_K = L_KK^{-1}).  One workgroup barrier per 16-row block: in step K wave 0
+// applies the previous block's result to block K and multiplies by Z_K (the
+// chain), while waves 1.. apply the previous block's result to the blocks
+// beyond K.  Starts and ends with a barrier.
 __device__ void posv_solve(const PosvLds &L, int nb)
 {
 	const int t = threadIdx.x;
 	float *R = L.R;
 	__syncthreads();
-	// forward: y_K = Z_K r_K ; r_I -= L(I,K) y_K
+	// forward: y_K = Z_K (r_K - L(K,K-1) y_{K-1}); blocks I > K take -= L(I,K-1) y_{K-1}
 	for (int K = 0; K < nb; K++)
 	{
-		const float *Z = L.F + tix(K, K) * 256;
 		if (t < 64)
 		{
 			const int i = t & 15, c = t >> 4;
+			float v = R[(16 * K + i) * PV_R + c];
+			if (K > 0)
+			{
+				const float *LK = L.F + tix(K, K - 1) * 256 + i;
+#pragma unroll
+				for (int k = 0; k < 16; k++)
+					v -= LK[k * 16] * R[(16 * (K - 1) + k) * PV_R + c];
+			}
+			R[(16 * K + i) * PV_R + c] = v;
+			wave_sync();
+			const float *Z = L.F + tix(K, K) * 256;
 			float s = 0.0f;
 #pragma unroll
 			for (int k = 0; k < 16; k++)
 				s += Z[k * 16 + i] * R[(16 * K + k) * PV_R + c];
+			wave_sync();
 			R[(16 * K + i) * PV_R + c] = s;
 		}
-		__syncthreads();
-		const int np = (nb - K - 1) * 16 * PV_R;
-		for (int p = t; p < np; p += PV_NT)
+		else if (K > 0)
 		{
-			const int c = p & 3, ir = 16 * (K + 1) + (p >> 2);
-			const float *LI = L.F + tix(ir >> 4, K) * 256 + (ir & 15);
-			float s = 0.0f;
+			const int np = (nb - K - 1) * 16 * PV_R;
+			for (int p = t - 64; p < np; p += PV_NT - 64)
+			{
+				const int c = p & 3, ir = 16 * (K + 1) + (p >> 2);
+				const float *LI = L.F + tix(ir >> 4, K - 1) * 256 + (ir & 15);
+				float s = 0.0f;
 #pragma unroll
-			for (int k = 0; k < 16; k++)
-				s += LI[k * 16] * R[(16 * K + k) * PV_R + c];
-			R[ir * PV_R + c] -= s;
+				for (int k = 0; k < 16; k++)
+					s += LI[k * 16] * R[(16 * (K - 1) + k) * PV_R + c];
+				R[ir * PV_R + c] -= s;
+			}
 		}
 		__syncthreads();
 	}
-	// backward: x_K = Z_K^T y_K ; y_J -= L(K,J)^T x_K
+	// backward: x_K = Z_K^T (y_K - L(K+1,K)^T x_{K+1}); blocks J < K take -= L(K+1,J)^T x_{K+1}
 	for (int K = nb - 1; K >= 0; K--)
 	{
-		const float *Z = L.F + tix(K, K) * 256;
 		if (t < 64)
 		{
 			const int i = t & 15, c = t >> 4;
+			float v = R[(16 * K + i) * PV_R + c];
+			if (K < nb - 1)
+			{
+				const float *LK = L.F + tix(K + 1, K) * 256 + i * 16;
+#pragma unroll
+				for (int k = 0; k < 16; k++)
+					v -= LK[k] * R[(16 * (K + 1) + k) * PV_R + c];
+			}
+			R[(16 * K + i) * PV_R + c] = v;
+			wave_sync();
+			const float *Z = L.F + tix(K, K) * 256;
 			float s = 0.0f;
 #pragma unroll
 			for (int k = 0; k < 16; k++)
 				s += Z[i * 16 + k] * R[(16 * K + k) * PV_R + c];
+			wave_sync();
 			R[(16 * K + i) * PV_R + c] = s;
 		}
-		__syncthreads();
-		const int np = K * 16 * PV_R;
-		for (int p = t; p < np; p += PV_NT)
+		else if (K < nb - 1)
 		{
-			const int c = p & 3, ir = p >> 2;
-			const float *LK = L.F + tix(K, ir >> 4) * 256 + (ir & 15) * 16;
-			float s = 0.0f;
+			const int np = K * 16 * PV_R;
+			for (int p = t - 64; p < np; p += PV_NT - 64)
+			{
+				const int c = p & 3, ir = p >> 2;
+				const float *LK = L.F + tix(K + 1, ir >> 4) * 256 + (ir & 15) * 16;
+				float s = 0.0f;
 #pragma unroll
-			for (int k = 0; k < 16; k++)
-				s += LK[k] * R[(16 * K + k) * PV_R + c];
-			R[ir * PV_R + c] -= s;
+				for (int k = 0; k < 16; k++)
+					s += LK[k] * R[(16 * (K + 1) + k) * PV_R + c];
+				R[ir * PV_R + c] -= s;
+			}
 		}
 		__syncthreads();
 	}

@@ -10,7 +10,8 @@
 //   (v_readlane broadcasts, reciprocal square roots) and inverts it in FP64;
 //   the FP32 inverse Z_K replaces the tile; the panel L(I,K) = A(I,K) Z_K^T
 //   and the trailing update A(I,J) -= L(I,K) L(J,K)^T run on the FP32 matrix
-//   cores (v_mfma_f32_16x16x4f32).
+//   cores (v_mfma_f32_16x16x4f32); the next diagonal tile is factored while
+//   the other waves apply the trailing update (look-ahead of one column).
 // Solves with the factor apply Z_K to the 16-row blocks, so each block step
 // is a small matrix-vector product instead of a 16-step substitution.
 // Refinement: r = b - A x is formed in FP64 from the FP64 A in global memory
@@ -92,8 +93,64 @@ __device__ __forceinline__ void block_max(double *v, double *red)
 	}
 }
 
-// In-LDS Cholesky of the nb x nb tile matrix.  Returns (uniformly) false if a
-// pivot was not positive.
+// Wave 0: FP64 Cholesky of the diagonal tile FKK (FP32 in LDS), replaced by
+// Z = L_KK^{-1} in FP32.  Lanes 0..15 hold row `lane`; L(jj,j) is broadcast
+// with v_readlane.  Sets *flag on a non-positive (or NaN) pivot.
+__device__ __forceinline__ void posv_diag(float *FKK, int *flag)
+{
+	const int lane = lane_id();
+	double x[16];
+#pragma unroll
+	for (int j = 0; j < 16; j++)
+		x[j] = lane < 16 ? (double)FKK[j * 16 + lane] : 0.0;
+	double iv[16];
+	bool bad = false;
+#pragma unroll
+	for (int j = 0; j < 16; j++)
+	{
+		const double c = readlane(x[j], j);
+		const bool ok = c > 0.0;
+		bad = bad || !ok;
+		const double inv = ok ? rsqrt_fast(c) : 0.0;
+		iv[j] = inv;
+		if (lane > j)
+			x[j] *= inv;
+		else if (lane == j)
+			x[j] = c * inv;
+#pragma unroll
+		for (int jj = j + 1; jj < 16; jj++)
+		{
+			const double l = readlane(x[j], jj);
+			if (lane >= jj)
+				x[jj] -= x[j] * l;
+		}
+	}
+	// Z = L_KK^{-1}: lane c computes column c by forward substitution
+	double z[16];
+#pragma unroll
+	for (int i = 0; i < 16; i++)
+	{
+		double s = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+		for (int k = 0; k < i; k++)
+			s -= readlane(x[k], i) * z[k];
+		z[i] = s * iv[i];
+	}
+	wave_sync();
+	if (lane < 16)
+#pragma unroll
+		for (int i = 0; i < 16; i++)
+			FKK[lane * 16 + i] = (float)z[i];
+	if (bad && lane == 0)
+		*flag = 1;
+}
+
+// In-LDS Cholesky of the nb x nb tile matrix, right-looking with a look-ahead
+// of one tile column: in step K all waves form the panel L(I,K) = A(I,K) Z_K^T;
+// then wave 0 updates the next diagonal tile and factors it (the FP64 chain)
+// while waves 1.. apply the rank-16 update to the rest of the trailing
+// matrix.  Two workgroup barriers per tile column.  Returns (uniformly) false
+// if a pivot was not positive.
 __device__ bool posv_factor(const PosvLds &L, int nb)
 {
 	typedef f4_t V4;
@@ -102,58 +159,13 @@ __device__ bool posv_factor(const PosvLds &L, int nb)
 	if (threadIdx.x == 0)
 		L.flag[0] = 0;
 	__syncthreads();
+	if (w == 0)
+		posv_diag(L.F, L.flag);
+	__syncthreads();
 	for (int K = 0; K < nb; K++)
 	{
-		float *FKK = L.F + tix(K, K) * 256;
-		if (w == 0)
-		{
-			// lanes 0..15 hold row `lane` of the diagonal tile
-			double x[16];
-#pragma unroll
-			for (int j = 0; j < 16; j++)
-				x[j] = lane < 16 ? (double)FKK[j * 16 + lane] : 0.0;
-			double iv[16];
-			bool bad = false;
-#pragma unroll
-			for (int j = 0; j < 16; j++)
-			{
-				const double c = readlane(x[j], j);
-				const bool ok = c > 0.0;
-				bad = bad || !ok;
-				const double inv = ok ? rsqrt_fast(c) : 0.0;
-				iv[j] = inv;
-				if (lane > j)
-					x[j] *= inv;
-				else if (lane == j)
-					x[j] = c * inv;
-#pragma unroll
-				for (int jj = j + 1; jj < 16; jj++)
-				{
-					const double l = readlane(x[j], jj);
-					if (lane >= jj)
-						x[jj] -= x[j] * l;
-				}
-			}
-			// Z = L_KK^{-1}: lane c computes column c by forward substitution
-			double z[16];
-#pragma unroll
-			for (int i = 0; i < 16; i++)
-			{
-				double s = (i == lane) ? 1.0 : 0.0;
-#pragma unroll
-				for (int k = 0; k < i; k++)
-					s -= readlane(x[k], i) * z[k];
-				z[i] = s * iv[i];
-			}
-			if (lane < 16)
-#pragma unroll
-				for (int i = 0; i < 16; i++)
-					FKK[lane * 16 + i] = (float)z[i];
-			if (bad && lane == 0)
-				L.flag[0] = 1;
-		}
-		__syncthreads();
-		// panel: L(I,K) = A(I,K) Z^T
+		const float *FKK = L.F + tix(K, K) * 256;
+		// panel: L(I,K) = A(I,K) Z_K^T
 		for (int I = K + 1 + w; I < nb; I += PV_NW)
 		{
 			float *FIK = L.F + tix(I, K) * 256;
@@ -167,12 +179,31 @@ __device__ bool posv_factor(const PosvLds &L, int nb)
 			*reinterpret_cast<V4 *>(FIK + (lane & 15) * 16 + 4 * (lane >> 4)) = acc;
 		}
 		__syncthreads();
-		// trailing update: tasks (J, group of up to 4 tiles I >= J)
+		if (K + 1 >= nb)
+			break;
+		if (w == 0)
 		{
+			// next diagonal tile: update, then factor
+			float *FN = L.F + tix(K + 1, K + 1) * 256;
+			const float *FNK = L.F + tix(K + 1, K) * 256;
+			V4 acc = *reinterpret_cast<const V4 *>(FN + (lane & 15) * 16 + 4 * (lane >> 4));
+#pragma unroll
+			for (int kk = 0; kk < 16; kk += 4)
+			{
+				const int o = (kk + (lane >> 4)) * 16 + (lane & 15);
+				acc = mfma16(-FNK[o], FNK[o], acc);
+			}
+			*reinterpret_cast<V4 *>(FN + (lane & 15) * 16 + 4 * (lane >> 4)) = acc;
+			wave_sync();
+			posv_diag(FN, L.flag);
+		}
+		else
+		{
+			// trailing update without tile (K+1, K+1): tasks (J, group of up to 4 tiles I >= J)
 			int ntask = 0;
 			for (int J = K + 1; J < nb; J++)
 				ntask += (nb - J + 3) / 4;
-			for (int task = w; task < ntask; task += PV_NW)
+			for (int task = w - 1; task < ntask; task += PV_NW - 1)
 			{
 				int J = K + 1, rem = task;
 				while (rem >= (nb - J + 3) / 4)
@@ -181,11 +212,12 @@ __device__ bool posv_factor(const PosvLds &L, int nb)
 					J++;
 				}
 				const int g = J + 4 * rem;
+				const int t0 = (J == K + 1 && g == K + 1) ? 1 : 0;   // skip the diagonal tile wave 0 owns
 				const float *FJK = L.F + tix(J, K) * 256;
 				V4 acc[4];
 #pragma unroll
 				for (int t = 0; t < 4; t++)
-					if (g + t < nb)
+					if (t >= t0 && g + t < nb)
 						acc[t] = *reinterpret_cast<const V4 *>(L.F + tix(g + t, J) * 256 + (lane & 15) * 16 + 4 * (lane >> 4));
 #pragma unroll
 				for (int kk = 0; kk < 16; kk += 4)
@@ -194,12 +226,12 @@ __device__ bool posv_factor(const PosvLds &L, int nb)
 					const float b = FJK[o];
 #pragma unroll
 					for (int t = 0; t < 4; t++)
-						if (g + t < nb)
+						if (t >= t0 && g + t < nb)
 							acc[t] = mfma16(-L.F[tix(g + t, K) * 256 + o], b, acc[t]);
 				}
 #pragma unroll
 				for (int t = 0; t < 4; t++)
-					if (g + t < nb)
+					if (t >= t0 && g + t < nb)
 						*reinterpret_cast<V4 *>(L.F + tix(g + t, J) * 256 + (lane & 15) * 16 + 4 * (lane >> 4)) = acc[t];
 			}
 		}

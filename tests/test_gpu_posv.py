@@ -137,8 +137,9 @@ def test_posv_fallback_matches_fp64_sequence(gpu, ps):
     o = out[1]
     assert o["info"]["berr"] == -1.0
     assert _berr(o["A"], o["X"], o["B"]) <= 1e-13
-    # max_iter = 0 and a size bound below n send everything to the FP64 path
-    cases = [(70, 2, _spd_cond(rng, 70, 1e3)), (300, 4, _spd_cond(rng, 300, 1e4))]
+    # max_iter = 0 and a size bound below n send everything to the FP64 path;
+    # cond 10 keeps X itself comparable elementwise at 1e-11
+    cases = [(70, 2, _spd_cond(rng, 70, 10.0)), (300, 4, _spd_cond(rng, 300, 10.0))]
     out = _run(ps, cases, 1e-14, 0, 320, 51 + ps)
     for k, o in enumerate(out):
         assert o["info"]["path"] == 1, (k, o["info"])

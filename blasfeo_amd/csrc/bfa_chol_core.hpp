@@ -163,8 +163,9 @@ __device__ void chol_instance(T *P, int LDP, T *L11, T *invs,
 namespace bfa {
 
 // ---------------------------------------------------------------------------
-// In-LDS right-looking blocked Cholesky of an n x n matrix held column-major
-// in LDS (M[c*ld + r], lower triangle used, n <= NMAX <= 128), in place: M <- L,
+// In-LDS right-looking blocked Cholesky of an n x n matrix held in LDS and
+// addressed through M(r, c) (column-major, or 16x16 tiles; lower triangle
+// used, n <= NMAX <= 128), in place: M <- L,
 // invd[j] = 1/sqrt(pivot) with the zero-pivot rule above (L_jj = c*invd[j]).
 // Rows and columns n .. ceil16(n)-1 must hold zeros on entry.
 // Per 16-column block: wave 0 factors the whole panel (diagonal block and the
@@ -174,8 +175,8 @@ namespace bfa {
 // block column of tiles at a time with independent accumulators.
 // Called by all NT threads of the workgroup; starts and ends with a barrier.
 // ---------------------------------------------------------------------------
-template <typename T, int NT = NTL, int NMAX = 128>
-__device__ void chol_lds(T *M, int ld, int n, T *invd)
+template <typename T, int NT, int NMAX, typename ACC>
+__device__ void chol_acc(const ACC M, int n, T *invd)
 {
 	typedef typename Vec4<T>::type V4;
 	constexpr int NW = NT / WAVE;
@@ -195,8 +196,8 @@ __device__ void chol_lds(T *M, int ld, int n, T *invd)
 #pragma unroll
 			for (int j = 0; j < 16; j++)
 			{
-				x0[j] = r0ok ? M[(c0 + j) * ld + c0 + lane] : T(0);
-				x1[j] = r1ok ? M[(c0 + j) * ld + c0 + 64 + lane] : T(0);
+				x0[j] = r0ok ? M(c0 + lane, c0 + j) : T(0);
+				x1[j] = r1ok ? M(c0 + 64 + lane, c0 + j) : T(0);
 			}
 #pragma unroll
 			for (int j = 0; j < 16; j++)
@@ -229,9 +230,9 @@ __device__ void chol_lds(T *M, int ld, int n, T *invd)
 			for (int j = 0; j < 16; j++)
 			{
 				if (r0ok)
-					M[(c0 + j) * ld + c0 + lane] = x0[j];
+					M(c0 + lane, c0 + j) = x0[j];
 				if (r1ok)
-					M[(c0 + j) * ld + c0 + 64 + lane] = x1[j];
+					M(c0 + 64 + lane, c0 + j) = x1[j];
 			}
 		}
 		__syncthreads();
@@ -257,19 +258,19 @@ __device__ void chol_lds(T *M, int ld, int n, T *invd)
 					if (g + t < nt)
 #pragma unroll
 						for (int r = 0; r < 4; r++)
-							acc[t][r] = M[(bj * 16 + (lane & 15)) * ld + (g + t) * 16 + acc_row<T>(lane, r)];
+							acc[t][r] = M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + (lane & 15));
 				}
 #pragma unroll
 				for (int kk = 0; kk < 16; kk += 4)
 				{
 					const int kq = c0 + kk + (lane >> 4);
-					const T b = M[kq * ld + bj * 16 + (lane & 15)];
+					const T b = M(bj * 16 + (lane & 15), kq);
 #pragma unroll
 					for (int t = 0; t < 4; t++)
 					{
 						if (g + t < nt)
 						{
-							const T a = M[kq * ld + (g + t) * 16 + (lane & 15)];
+							const T a = M((g + t) * 16 + (lane & 15), kq);
 							acc[t] = mfma16(-a, b, acc[t]);
 						}
 					}
@@ -280,12 +281,28 @@ __device__ void chol_lds(T *M, int ld, int n, T *invd)
 					if (g + t < nt)
 #pragma unroll
 						for (int r = 0; r < 4; r++)
-							M[(bj * 16 + (lane & 15)) * ld + (g + t) * 16 + acc_row<T>(lane, r)] = acc[t][r];
+							M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + (lane & 15)) = acc[t][r];
 				}
 			}
 		}
 		__syncthreads();
 	}
+}
+
+
+// Column-major LDS matrix: element (r, c) at p[c*ld + r].
+template <typename T>
+struct ColMajor
+{
+	T *p;
+	int ld;
+	__device__ __forceinline__ T &operator()(int r, int c) const { return p[c * ld + r]; }
+};
+
+template <typename T, int NT = NTL, int NMAX = 128>
+__device__ __forceinline__ void chol_lds(T *M, int ld, int n, T *invd)
+{
+	chol_acc<T, NT, NMAX>(ColMajor<T>{M, ld}, n, invd);
 }
 
 } // namespace bfa

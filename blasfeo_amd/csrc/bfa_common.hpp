@@ -155,6 +155,71 @@ __device__ __forceinline__ void s2g(PM<T, PS> g, int r0, int c0, int Rv, int Cv,
 	}
 }
 
+// The same two copies with the LDS side addressed through an accessor
+// A(i, c) (element (i, c) of the block).  With LOWER, g2s_acc skips elements
+// whose 16x16 tile lies above the diagonal (packed lower-tile storage has no
+// such tiles) and zero-fills the upper part of diagonal tiles.
+template <typename T, int PS, int NT, bool LOWER, typename ACC>
+__device__ __forceinline__ void g2s_acc(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, const ACC A)
+{
+	constexpr int NW = NT / WAVE;
+	const int rr = r0 & (PS - 1);
+	const int np = (rr + R + PS - 1) / PS;
+	const int run = PS * C;
+	const T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
+	const int lane = threadIdx.x & (WAVE - 1);
+	for (int p = threadIdx.x / WAVE; p < np; p += NW)
+	{
+		const T *src = base + (size_t)p * PS * g.cn;
+		for (int e = lane; e < run; e += WAVE)
+		{
+			const int c = (unsigned)e / PS;
+			const int q = e & (PS - 1);
+			const int i = p * PS + q - rr;
+			if (i < 0 || i >= R)
+				continue;
+			if (LOWER && (i >> 4) < (c >> 4))
+				continue;
+			T v = T(0);
+			if (i < Rv && c < Cv && (!LOWER || i >= c))
+				v = src[e];
+			A(i, c) = v;
+		}
+	}
+}
+
+template <typename T, int PS, int NT, bool LOWER, typename ACC>
+__device__ __forceinline__ void s2g_acc(PM<T, PS> g, int r0, int c0, int Rv, int Cv, const ACC A, int diag)
+{
+	constexpr int NW = NT / WAVE;
+	const int rr = r0 & (PS - 1);
+	const int np = (rr + Rv + PS - 1) / PS;
+	const int run = PS * Cv;
+	T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
+	const int lane = threadIdx.x & (WAVE - 1);
+	for (int p = threadIdx.x / WAVE; p < np; p += NW)
+	{
+		T *dst = base + (size_t)p * PS * g.cn;
+		int cend = run;
+		if (LOWER)
+		{
+			const int last = min(p * PS + PS - 1 - rr, Rv - 1);
+			cend = min(run, (last - diag + 1) * PS);
+		}
+		for (int e = lane; e < cend; e += WAVE)
+		{
+			const int c = (unsigned)e / PS;
+			const int q = e & (PS - 1);
+			const int i = p * PS + q - rr;
+			if (i < 0 || i >= Rv)
+				continue;
+			if (LOWER && i - c < diag)
+				continue;
+			dst[e] = A(i, c);
+		}
+	}
+}
+
 // wave-wide reductions via cross-lane moves
 __device__ __forceinline__ double shfl_xor_d(double v, int mask)
 {

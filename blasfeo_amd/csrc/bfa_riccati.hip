@@ -34,134 +34,155 @@ constexpr int RIC_GENERAL = RIC_NCLASS;
 __constant__ int c_ric_cap[RIC_NCLASS] = {8192, 16384, 24576, 32768, 40960, 54272, 81920, 163840};
 static const int h_ric_cap[RIC_NCLASS] = {8192, 16384, 24576, 32768, 40960, 54272, 81920, 163840};
 
+// LDS storage of the resident form: 16x16 tiles with a leading dimension of 17
+// inside the tile (a row of a tile then spans distinct banks), the stage
+// matrix M as packed lower tiles (tile (I,J), I >= J, at I(I+1)/2 + J) and
+// W as a rectangle of tiles.  Against a square column-major M this keeps
+// only the lower tiles, so the largest stage (nu+nx = 80, nx = 64) needs
+// 76,800 B and two instances share a CU instead of one.
+constexpr int RT_LD = 17;
+constexpr int RT_SZ = 16 * RT_LD;
+
+__host__ __device__ __forceinline__ int ric_tix(int I, int J) { return I * (I + 1) / 2 + J; }
+
+struct TileLower
+{
+	double *p;
+	__device__ __forceinline__ double &operator()(int r, int c) const
+	{
+		return p[ric_tix(r >> 4, c >> 4) * RT_SZ + (c & 15) * RT_LD + (r & 15)];
+	}
+};
+
+struct TileRect
+{
+	double *p;
+	int nct;  // tile columns per tile row
+	__device__ __forceinline__ double &operator()(int r, int c) const
+	{
+		return p[((r >> 4) * nct + (c >> 4)) * RT_SZ + (c & 15) * RT_LD + (r & 15)];
+	}
+};
+
 // LDS bytes of the resident form for an instance whose largest stage is
 // nkmax = max(nu+nx) and whose largest state is nxmax.
 __host__ __device__ __forceinline__ int ric_footprint(int nkmax, int nxmax)
 {
-	const int nkp = (nkmax + 15) / 16 * 16;
-	const int nxp = (nxmax + 15) / 16 * 16;
-	return ((nkp + nxp) * (nkp + 1) + nkp) * (int)sizeof(double);
+	const int nkt = (nkmax + 15) / 16;
+	const int nxt = (nxmax + 15) / 16;
+	return ((nkt * (nkt + 1) / 2 + nkt * nxt) * RT_SZ + nkt * 16) * (int)sizeof(double);
 }
 
 // ---------------------------------------------------------------------------
 // the stage sweep with the working set in LDS (NT threads per instance)
-//   M [nkp x LDM]: L[k+1] at the start of stage k, then RSQ[k] + W W^T, then L[k]
-//   W [nxp x LDM]: BAbt[k], then W = BAbt[k] * L[k+1](nu1:, nu1:)
-// column-major with LDM = nkp + 1, so 16 lanes reading one row of a column
-// block hit distinct banks.
-// ---------------------------------------------------------------------------
-// The two matrix-core stages of a Riccati step, kept out of line: inlined
+//   M (packed lower tiles): L[k+1] at the start of stage k, then
+//     RSQ[k] + W W^T, then L[k]
+//   W (tile rectangle): BAbt[k], then W = BAbt[k] * L[k+1](nu1:, nu1:)
+// The two matrix-core stages of a Riccati step are kept out of line: inlined
 // into the stage loop, their address arithmetic is hoisted across the whole
 // sweep and the one-wave kernel needs 248 VGPRs + 64 AGPRs (one wave per
-// SIMD); out of line it needs 102 + 32 and three instances share a SIMD.
-//
+// SIMD); out of line three instances share a SIMD.
+// ---------------------------------------------------------------------------
 // W <- W * Lxx (trmm_rlnn), Lxx = L[k+1](u1:u1+n1, u1:u1+n1), held in M.
+// 16-row strips per wave, the strip's 16 x 64 result in four accumulators;
+// only this wave reads its strip, so it is overwritten in place.
 template <int NT>
-__device__ __attribute__((noinline)) void ric_trmm_w(const double *M, double *W, int LDM, int nkt, int n1, int n1t, int u1)
+__device__ __attribute__((noinline)) void ric_trmm_w(const TileLower M, const TileRect W, int nkt, int n1, int n1t, int u1)
 {
 	typedef d4_t V4;
 	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
 	const int w = wave_id();
-		// W <- W * Lxx, Lxx = L[k+1](u1:u1+n1, u1:u1+n1) lower; 16-row strips per wave,
-		// the strip's 16 x 64 result kept in four accumulators
-		for (int rt = w; rt < nkt; rt += NW)
+	for (int rt = w; rt < nkt; rt += NW)
+	{
+		V4 acc[4];
+#pragma unroll
+		for (int ct = 0; ct < 4; ct++)
+			acc[ct] = V4{0, 0, 0, 0};
+		for (int l0 = 0; l0 < n1; l0 += 4)
 		{
-			V4 acc[4];
+			const int lk = l0 + (lane >> 4);
+			const double av = W(rt * 16 + (lane & 15), lk);
 #pragma unroll
 			for (int ct = 0; ct < 4; ct++)
-				acc[ct] = V4{0, 0, 0, 0};
-			for (int l0 = 0; l0 < n1; l0 += 4)
 			{
-				const int lk = l0 + (lane >> 4);
-				const double av = W[lk * LDM + rt * 16 + (lane & 15)];
-#pragma unroll
-				for (int ct = 0; ct < 4; ct++)
+				if (ct < n1t && l0 + 3 >= ct * 16)
 				{
-					if (ct < n1t && l0 + 3 >= ct * 16)
-					{
-						const int j = ct * 16 + (lane & 15);
-						const double bv = (lk < n1 && lk >= j) ? M[(u1 + j) * LDM + u1 + lk] : 0.0;
-						acc[ct] = mfma16(av, bv, acc[ct]);
-					}
+					const int j = ct * 16 + (lane & 15);
+					const double bv = (lk < n1 && lk >= j) ? M(u1 + lk, u1 + j) : 0.0;
+					acc[ct] = mfma16(av, bv, acc[ct]);
 				}
 			}
-			// only this wave reads this strip of W, so it is overwritten in place
-			wave_sync();
-#pragma unroll
-			for (int ct = 0; ct < 4; ct++)
-				if (ct < n1t)
-#pragma unroll
-					for (int r = 0; r < 4; r++)
-						W[(ct * 16 + (lane & 15)) * LDM + rt * 16 + acc_row<double>(lane, r)] = acc[ct][r];
 		}
+		wave_sync();
+#pragma unroll
+		for (int ct = 0; ct < 4; ct++)
+			if (ct < n1t)
+#pragma unroll
+				for (int r = 0; r < 4; r++)
+					W(rt * 16 + acc_row<double>(lane, r), ct * 16 + (lane & 15)) = acc[ct][r];
+	}
 }
 
-// M += W W^T on the lower tiles (syrk_ln).
+// M += W W^T on the lower tiles (syrk_ln): (block column bj, group of up to
+// 4 tiles bi = g..g+3) per wave, the B operand shared by the group.
 template <int NT>
-__device__ __attribute__((noinline)) void ric_syrk_m(double *M, const double *W, int LDM, int nkt, int n1)
+__device__ __attribute__((noinline)) void ric_syrk_m(const TileLower M, const TileRect W, int nkt, int n1)
 {
 	typedef d4_t V4;
 	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
 	const int w = wave_id();
-		// M += W W^T on the lower tiles: (block column bj, group of up to 4 tiles
-		// bi = g..g+3) per wave, the B operand shared by the group
+	int ntask = 0;
+	for (int bj = 0; bj < nkt; bj++)
+		ntask += (nkt - bj + 3) / 4;
+	for (int task = w; task < ntask; task += NW)
+	{
+		int bj = 0, rem = task;
+		while (rem >= (nkt - bj + 3) / 4)
 		{
-			int ntask = 0;
-			for (int bj = 0; bj < nkt; bj++)
-				ntask += (nkt - bj + 3) / 4;
-			for (int task = w; task < ntask; task += NW)
+			rem -= (nkt - bj + 3) / 4;
+			bj++;
+		}
+		const int g = bj + 4 * rem;
+		V4 acc[4];
+#pragma unroll
+		for (int t = 0; t < 4; t++)
+		{
+			if (g + t < nkt)
+#pragma unroll
+				for (int r = 0; r < 4; r++)
+					acc[t][r] = M((g + t) * 16 + acc_row<double>(lane, r), bj * 16 + (lane & 15));
+		}
+		for (int l0 = 0; l0 < n1; l0 += 4)
+		{
+			const int lk = l0 + (lane >> 4);
+			const double bv = W(bj * 16 + (lane & 15), lk);
+#pragma unroll
+			for (int t = 0; t < 4; t++)
 			{
-				int bj = 0, rem = task;
-				while (rem >= (nkt - bj + 3) / 4)
+				if (g + t < nkt)
 				{
-					rem -= (nkt - bj + 3) / 4;
-					bj++;
-				}
-				const int g = bj + 4 * rem;
-				V4 acc[4];
-#pragma unroll
-				for (int t = 0; t < 4; t++)
-				{
-					if (g + t < nkt)
-#pragma unroll
-						for (int r = 0; r < 4; r++)
-							acc[t][r] = M[(bj * 16 + (lane & 15)) * LDM + (g + t) * 16 + acc_row<double>(lane, r)];
-				}
-				for (int l0 = 0; l0 < n1; l0 += 4)
-				{
-					const int lk = l0 + (lane >> 4);
-					const double bv = W[lk * LDM + bj * 16 + (lane & 15)];
-#pragma unroll
-					for (int t = 0; t < 4; t++)
-					{
-						if (g + t < nkt)
-						{
-							const double av = W[lk * LDM + (g + t) * 16 + (lane & 15)];
-							acc[t] = mfma16(av, bv, acc[t]);
-						}
-					}
-				}
-#pragma unroll
-				for (int t = 0; t < 4; t++)
-				{
-					if (g + t < nkt)
-#pragma unroll
-						for (int r = 0; r < 4; r++)
-							M[(bj * 16 + (lane & 15)) * LDM + (g + t) * 16 + acc_row<double>(lane, r)] = acc[t][r];
+					const double av = W((g + t) * 16 + (lane & 15), lk);
+					acc[t] = mfma16(av, bv, acc[t]);
 				}
 			}
 		}
+#pragma unroll
+		for (int t = 0; t < 4; t++)
+		{
+			if (g + t < nkt)
+#pragma unroll
+				for (int r = 0; r < 4; r++)
+					M((g + t) * 16 + acc_row<double>(lane, r), bj * 16 + (lane & 15)) = acc[t][r];
+		}
+	}
 }
 
 template <int PS, int NT>
 __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 {
-	typedef d4_t V4;
-	constexpr int NW = NT / WAVE;
-	const int lane = lane_id();
-	const int w = wave_id();
+	constexpr int NMAXC = (NT == WAVE) ? 64 : 128;
 	const int N = a.N;
 	const int *nx = a.nx;
 	const int *nu = a.nu;
@@ -174,20 +195,20 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 		nkmax = max(nkmax, nu[k] + nx[k]);
 		nxmax = max(nxmax, nx[k]);
 	}
-	const int nkp = (nkmax + 15) / 16 * 16;
-	const int nxp = (nxmax + 15) / 16 * 16;
-	const int LDM = nkp + 1;
-	double *M = smem;
-	double *W = M + (size_t)nkp * LDM;
-	double *invd = W + (size_t)nxp * LDM;
+	const int nkt_max = (nkmax + 15) / 16;
+	const int nxt_max = (nxmax + 15) / 16;
+	const int nlow = nkt_max * (nkt_max + 1) / 2;
+	const TileLower M{smem};
+	const TileRect W{smem + (size_t)nlow * RT_SZ, nxt_max};
+	double *invd = smem + (size_t)(nlow + nkt_max * nxt_max) * RT_SZ;
 
 	// last stage: L[N] = chol(Q_N)
 	{
 		const int m = nx[N];
 		const int mt = (m + 15) / 16;
-		g2s<double, PS, NT, false, true>(view<double, PS>(&RSQ[N]), 0, 0, mt * 16, mt * 16, m, m, M, LDM);
-		chol_lds<double, NT, (NT == WAVE ? 64 : 128)>(M, LDM, m, invd);
-		s2g<double, PS, NT, true>(view<double, PS>(&L[N]), 0, 0, m, m, M, LDM, 0);
+		g2s_acc<double, PS, NT, true>(view<double, PS>(&RSQ[N]), 0, 0, mt * 16, mt * 16, m, m, M);
+		chol_acc<double, NT, NMAXC>(M, m, invd);
+		s2g_acc<double, PS, NT, true>(view<double, PS>(&L[N]), 0, 0, m, m, M, 0);
 		for (int j = threadIdx.x; j < m; j += NT)
 			L[N].dA[j] = invd[j];
 		if (threadIdx.x == 0)
@@ -202,17 +223,17 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 		const int nkt = (nk + 15) / 16;
 		const int n1t = (n1 + 15) / 16;
 		// stage BAbt[k] (nk x n1) into W; M still holds L[k+1]
-		g2s<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W, LDM);
+		g2s_acc<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W);
 		__syncthreads();
-		ric_trmm_w<NT>(M, W, LDM, nkt, n1, n1t, u1);
+		ric_trmm_w<NT>(M, W, nkt, n1, n1t, u1);
 		__syncthreads();
 		// M <- lower(RSQ[k]) (L[k+1] is no longer needed)
-		g2s<double, PS, NT, false, true>(view<double, PS>(&RSQ[k]), 0, 0, nkt * 16, nkt * 16, nk, nk, M, LDM);
+		g2s_acc<double, PS, NT, true>(view<double, PS>(&RSQ[k]), 0, 0, nkt * 16, nkt * 16, nk, nk, M);
 		__syncthreads();
-		ric_syrk_m<NT>(M, W, LDM, nkt, n1);
+		ric_syrk_m<NT>(M, W, nkt, n1);
 		// L[k] = chol(M)
-		chol_lds<double, NT, (NT == WAVE ? 64 : 128)>(M, LDM, nk, invd);
-		s2g<double, PS, NT, true>(view<double, PS>(&L[k]), 0, 0, nk, nk, M, LDM, 0);
+		chol_acc<double, NT, NMAXC>(M, nk, invd);
+		s2g_acc<double, PS, NT, true>(view<double, PS>(&L[k]), 0, 0, nk, nk, M, 0);
 		for (int j = threadIdx.x; j < nk; j += NT)
 			L[k].dA[j] = invd[j];
 		if (threadIdx.x == 0)
@@ -311,11 +332,12 @@ __global__ void k_ric_plan(const blasfeo_amd_riccati_arg *__restrict__ args, int
 	if (bad)
 		return;
 	int cls = RIC_GENERAL;
-	if (nxmax <= 64)
+	if (nxmax <= 64 && nkmax <= 128)
 	{
+		// one-wave classes (cap <= 40960) factor with one panel row per lane: nkmax <= 64
 		const int f = ric_footprint(nkmax, nxmax);
 		for (int c = RIC_NCLASS - 1; c >= 0; c--)
-			if (f <= c_ric_cap[c])
+			if (f <= c_ric_cap[c] && (c_ric_cap[c] > 40960 || nkmax <= 64))
 				cls = c;
 	}
 	if (cls == RIC_GENERAL && nkmax > mmax)
@@ -336,7 +358,7 @@ __global__ __launch_bounds__(NT) void k_ric_class(const blasfeo_amd_riccati_arg 
 
 // one wave per instance: registers capped at 168 so that three instances
 // share each SIMD.  Only classes of at most 40960 B use it, whose stages have
-// nu+nx <= 48, so the Cholesky keeps one panel row per lane (n <= 64).
+// nu+nx <= 64 (the plan's rule), so the Cholesky keeps one panel row per lane.
 template <int PS>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3))) void k_ric_class1(
 	const blasfeo_amd_riccati_arg *__restrict__ args, const int *__restrict__ count, const int *__restrict__ list)

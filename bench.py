@@ -147,6 +147,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-max-instances", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="riccati: pick launch shapes and the panel size with blasfeo_amd.autotune on "
+                         "this workload before timing")
     ap.add_argument("--graph", action="store_true",
                     help="riccati: time replays of the batched sweep captured in a HIP graph")
     ap.add_argument("--workload", default="riccati",
@@ -191,6 +194,12 @@ def run_riccati(args, torch, bfa):
     rng = np.random.default_rng(args.seed)
     nxs = rng.integers(8, 65, args.batch)
     nus = rng.integers(2, 17, args.batch)
+    tuning = None
+    if args.tune:
+        from blasfeo_amd import autotune
+        tuning = autotune.tune_riccati(nxs, nus, args.horizon, log=log)
+        args.ps = tuning["ps"]
+        log("tuned: %s" % json.dumps(tuning))
     log("building %d Riccati instances (N=%d)" % (args.batch, args.horizon))
     rb = RiccatiBatch(nxs, nus, args.horizon, args.ps)
     inp = rb.fill_inputs(rng)
@@ -261,7 +270,8 @@ def run_riccati(args, torch, bfa):
         "config": {"workload": "riccati_trf_batched", "batch": args.batch, "horizon": args.horizon,
                    "nx": "U{8..64}", "nu": "U{2..16}", "ps": args.ps, "precision": "fp64",
                    "parallelism": "one workgroup per instance, 1 GPU",
-                   "launch": "hip graph replay" if args.graph else "eager"},
+                   "launch": "hip graph replay" if args.graph else "eager",
+                   "tuning": tuning},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "algorithmic_bytes": algo_bytes, "flops_per_launch": flops,

@@ -94,6 +94,10 @@ def lib():
     L.blasfeo_amd_dposv_mixed_batched.argtypes = [ip, ip, vp, ctypes.c_double, ip,
                                                   ctypes.POINTER(HINT), vp]
     L.blasfeo_amd_dposv_mixed_batched.restype = ip
+    L.blasfeo_amd_tune_set.argtypes = [ctypes.c_char_p, ip, ip]
+    L.blasfeo_amd_tune_set.restype = ip
+    L.blasfeo_amd_tune_get.argtypes = [ctypes.c_char_p, ip]
+    L.blasfeo_amd_tune_get.restype = ip
     L.blasfeo_amd_version.restype = ctypes.c_char_p
     L.blasfeo_amd_last_error.restype = ctypes.c_char_p
     L.blasfeo_amd_set_ps.argtypes = [ip]

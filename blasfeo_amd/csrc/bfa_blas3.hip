@@ -347,7 +347,10 @@ static int tiles_y(const blasfeo_amd_hint *hint, int tm, int tn)
 	int mt = (hint->max_m + tm - 1) / tm;
 	int nt = hint->max_n > 0 ? (hint->max_n + tn - 1) / tn : 1;
 	int y = mt * nt;
-	return y < 1 ? 1 : (y > 64 ? 64 : y);
+	int cap = tune_value(TUNE_GEMM_GRID_Y, 0);
+	if (cap <= 0)
+		cap = 64;
+	return y < 1 ? 1 : (y > cap ? cap : y);
 }
 
 static int check_common(int ps, int batch, const void *args, bool is_float, const char *name)

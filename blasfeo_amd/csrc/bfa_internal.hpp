@@ -21,4 +21,8 @@ int num_cus();
 // are ordered, so reusing it across calls on that stream is safe.
 void *workspace(size_t bytes, hipStream_t stream);
 
+// Launch-shape knobs (blasfeo_amd_tune_set); 0 means the built-in choice.
+enum TuneKnob { TUNE_RIC_PER_CU = 0, TUNE_RIC_THREADS = 1, TUNE_GEMM_GRID_Y = 2 };
+int tune_value(TuneKnob knob, int index);
+
 } // namespace bfa

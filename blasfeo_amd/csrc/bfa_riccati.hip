@@ -423,14 +423,20 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 	for (int cls = RIC_NCLASS - 1; cls >= 0; cls--)
 	{
 		// resident instances per CU: LDS share, and for one-wave instances at
-		// most 3 waves per SIMD (the register cap of k_ric_class1)
+		// most 3 waves per SIMD (the register cap of k_ric_class1); threads per
+		// instance by footprint.  Both can be overridden by the tuning knobs.
 		int per_cu = RIC_LDS_MAX / h_ric_cap[cls];
 		if (h_ric_cap[cls] <= 40960)
 			per_cu = min(per_cu, 12);
+		int nt = h_ric_cap[cls] <= 40960 ? 64 : (h_ric_cap[cls] <= 81920 ? 128 : 256);
+		if (const int v = tune_value(TUNE_RIC_PER_CU, cls))
+			per_cu = v;
+		if (const int v = tune_value(TUNE_RIC_THREADS, cls))
+			nt = v;
 		const int grid = min(batch, cus * per_cu);
-		if (h_ric_cap[cls] <= 40960)
+		if (nt == 64)
 			launch_class<PS, 64>(cls, grid, args, counts, lists, batch, st);
-		else if (h_ric_cap[cls] <= 81920)
+		else if (nt == 128)
 			launch_class<PS, 128>(cls, grid, args, counts, lists, batch, st);
 		else
 			launch_class<PS, 256>(cls, grid, args, counts, lists, batch, st);

@@ -119,6 +119,20 @@ typedef void *blasfeo_amd_stream_t;
 const char *blasfeo_amd_version(void);
 const char *blasfeo_amd_last_error(void);
 
+/* Launch-shape knobs read by the batched entries (process-wide; no counterpart
+ * in the reference, whose block sizes are fixed at build time).  Set by the
+ * autotuner (blasfeo_amd/autotune.py) from measurements on the caller's
+ * workload; 0 restores the built-in choice.
+ *   "ric_per_cu",  index = Riccati LDS class 0..7: resident instances per CU
+ *                  of the class's persistent launch (1..64);
+ *   "ric_threads", index = class 0..7: threads per instance (64, 128 or 256;
+ *                  64 only for classes 0..4);
+ *   "gemm_grid_y", index 0: most workgroups per gemm/syrk/trmm instance (1..64).
+ * blasfeo_amd_tune_set returns BLASFEO_AMD_OK or BLASFEO_AMD_EINVAL;
+ * blasfeo_amd_tune_get returns the value (0 = built-in) or BLASFEO_AMD_EINVAL. */
+int blasfeo_amd_tune_set(const char *name, int index, int value);
+int blasfeo_amd_tune_get(const char *name, int index);
+
 /* Panel size used by the single-instance API and by blasfeo_memsize_dmat /
  * blasfeo_create_dmat (double: 4 or 8, float: panel size is 8 for ps=4 and
  * 16 for ps=8, following include/blasfeo_block_size.h:99-108 and :53-62).

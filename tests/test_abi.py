@@ -154,3 +154,30 @@ def test_argument_errors_without_gpu():
     assert f(4, 1, ctypes.c_void_p(8), 1e-14, 10, None, None) == -1
     assert f(6, 1, ctypes.c_void_p(8), 1e-14, 10, ctypes.byref(ok), None) == -1
     assert f(4, 0, None, 1e-14, 10, ctypes.byref(ok), None) == 0
+
+
+def test_tuning_knobs_without_gpu():
+    """Launch-shape knobs: defaults, validation, and the Python footprint
+    rule used by the autotuner agreeing with the class rule of the kernel
+    (classes by LDS bytes; one-wave classes only for nu+nx <= 64)."""
+    from blasfeo_amd import autotune as at
+    assert at.get_knob("ric_per_cu", 3) == 0
+    at.set_knob("ric_per_cu", 3, 7)
+    assert at.get_knob("ric_per_cu", 3) == 7
+    at.set_knob("ric_threads", 2, 64)
+    with pytest.raises(RuntimeError):
+        at.set_knob("ric_threads", 6, 64)      # one wave needs nu+nx <= 64
+    with pytest.raises(RuntimeError):
+        at.set_knob("ric_per_cu", 8, 1)        # no such class
+    with pytest.raises(RuntimeError):
+        at.set_knob("gemm_grid_y", 0, 65)
+    with pytest.raises(RuntimeError):
+        at.set_knob("nonsense", 0, 1)
+    at.reset()
+    assert at.get_knob("ric_per_cu", 3) == 0 and at.get_knob("ric_threads", 2) == 0
+    # footprints of the resident form: largest bench stage (nu+nx = 80, nx = 64)
+    # fits two per CU, nu+nx = 64, nx = 48 three per CU
+    assert at.ric_footprint(80, 64) == 76800 and at.ric_class(80, 64) == 6
+    assert at.ric_class(64, 48) == 5
+    assert at.ric_class(65, 8) >= 5          # nu+nx > 64 never takes a one-wave class
+    assert at.ric_class(60, 70) == 8 and at.ric_class(130, 10) == 8

@@ -520,7 +520,7 @@ def run_posv(args, torch, bfa):
     import oracle
     ps = args.ps
     batch, n, nrhs = 2048, 256, 4
-    tol, max_iter = 1e-14, 10
+    tol, max_iter = 1e-14, 30
     gen = torch.Generator(device="cuda")
     gen.manual_seed(args.seed)
     stream = torch.cuda.Stream()

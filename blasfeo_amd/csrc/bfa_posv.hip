@@ -14,12 +14,15 @@
 //   the other waves apply the trailing update (look-ahead of one column).
 // Solves with the factor apply Z_K to the 16-row blocks, so each block step
 // is a small matrix-vector product instead of a 16-step substitution.
-// Refinement: r = b - A x is formed in FP64 from the FP64 A in global memory
-// (the symmetric product from the stored lower triangle), the correction
-// solves in FP32 and is accumulated into the FP64 x held in LDS, until the
-// normwise backward error max_c |r_c|/(|A| |x_c| + |b_c|) (infinity norms)
-// is <= tol.  Instances whose FP32 factor meets a non-positive pivot or whose
-// refinement does not converge are solved again by the FP64 kernels
+// Refinement: conjugate gradients in FP64 preconditioned by the FP32 factor
+// (CG-based iterative refinement): the products A p are formed in FP64 from
+// the FP64 A in global memory (the symmetric product from the stored lower
+// triangle), the preconditioner solves run in FP32 from LDS, until the
+// normwise backward error max_c |b_c - A x_c|/(|A| |x_c| + |b_c|) (infinity
+// norms) of the true residual is <= tol.  Against stationary refinement this
+// keeps converging when cond(A) * eps32 approaches 1 (cond ~ 1e8).
+// Instances whose FP32 factor meets a non-positive pivot, or whose iteration
+// breaks down or does not converge, are solved again by the FP64 kernels
 // (potrf_l + trsm_llnn + trsm_lltn) in the same stream.
 #include "bfa_chol_core.hpp"
 #include "bfa_internal.hpp"
@@ -91,6 +94,94 @@ __device__ __forceinline__ void block_max(double *v, double *red)
 			r = fmax(r, red[ww * 12 + q]);
 		v[q] = r;
 	}
+}
+
+// Block-wide sum of NV values per thread (every thread gets the same result:
+// one lane per wave contributes its wave's total, summed in wave order).
+template <int NV>
+__device__ __forceinline__ void block_sum(double *v, double *red)
+{
+	const int lane = lane_id(), w = wave_id();
+#pragma unroll
+	for (int q = 0; q < NV; q++)
+		for (int m = 1; m < WAVE; m <<= 1)
+			v[q] += __shfl_xor(v[q], m, WAVE);
+	__syncthreads();
+	if (lane == 0)
+#pragma unroll
+		for (int q = 0; q < NV; q++)
+			red[w * 12 + q] = v[q];
+	__syncthreads();
+#pragma unroll
+	for (int q = 0; q < NV; q++)
+	{
+		double r = red[q];
+		for (int ww = 1; ww < PV_NW; ww++)
+			r += red[ww * 12 + q];
+		v[q] = r;
+	}
+}
+
+// q = A v for the thread's row i (symmetric A from its stored lower triangle,
+// FP64, global memory), v = V[j*4 + c] in LDS.  Thread t takes row i = t & 255
+// and half t >> 8 of the columns; the second half's partial sums go through S
+// and the row's owner (first half) returns the total in q and the row's
+// sum of |a_ij| in sabs.  Columns are walked in chunks of four with the loads
+// of a chunk issued together (element (i,j), j <= i, from row i; j > i from
+// column i, where four consecutive rows of a panel are contiguous).
+template <int PS>
+__device__ __forceinline__ void posv_matvec(const PM<double, PS> gA, int ai, int aj, int n, const double *V,
+	double *S, double *q, double &sabs)
+{
+	const int t = threadIdx.x;
+	const int h = t >> 8, i = t & 255;
+	const int jm = ((n + 7) >> 3) << 2;
+	const int jlo = h ? jm : 0, jhi = h ? n : jm;
+	double acc[PV_R] = {0, 0, 0, 0};
+	double sa = 0.0;
+	if (i < n)
+	{
+#pragma unroll 4
+		for (int j0 = jlo; j0 < jhi; j0 += 4)
+		{
+			double a[4];
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+			{
+				// past the end of the range, load a valid element and drop it
+				const int j = min(j0 + u, jhi - 1);
+				const int r = j <= i ? i : j, c = j <= i ? j : i;
+				const double e = gA.at(ai + r, aj + c);
+				a[u] = j0 + u < jhi ? e : 0.0;
+			}
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+			{
+				const int j = min(j0 + u, jhi - 1);
+				const double *vj = V + j * PV_R;
+#pragma unroll
+				for (int c = 0; c < PV_R; c++)
+					acc[c] += a[u] * vj[c];
+				sa += fabs(a[u]);
+			}
+		}
+	}
+	if (h == 1 && i < n)
+	{
+#pragma unroll
+		for (int c = 0; c < PV_R; c++)
+			S[i * 5 + c] = acc[c];
+		S[i * 5 + 4] = sa;
+	}
+	__syncthreads();
+	if (h == 0 && i < n)
+	{
+#pragma unroll
+		for (int c = 0; c < PV_R; c++)
+			q[c] = acc[c] + S[i * 5 + c];
+		sabs = sa + S[i * 5 + 4];
+	}
+	__syncthreads();
 }
 
 // Wave 0: FP64 Cholesky of the diagonal tile FKK (FP32 in LDS), replaced by
@@ -395,126 +486,164 @@ __global__ __launch_bounds__(PV_NT) void k_posv_mixed(const blasfeo_amd_dposv_ar
 	}
 	const bool ok = posv_factor(L, nb);
 
+	// preconditioned conjugate gradients in FP64 per group of 4 right-hand
+	// sides, preconditioner M = L L^T with the FP32 factor in LDS; the row
+	// owner (t < 256, row t) keeps x, r, p of its row in registers, p is also
+	// published in LDS (L.X) for the products A p.  A converged recursive
+	// residual is confirmed with the true residual b - A x before stopping;
+	// otherwise CG restarts from the true residual.
+	const int i = t & 255;
+	const bool owner = t < 256 && i < n;
 	int iters = 0;
 	double berr = 0.0;
 	bool conv = ok;
+	double anorm = -1.0;
 	for (int c0 = 0; conv && c0 < nrhs; c0 += PV_R)
 	{
 		const int nr = min(PV_R, nrhs - c0);
-		// b -> R (FP32), x = 0, |b|_inf
-		double bmax[PV_R] = {0, 0, 0, 0};
-		for (int p = t; p < n16 * PV_R; p += PV_NT)
-		{
-			const int c = p & 3, i = p >> 2;
-			double v = 0.0;
-			if (i < n && c < nr)
-				v = gB.at(a.bi + i, a.bj + c0 + c);
-			L.R[p] = (float)v;
-			L.X[p] = 0.0;
+		double xr[PV_R] = {0, 0, 0, 0}, rr[PV_R] = {0, 0, 0, 0}, pr[PV_R] = {0, 0, 0, 0};
+		double bq[PV_R] = {0, 0, 0, 0};
+		double v[PV_R];
+		if (owner)
 #pragma unroll
-			for (int q = 0; q < PV_R; q++)
-				if (q == c)
-					bmax[q] = fmax(bmax[q], fabs(v));
-		}
-		block_max<PV_R>(bmax, L.red);
-		double anorm = 0.0;
-		double eta_prev = 1e300;
+			for (int c = 0; c < PV_R; c++)
+				bq[c] = rr[c] = c < nr ? gB.at(a.bi + i, a.bj + c0 + c) : 0.0;
+#pragma unroll
+		for (int c = 0; c < PV_R; c++)
+			v[c] = fabs(bq[c]);
+		block_max<PV_R>(v, L.red);
+		double bmax[PV_R];
+#pragma unroll
+		for (int c = 0; c < PV_R; c++)
+			bmax[c] = v[c];
+		double rho[PV_R] = {0, 0, 0, 0};
+		bool restart = true;
+		bool fail = false;
 		int it = 0;
 		for (;;)
 		{
+			// z = M^{-1} r (FP32); rho' = r.z; p = z + beta p
+			if (t < 256 && i < n16)
+#pragma unroll
+				for (int c = 0; c < PV_R; c++)
+					L.R[i * PV_R + c] = owner ? (float)rr[c] : 0.0f;
 			posv_solve(L, nb);
-			for (int p = t; p < n16 * PV_R; p += PV_NT)
-				L.X[p] += (double)L.R[p];
+			double z[PV_R] = {0, 0, 0, 0};
+			if (owner)
+#pragma unroll
+				for (int c = 0; c < PV_R; c++)
+					z[c] = (double)L.R[i * PV_R + c];
+#pragma unroll
+			for (int c = 0; c < PV_R; c++)
+				v[c] = rr[c] * z[c];
+			block_sum<PV_R>(v, L.red);
+#pragma unroll
+			for (int c = 0; c < PV_R; c++)
+			{
+				const double beta = (restart || rho[c] == 0.0) ? 0.0 : v[c] / rho[c];
+				rho[c] = v[c];
+				pr[c] = z[c] + beta * pr[c];
+			}
+			restart = false;
+			if (t < 256 && i < n16)
+#pragma unroll
+				for (int c = 0; c < PV_R; c++)
+					L.X[i * PV_R + c] = owner ? pr[c] : 0.0;
 			__syncthreads();
-			// r = b - A x in FP64; half h of the threads takes columns [jlo, jhi)
-			const int h = t >> 8, i = t & 255;
-			const int jm = (n + 1) >> 1;
-			const int jlo = h ? jm : 0, jhi = h ? n : jm;
-			double acc[PV_R] = {0, 0, 0, 0};
+			// q = A p ; alpha = rho / p.q ; x += alpha p ; r -= alpha q
+			double qv[PV_R] = {0, 0, 0, 0};
 			double sabs = 0.0;
-			if (i < n)
+			posv_matvec<PS>(gA, ai, aj, n, L.X, L.S, qv, sabs);
+			if (anorm < 0.0)
 			{
-				for (int j = jlo; j < jhi; j++)
-				{
-					const double v = (j <= i) ? gA.at(ai + i, aj + j) : gA.at(ai + j, aj + i);
-					const double *xj = L.X + j * PV_R;
-#pragma unroll
-					for (int q = 0; q < PV_R; q++)
-						acc[q] += v * xj[q];
-					sabs += fabs(v);
-				}
+				double an = sabs;
+				block_max<1>(&an, L.red);
+				anorm = an;
 			}
-			if (h == 1 && i < n)
+#pragma unroll
+			for (int c = 0; c < PV_R; c++)
+				v[c] = pr[c] * qv[c];
+			block_sum<PV_R>(v, L.red);
+#pragma unroll
+			for (int c = 0; c < PV_R; c++)
 			{
-#pragma unroll
-				for (int q = 0; q < PV_R; q++)
-					L.S[i * 5 + q] = acc[q];
-				L.S[i * 5 + 4] = sabs;
+				if (c < nr && rho[c] != 0.0 && !(v[c] > 0.0))
+					fail = true;  // p^T A p <= 0 (or NaN): not SPD at this precision
+				const double alpha = (c < nr && rho[c] != 0.0 && v[c] > 0.0) ? rho[c] / v[c] : 0.0;
+				xr[c] += alpha * pr[c];
+				rr[c] -= alpha * qv[c];
 			}
-			__syncthreads();
-			double v8[2 * PV_R + 1];
+			it++;
+			if (fail)
+				break;
+			// normwise backward error of the recursive residual
+			double m8[2 * PV_R];
 #pragma unroll
-			for (int q = 0; q < 2 * PV_R + 1; q++)
-				v8[q] = 0.0;
-			if (h == 0 && i < n)
+			for (int c = 0; c < PV_R; c++)
 			{
-#pragma unroll
-				for (int q = 0; q < PV_R; q++)
-				{
-					const double bq = q < nr ? gB.at(a.bi + i, a.bj + c0 + q) : 0.0;
-					const double rq = bq - (acc[q] + L.S[i * 5 + q]);
-					L.R[i * PV_R + q] = (float)rq;
-					v8[q] = fabs(rq);
-					v8[PV_R + q] = fabs(L.X[i * PV_R + q]);
-				}
-				v8[2 * PV_R] = sabs + L.S[i * 5 + 4];
+				m8[c] = fabs(rr[c]);
+				m8[PV_R + c] = fabs(xr[c]);
 			}
-			if (h == 0 && i >= n && i < n16)
-#pragma unroll
-				for (int q = 0; q < PV_R; q++)
-					L.R[i * PV_R + q] = 0.0f;
-			block_max<2 * PV_R + 1>(v8, L.red);
-			if (it == 0)
-				anorm = v8[2 * PV_R];
+			block_max<2 * PV_R>(m8, L.red);
 			double eta = 0.0;
 #pragma unroll
-			for (int q = 0; q < PV_R; q++)
-			{
-				if (q < nr)
+			for (int c = 0; c < PV_R; c++)
+				if (c < nr)
 				{
-					const double den = anorm * v8[PV_R + q] + bmax[q];
-					const double e = den > 0.0 ? v8[q] / den : (v8[q] > 0.0 ? 1e300 : 0.0);
-					eta = fmax(eta, e);
+					const double den = anorm * m8[PV_R + c] + bmax[c];
+					eta = fmax(eta, den > 0.0 ? m8[c] / den : (m8[c] > 0.0 ? 1e300 : 0.0));
 				}
-			}
-			if (!(eta <= tol))
-			{
-				// not converged: stop on the iteration limit, on growth, or on NaN
-				if (it >= max_iter || !(eta < eta_prev))
-				{
-					conv = false;
-					berr = fmax(berr, eta);
-					iters = max(iters, it);
-					break;
-				}
-				eta_prev = eta;
-				it++;
+			if (!(eta <= tol) && eta == eta && it < max_iter)
 				continue;
+			// confirm with the true residual b - A x
+			if (t < 256 && i < n16)
+#pragma unroll
+				for (int c = 0; c < PV_R; c++)
+					L.X[i * PV_R + c] = owner ? xr[c] : 0.0;
+			__syncthreads();
+			double ax[PV_R] = {0, 0, 0, 0};
+			double sab2 = 0.0;
+			posv_matvec<PS>(gA, ai, aj, n, L.X, L.S, ax, sab2);
+#pragma unroll
+			for (int c = 0; c < PV_R; c++)
+			{
+				rr[c] = owner ? bq[c] - ax[c] : 0.0;
+				m8[c] = fabs(rr[c]);
+				m8[PV_R + c] = fabs(xr[c]);
 			}
-			berr = fmax(berr, eta);
-			iters = max(iters, it);
+			block_max<2 * PV_R>(m8, L.red);
+			double eta_t = 0.0;
+#pragma unroll
+			for (int c = 0; c < PV_R; c++)
+				if (c < nr)
+				{
+					const double den = anorm * m8[PV_R + c] + bmax[c];
+					eta_t = fmax(eta_t, den > 0.0 ? m8[c] / den : (m8[c] > 0.0 ? 1e300 : 0.0));
+				}
+			if (eta_t <= tol)
+			{
+				berr = fmax(berr, eta_t);
+				break;
+			}
+			if (it >= max_iter || !(eta_t == eta_t))
+			{
+				fail = true;
+				break;
+			}
+			restart = true;
+		}
+		iters = max(iters, it);
+		if (fail)
+		{
+			conv = false;
 			break;
 		}
-		if (conv)
-		{
-			__syncthreads();
-			for (int p = t; p < n * PV_R; p += PV_NT)
-			{
-				const int c = p & 3, i = p >> 2;
+		if (owner)
+#pragma unroll
+			for (int c = 0; c < PV_R; c++)
 				if (c < nr)
-					gX.at(a.xi + i, a.xj + c0 + c) = L.X[p];
-			}
-		}
+					gX.at(a.xi + i, a.xj + c0 + c) = xr[c];
+		__syncthreads();
 	}
 	if (t == 0)
 	{

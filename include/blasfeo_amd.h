@@ -344,7 +344,7 @@ int blasfeo_amd_d_back_ric_trf_batched(int ps, int batch, const struct blasfeo_a
 /* ------------------------------------------------------------------------ */
 
 /* Outcome of one mixed-precision solve.
- * iters: refinement corrections applied after the first solve;
+ * iters: conjugate-gradient iterations (products A p) of the refinement;
  * path:  0 = FP32 factor + FP64 refinement converged,
  *        1 = solved by the FP64 factorisation (the FP32 factor broke down or
  *            the refinement did not reach tol within max_iter);
@@ -374,10 +374,11 @@ struct blasfeo_amd_dposv_arg
  * blasfeo_dpotrf_l + blasfeo_dtrsm_llnn + blasfeo_dtrsm_lltn
  * (blasfeo_hp_pm/d_lapack_lib4.c:4269; d_blas3_lib4.c:7236,3782) computes, to
  * normwise backward error <= tol.  The factor is formed in FP32 and kept in
- * LDS (FP32 matrix-core trailing updates, FP64 16x16 diagonal blocks), the
- * residuals b - A x are formed in FP64 from the FP64 A, and corrections are
- * applied until berr <= tol (at most max_iter).  Instances that do not
- * converge are re-solved in FP64 into sL on the device.  n <= 256. */
+ * LDS (FP32 matrix-core trailing updates, FP64 16x16 diagonal blocks); the
+ * refinement is conjugate gradients in FP64 (products with the FP64 A)
+ * preconditioned by that factor, at most max_iter iterations, stopped when the
+ * true residual meets berr <= tol.  Instances that do not converge are
+ * re-solved in FP64 into sL on the device; n > 256 always takes that path. */
 int blasfeo_amd_dposv_mixed_batched(int ps, int batch, const struct blasfeo_amd_dposv_arg *args, double tol, int max_iter, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 
 #ifdef __cplusplus

@@ -91,15 +91,17 @@ def _run(ps, cases, tol, max_iter, nmax, seed):
 def test_posv_mixed_converges(gpu, ps):
     """cfg-4-like instances (Haar Q, geometric spectrum, cond log-uniform in
     [1e2, 1e8]) at uneven n and nrhs: every instance converges on the mixed
-    path to berr <= 1e-14, and X agrees with the FP64 solve to the accuracy
-    the condition number allows."""
+    path (conjugate gradients preconditioned by the FP32 factor, at most 30
+    iterations) to berr <= 1e-14, and X agrees with the FP64 solve to the
+    accuracy the condition number allows."""
     rng = np.random.default_rng(900 + ps)
     cases = []
     for n, nrhs in [(1, 1), (5, 2), (16, 4), (17, 3), (33, 4), (64, 5), (100, 4),
                     (129, 8), (200, 4), (256, 4), (256, 1), (250, 6)]:
         cond = 10 ** rng.uniform(2, 8)
         cases.append((n, nrhs, _spd_cond(rng, n, cond)))
-    out = _run(ps, cases, 1e-14, 10, 256, 31 + ps)
+    cases.append((256, 4, _spd_cond(rng, 256, 1e8)))  # the top of the range
+    out = _run(ps, cases, 1e-14, 30, 256, 31 + ps)
     for k, (o, (n, nrhs, A)) in enumerate(zip(out, cases)):
         inf = o["info"]
         assert inf["path"] == 0, (k, n, inf)

@@ -422,13 +422,28 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 	(void)maxfoot;
 	for (int cls = RIC_NCLASS - 1; cls >= 0; cls--)
 	{
-		// resident instances per CU: LDS share, and for one-wave instances at
-		// most 3 waves per SIMD (the register cap of k_ric_class1); threads per
-		// instance by footprint.  Both can be overridden by the tuning knobs.
-		int per_cu = RIC_LDS_MAX / h_ric_cap[cls];
-		if (h_ric_cap[cls] <= 40960)
-			per_cu = min(per_cu, 12);
-		int nt = h_ric_cap[cls] <= 40960 ? 64 : (h_ric_cap[cls] <= 81920 ? 128 : 256);
+		// built-in launch shape per class (threads per instance, resident
+		// instances per CU), from autotuner runs on the 8192 x N=20 batch
+		// (nx ~ U{8..64}, nu ~ U{2..16}) on MI355X: one wave and up to 12 per CU
+		// (3 per SIMD, the register cap of k_ric_class1) for footprints up to
+		// 24 KB, two waves and 4 per CU up to 40 KB, four waves and 2 per CU
+		// above.  The tuning knobs override both.
+		int per_cu, nt;
+		if (h_ric_cap[cls] <= 24576)
+		{
+			nt = 64;
+			per_cu = min(RIC_LDS_MAX / h_ric_cap[cls], 12);
+		}
+		else if (h_ric_cap[cls] <= 40960)
+		{
+			nt = 128;
+			per_cu = 4;
+		}
+		else
+		{
+			nt = 256;
+			per_cu = max(1, min(2, RIC_LDS_MAX / h_ric_cap[cls]));
+		}
 		if (const int v = tune_value(TUNE_RIC_PER_CU, cls))
 			per_cu = v;
 		if (const int v = tune_value(TUNE_RIC_THREADS, cls))

@@ -253,7 +253,7 @@ def run_riccati(args, torch, bfa):
         log("checked %d instances: stage backward error <= %.2f n eps, L_0 rel diff vs oracle %.1e"
             % (nchk, berr, fdiff))
 
-    traffic = read_traffic("k_ric_class1", "riccati-%dx%d" % (args.batch, args.horizon))
+    traffic = read_traffic("k_ric_class", "riccati-%dx%d" % (args.batch, args.horizon))
     out = {
         "metric": METRIC,
         "value": solves,
@@ -275,7 +275,7 @@ def run_riccati(args, torch, bfa):
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "algorithmic_bytes": algo_bytes, "flops_per_launch": flops,
-                     "kernel": "k_ric_class1 (+ k_ric_class for the largest instances)"},
+                     "kernel": "k_ric_class<ps,256> + k_ric_class<ps,128> + k_ric_class1<ps> (the launches of one batched call)"},
         "cpu_baseline": cpu,
         "checks": checks,
     }

@@ -461,28 +461,49 @@ __global__ __launch_bounds__(PV_NT) void k_posv_mixed(const blasfeo_amd_dposv_ar
 	PM<double, PS> gX = view<double, PS>(a.sX);
 	const int ai = a.ai, aj = a.aj;
 
-	// A (lower triangle, FP64) -> FP32 tiles; identity beyond n
+	// A (lower triangle, FP64) -> FP32 tiles; identity beyond n.  Wave w takes
+	// tiles w, w + 8, ..., four per step with all sixteen loads issued first.
 	{
-		int tt = 0;
-		for (int I = 0; I < nb; I++)
-			for (int J = 0; J <= I; J++, tt++)
+		const int ntile = nb * (nb + 1) / 2;
+		const int i = lane & 15;
+		for (int t0 = w; t0 < ntile; t0 += 4 * PV_NW)
+		{
+			double v[4][4];
+			int tI[4], tJ[4];
+#pragma unroll
+			for (int u = 0; u < 4; u++)
 			{
-				if (tt % PV_NW != w)
-					continue;
-				float *FT = L.F + tt * 256;
-				const int i = lane & 15;
+				const int t = t0 + u * PV_NW;
+				int I = 0;
+				while ((I + 1) * (I + 2) / 2 <= t)
+					I++;
+				tI[u] = I;
+				tJ[u] = t - I * (I + 1) / 2;
 				const int r = I * 16 + i;
 #pragma unroll
 				for (int k = 0; k < 4; k++)
 				{
-					const int j = 4 * k + (lane >> 4);
-					const int c = J * 16 + j;
-					double v = (r == c) ? 1.0 : 0.0;
-					if (r < n && c < n && r >= c)
-						v = gA.at(ai + r, aj + c);
-					FT[j * 16 + i] = (float)v;
+					const int c = tJ[u] * 16 + 4 * k + (lane >> 4);
+					const bool in = t < ntile && r < n && c < n && r >= c;
+					double e = (r == c) ? 1.0 : 0.0;
+					if (in)
+						e = gA.at(ai + r, aj + c);
+					v[u][k] = e;
 				}
 			}
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+			{
+				const int t = t0 + u * PV_NW;
+				if (t < ntile)
+				{
+					float *FT = L.F + t * 256;
+#pragma unroll
+					for (int k = 0; k < 4; k++)
+						FT[(4 * k + (lane >> 4)) * 16 + i] = (float)v[u][k];
+				}
+			}
+		}
 	}
 	const bool ok = posv_factor(L, nb);
 

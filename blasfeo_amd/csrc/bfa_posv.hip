@@ -122,64 +122,85 @@ __device__ __forceinline__ void block_sum(double *v, double *red)
 	}
 }
 
-// q = A v for the thread's row i (symmetric A from its stored lower triangle,
-// FP64, global memory), v = V[j*4 + c] in LDS.  Thread t takes row i = t & 255
-// and half t >> 8 of the columns; the second half's partial sums go through S
-// and the row's owner (first half) returns the total in q and the row's
-// sum of |a_ij| in sabs.  Columns are walked in chunks of four with the loads
-// of a chunk issued together (element (i,j), j <= i, from row i; j > i from
-// column i, where four consecutive rows of a panel are contiguous).
-template <int PS>
-__device__ __forceinline__ void posv_matvec(const PM<double, PS> gA, int ai, int aj, int n, const double *V,
-	double *S, double *q, double &sabs)
+// Y = A V on the FP64 matrix cores, A symmetric from its stored lower
+// triangle in global memory (16x16 tiles), V = V[j*4 + c] in LDS, Y[i*4 + c]
+// and (ABS) the row sums of |A| in rowabs[i], both in LDS.  Wave w owns tile
+// rows and columns {w, 15 - w} (17 tiles of the direct products A_IJ V_J, 15
+// of the transposed products V_I^T A_IJ for nb = 16), so the two partial sums
+// of a row meet in one wave without atomics and the result does not depend
+// on timing.  The direct products read a tile as 16-row by 4-column slabs,
+// the transposed ones as 4-row by 16-column slabs, which are contiguous runs
+// of the panel-major storage.  Ends with a barrier.
+template <int PS, bool ABS>
+__device__ void posv_matvec_mfma(const PM<double, PS> gA, int ai, int aj, int n, int nb,
+	const double *V, double *Y, double *rowabs)
 {
-	const int t = threadIdx.x;
-	const int h = t >> 8, i = t & 255;
-	const int jm = ((n + 7) >> 3) << 2;
-	const int jlo = h ? jm : 0, jhi = h ? n : jm;
-	double acc[PV_R] = {0, 0, 0, 0};
-	double sa = 0.0;
-	if (i < n)
+	const int lane = lane_id(), w = wave_id();
+	const int c16 = lane & 15, q = lane >> 4;
+	for (int sel = 0; sel < 2; sel++)
 	{
-#pragma unroll 4
-		for (int j0 = jlo; j0 < jhi; j0 += 4)
+		const int I = sel == 0 ? w : 2 * PV_NW - 1 - w;
+		if (I >= nb)
+			continue;
+		d4_t acc_d = d4_t{0, 0, 0, 0}, abs_d = d4_t{0, 0, 0, 0};
+		d4_t acc_t = d4_t{0, 0, 0, 0}, abs_t = d4_t{0, 0, 0, 0};
+		// direct: y_I += A_IJ v_J for J <= I (the diagonal tile completed symmetrically)
+		const int r = 16 * I + c16;
+		for (int J = 0; J <= I; J++)
 		{
 			double a[4];
 #pragma unroll
 			for (int u = 0; u < 4; u++)
 			{
-				// past the end of the range, load a valid element and drop it
-				const int j = min(j0 + u, jhi - 1);
-				const int r = j <= i ? i : j, c = j <= i ? j : i;
-				const double e = gA.at(ai + r, aj + c);
-				a[u] = j0 + u < jhi ? e : 0.0;
+				const int cidx = 16 * J + 4 * u + q;
+				const bool up = cidx > r;
+				const int rr = min(up ? cidx : r, n - 1), cc = min(up ? r : cidx, n - 1);
+				const double e = gA.at(ai + rr, aj + cc);
+				a[u] = (r < n && cidx < n) ? e : 0.0;
 			}
 #pragma unroll
 			for (int u = 0; u < 4; u++)
 			{
-				const int j = min(j0 + u, jhi - 1);
-				const double *vj = V + j * PV_R;
-#pragma unroll
-				for (int c = 0; c < PV_R; c++)
-					acc[c] += a[u] * vj[c];
-				sa += fabs(a[u]);
+				const double b = c16 < PV_R ? V[(16 * J + 4 * u + q) * PV_R + c16] : 0.0;
+				acc_d = mfma16(a[u], b, acc_d);
+				if (ABS)
+					abs_d = mfma16(fabs(a[u]), c16 == 0 ? 1.0 : 0.0, abs_d);
 			}
 		}
-	}
-	if (h == 1 && i < n)
-	{
+		// transposed: y_I^T += v_K^T A_KI for K > I
+		for (int K = I + 1; K < nb; K++)
+		{
+			double bt[4];
 #pragma unroll
-		for (int c = 0; c < PV_R; c++)
-			S[i * 5 + c] = acc[c];
-		S[i * 5 + 4] = sa;
-	}
-	__syncthreads();
-	if (h == 0 && i < n)
-	{
+			for (int u = 0; u < 4; u++)
+			{
+				const int rk = 16 * K + 4 * u + q, ci = 16 * I + c16;
+				const double e = gA.at(ai + min(rk, n - 1), aj + min(ci, n - 1));
+				bt[u] = (rk < n && ci < n) ? e : 0.0;
+			}
 #pragma unroll
-		for (int c = 0; c < PV_R; c++)
-			q[c] = acc[c] + S[i * 5 + c];
-		sabs = sa + S[i * 5 + 4];
+			for (int u = 0; u < 4; u++)
+			{
+				const double av = c16 < PV_R ? V[(16 * K + 4 * u + q) * PV_R + c16] : 0.0;
+				acc_t = mfma16(av, bt[u], acc_t);
+				if (ABS)
+					abs_t = mfma16(c16 == 0 ? 1.0 : 0.0, fabs(bt[u]), abs_t);
+			}
+		}
+		// direct: lane (c16 < 4) holds y_I[q + 4r][c16]; transposed: register 0 holds y_I[c16][q]
+		if (c16 < PV_R)
+#pragma unroll
+			for (int rg = 0; rg < 4; rg++)
+				Y[(16 * I + q + 4 * rg) * PV_R + c16] = acc_d[rg];
+		if (ABS && c16 == 0)
+#pragma unroll
+			for (int rg = 0; rg < 4; rg++)
+				rowabs[16 * I + q + 4 * rg] = abs_d[rg];
+		wave_sync();
+		Y[(16 * I + c16) * PV_R + q] += acc_t[0];
+		if (ABS && q == 0)
+			rowabs[16 * I + c16] += abs_t[0];
+		wave_sync();
 	}
 	__syncthreads();
 }
@@ -573,14 +594,19 @@ __global__ __launch_bounds__(PV_NT) void k_posv_mixed(const blasfeo_amd_dposv_ar
 			__syncthreads();
 			// q = A p ; alpha = rho / p.q ; x += alpha p ; r -= alpha q
 			double qv[PV_R] = {0, 0, 0, 0};
-			double sabs = 0.0;
-			posv_matvec<PS>(gA, ai, aj, n, L.X, L.S, qv, sabs);
 			if (anorm < 0.0)
 			{
-				double an = sabs;
+				posv_matvec_mfma<PS, true>(gA, ai, aj, n, nb, L.X, L.S, L.S + n16 * PV_R);
+				double an = owner ? L.S[n16 * PV_R + i] : 0.0;
 				block_max<1>(&an, L.red);
 				anorm = an;
 			}
+			else
+				posv_matvec_mfma<PS, false>(gA, ai, aj, n, nb, L.X, L.S, nullptr);
+			if (owner)
+#pragma unroll
+				for (int c = 0; c < PV_R; c++)
+					qv[c] = L.S[i * PV_R + c];
 #pragma unroll
 			for (int c = 0; c < PV_R; c++)
 				v[c] = pr[c] * qv[c];
@@ -623,8 +649,11 @@ __global__ __launch_bounds__(PV_NT) void k_posv_mixed(const blasfeo_amd_dposv_ar
 					L.X[i * PV_R + c] = owner ? xr[c] : 0.0;
 			__syncthreads();
 			double ax[PV_R] = {0, 0, 0, 0};
-			double sab2 = 0.0;
-			posv_matvec<PS>(gA, ai, aj, n, L.X, L.S, ax, sab2);
+			posv_matvec_mfma<PS, false>(gA, ai, aj, n, nb, L.X, L.S, nullptr);
+			if (owner)
+#pragma unroll
+				for (int c = 0; c < PV_R; c++)
+					ax[c] = L.S[i * PV_R + c];
 #pragma unroll
 			for (int c = 0; c < PV_R; c++)
 			{

@@ -168,155 +168,127 @@ namespace bfa {
 // used, n <= NMAX <= 128), in place: M <- L,
 // invd[j] = 1/sqrt(pivot) with the zero-pivot rule above (L_jj = c*invd[j]).
 // Rows and columns n .. ceil16(n)-1 must hold zeros on entry.
-// Wave 0 factors a whole 16-column panel (diagonal block and the rows below,
-// two rows per lane, one when NMAX <= 64) in registers, broadcasting L(jj,j)
-// with v_readlane, so the panel needs no workgroup barrier.  The rank-16
-// trailing updates run on the matrix cores in tasks of up to four tiles of one
-// block column.  Look-ahead of one panel: in step jb wave 0 applies panel jb to
-// block column jb+1 and factors panel jb+1 while the other waves apply panel
-// jb to block columns jb+2.. (with one wave, it does both in turn), so the
-// serial panel chain overlaps the bulk of the update and each step costs one
-// workgroup barrier.
+// Per 16-column block: wave 0 factors the whole panel (diagonal block and the
+// rows below, two rows per lane, one when NMAX <= 64) in registers, broadcasting L(jj,j) with
+// v_readlane, so the panel needs no workgroup barrier; then the waves apply
+// the rank-16 trailing update to the lower tiles on the matrix cores, one
+// block column of tiles at a time with independent accumulators.
 // Called by all NT threads of the workgroup; starts and ends with a barrier.
 // ---------------------------------------------------------------------------
-
-// Wave 0: factor panel jb in registers and store it back.  The panel and the
-// update task are kept out of line so that their registers are not added to
-// the caller's (the one-wave Riccati kernel must stay within 168 VGPRs).
-template <typename T, int NMAX, typename ACC>
-__device__ __attribute__((noinline)) void chol_panel(const ACC M, int n, T *invd, int jb)
-{
-	const int lane = lane_id();
-	const int c0 = jb * 16;
-	const int cw = min(16, n - c0);
-	const int nr = n - c0;
-	T x0[16], x1[16];
-	const bool r0ok = lane < nr, r1ok = NMAX > 64 && lane + 64 < nr;
-#pragma unroll
-	for (int j = 0; j < 16; j++)
-	{
-		x0[j] = r0ok ? M(c0 + lane, c0 + j) : T(0);
-		x1[j] = r1ok ? M(c0 + 64 + lane, c0 + j) : T(0);
-	}
-#pragma unroll
-	for (int j = 0; j < 16; j++)
-	{
-		if (j < cw)
-		{
-			const T c = readlane(x0[j], j);
-			const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
-			if (lane > j)
-				x0[j] *= inv;
-			else if (lane == j)
-				x0[j] = c * inv;
-			x1[j] *= inv;
-			if (lane == 0)
-				invd[c0 + j] = inv;
-#pragma unroll
-			for (int jj = j + 1; jj < 16; jj++)
-			{
-				if (jj < cw)
-				{
-					const T l = readlane(x0[j], jj);
-					if (lane >= jj)
-						x0[jj] -= x0[j] * l;
-					x1[jj] -= x1[j] * l;
-				}
-			}
-		}
-	}
-#pragma unroll
-	for (int j = 0; j < 16; j++)
-	{
-		if (r0ok)
-			M(c0 + lane, c0 + j) = x0[j];
-		if (r1ok)
-			M(c0 + 64 + lane, c0 + j) = x1[j];
-	}
-}
-
-// One wave: tiles g..g+3 (those < nt) of block column bj -= panel jb's rows.
-template <typename T, typename ACC>
-__device__ __attribute__((noinline)) void chol_update_task(const ACC M, int nt, int jb, int bj, int g)
-{
-	typedef typename Vec4<T>::type V4;
-	const int lane = lane_id();
-	const int c0 = jb * 16;
-	V4 acc[4];
-#pragma unroll
-	for (int t = 0; t < 4; t++)
-	{
-		if (g + t < nt)
-#pragma unroll
-			for (int r = 0; r < 4; r++)
-				acc[t][r] = M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + (lane & 15));
-	}
-#pragma unroll
-	for (int kk = 0; kk < 16; kk += 4)
-	{
-		const int kq = c0 + kk + (lane >> 4);
-		const T b = M(bj * 16 + (lane & 15), kq);
-#pragma unroll
-		for (int t = 0; t < 4; t++)
-		{
-			if (g + t < nt)
-			{
-				const T a = M((g + t) * 16 + (lane & 15), kq);
-				acc[t] = mfma16(-a, b, acc[t]);
-			}
-		}
-	}
-#pragma unroll
-	for (int t = 0; t < 4; t++)
-	{
-		if (g + t < nt)
-#pragma unroll
-			for (int r = 0; r < 4; r++)
-				M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + (lane & 15)) = acc[t][r];
-	}
-}
-
 template <typename T, int NT, int NMAX, typename ACC>
 __device__ void chol_acc(const ACC M, int n, T *invd)
 {
+	typedef typename Vec4<T>::type V4;
 	constexpr int NW = NT / WAVE;
+	const int lane = lane_id();
 	const int w = wave_id();
 	const int nt = (n + 15) / 16;
 	__syncthreads();
-	if (w == 0 && nt > 0)
-		chol_panel<T, NMAX>(M, n, invd, 0);
-	__syncthreads();
-	for (int jb = 0; jb + 1 < nt; jb++)
+	for (int jb = 0; jb < nt; jb++)
 	{
+		const int c0 = jb * 16;
+		const int cw = min(16, n - c0);
+		const int nr = n - c0;
 		if (w == 0)
 		{
-			// block column jb+1, then its panel
-			for (int g = jb + 1; g < nt; g += 4)
-				chol_update_task<T>(M, nt, jb, jb + 1, g);
-			wave_sync();
-			chol_panel<T, NMAX>(M, n, invd, jb + 1);
-		}
-		if (w > 0 || NW == 1)
-		{
-			// block columns jb+2 .. nt-1: tasks (block column, group of up to 4 tiles)
-			int ntask = 0;
-			for (int bj = jb + 2; bj < nt; bj++)
-				ntask += (nt - bj + 3) / 4;
-			const int w0 = NW == 1 ? 0 : w - 1, ws = NW == 1 ? 1 : NW - 1;
-			for (int task = w0; task < ntask; task += ws)
+			T x0[16], x1[16];
+			const bool r0ok = lane < nr, r1ok = NMAX > 64 && lane + 64 < nr;
+#pragma unroll
+			for (int j = 0; j < 16; j++)
 			{
-				int bj = jb + 2, rem = task;
+				x0[j] = r0ok ? M(c0 + lane, c0 + j) : T(0);
+				x1[j] = r1ok ? M(c0 + 64 + lane, c0 + j) : T(0);
+			}
+#pragma unroll
+			for (int j = 0; j < 16; j++)
+			{
+				if (j < cw)
+				{
+					const T c = readlane(x0[j], j);
+					const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
+					if (lane > j)
+						x0[j] *= inv;
+					else if (lane == j)
+						x0[j] = c * inv;
+					x1[j] *= inv;
+					if (lane == 0)
+						invd[c0 + j] = inv;
+#pragma unroll
+					for (int jj = j + 1; jj < 16; jj++)
+					{
+						if (jj < cw)
+						{
+							const T l = readlane(x0[j], jj);
+							if (lane >= jj)
+								x0[jj] -= x0[j] * l;
+							x1[jj] -= x1[j] * l;
+						}
+					}
+				}
+			}
+#pragma unroll
+			for (int j = 0; j < 16; j++)
+			{
+				if (r0ok)
+					M(c0 + lane, c0 + j) = x0[j];
+				if (r1ok)
+					M(c0 + 64 + lane, c0 + j) = x1[j];
+			}
+		}
+		__syncthreads();
+		// trailing update: tasks (block column bj > jb, group of up to 4 tiles
+		// bi = g..g+3 >= bj) spread over the waves, independent accumulators
+		{
+			int ntask = 0;
+			for (int bj = jb + 1; bj < nt; bj++)
+				ntask += (nt - bj + 3) / 4;
+			for (int task = w; task < ntask; task += NW)
+			{
+				int bj = jb + 1, rem = task;
 				while (rem >= (nt - bj + 3) / 4)
 				{
 					rem -= (nt - bj + 3) / 4;
 					bj++;
 				}
-				chol_update_task<T>(M, nt, jb, bj, bj + 4 * rem);
+				const int g = bj + 4 * rem;
+				V4 acc[4];
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+				{
+					if (g + t < nt)
+#pragma unroll
+						for (int r = 0; r < 4; r++)
+							acc[t][r] = M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + (lane & 15));
+				}
+#pragma unroll
+				for (int kk = 0; kk < 16; kk += 4)
+				{
+					const int kq = c0 + kk + (lane >> 4);
+					const T b = M(bj * 16 + (lane & 15), kq);
+#pragma unroll
+					for (int t = 0; t < 4; t++)
+					{
+						if (g + t < nt)
+						{
+							const T a = M((g + t) * 16 + (lane & 15), kq);
+							acc[t] = mfma16(-a, b, acc[t]);
+						}
+					}
+				}
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+				{
+					if (g + t < nt)
+#pragma unroll
+						for (int r = 0; r < 4; r++)
+							M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + (lane & 15)) = acc[t][r];
+				}
 			}
 		}
 		__syncthreads();
 	}
 }
+
 
 // Column-major LDS matrix: element (r, c) at p[c*ld + r].
 template <typename T>

@@ -144,47 +144,68 @@ __device__ void posv_matvec_mfma(const PM<double, PS> gA, int ai, int aj, int n,
 			continue;
 		d4_t acc_d = d4_t{0, 0, 0, 0}, abs_d = d4_t{0, 0, 0, 0};
 		d4_t acc_t = d4_t{0, 0, 0, 0}, abs_t = d4_t{0, 0, 0, 0};
-		// direct: y_I += A_IJ v_J for J <= I (the diagonal tile completed symmetrically)
+		// direct: y_I += A_IJ v_J for J <= I (the diagonal tile completed
+		// symmetrically); four tiles per step, their 16 loads issued together
 		const int r = 16 * I + c16;
-		for (int J = 0; J <= I; J++)
+		for (int J0 = 0; J0 <= I; J0 += 4)
 		{
-			double a[4];
+			double a[4][4];
 #pragma unroll
-			for (int u = 0; u < 4; u++)
+			for (int t = 0; t < 4; t++)
 			{
-				const int cidx = 16 * J + 4 * u + q;
-				const bool up = cidx > r;
-				const int rr = min(up ? cidx : r, n - 1), cc = min(up ? r : cidx, n - 1);
-				const double e = gA.at(ai + rr, aj + cc);
-				a[u] = (r < n && cidx < n) ? e : 0.0;
+				const int J = min(J0 + t, I);
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+				{
+					const int cidx = 16 * J + 4 * u + q;
+					const bool up = cidx > r;
+					const int rr = min(up ? cidx : r, n - 1), cc = min(up ? r : cidx, n - 1);
+					const double e = gA.at(ai + rr, aj + cc);
+					a[t][u] = (J0 + t <= I && r < n && cidx < n) ? e : 0.0;
+				}
 			}
 #pragma unroll
-			for (int u = 0; u < 4; u++)
+			for (int t = 0; t < 4; t++)
 			{
-				const double b = c16 < PV_R ? V[(16 * J + 4 * u + q) * PV_R + c16] : 0.0;
-				acc_d = mfma16(a[u], b, acc_d);
-				if (ABS)
-					abs_d = mfma16(fabs(a[u]), c16 == 0 ? 1.0 : 0.0, abs_d);
+				const int J = min(J0 + t, I);
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+				{
+					const double b = c16 < PV_R ? V[(16 * J + 4 * u + q) * PV_R + c16] : 0.0;
+					acc_d = mfma16(a[t][u], b, acc_d);
+					if (ABS)
+						abs_d = mfma16(fabs(a[t][u]), c16 == 0 ? 1.0 : 0.0, abs_d);
+				}
 			}
 		}
-		// transposed: y_I^T += v_K^T A_KI for K > I
-		for (int K = I + 1; K < nb; K++)
+		// transposed: y_I^T += v_K^T A_KI for K > I, four tiles per step
+		for (int K0 = I + 1; K0 < nb; K0 += 4)
 		{
-			double bt[4];
+			double bt[4][4];
 #pragma unroll
-			for (int u = 0; u < 4; u++)
+			for (int t = 0; t < 4; t++)
 			{
-				const int rk = 16 * K + 4 * u + q, ci = 16 * I + c16;
-				const double e = gA.at(ai + min(rk, n - 1), aj + min(ci, n - 1));
-				bt[u] = (rk < n && ci < n) ? e : 0.0;
+				const int K = min(K0 + t, nb - 1);
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+				{
+					const int rk = 16 * K + 4 * u + q, ci = 16 * I + c16;
+					const double e = gA.at(ai + min(rk, n - 1), aj + min(ci, n - 1));
+					bt[t][u] = (K0 + t < nb && rk < n && ci < n) ? e : 0.0;
+				}
 			}
 #pragma unroll
-			for (int u = 0; u < 4; u++)
+			for (int t = 0; t < 4; t++)
 			{
-				const double av = c16 < PV_R ? V[(16 * K + 4 * u + q) * PV_R + c16] : 0.0;
-				acc_t = mfma16(av, bt[u], acc_t);
-				if (ABS)
-					abs_t = mfma16(c16 == 0 ? 1.0 : 0.0, fabs(bt[u]), abs_t);
+				const int K = min(K0 + t, nb - 1);
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+				{
+					const double av = c16 < PV_R ? V[(16 * K + 4 * u + q) * PV_R + c16] : 0.0;
+					acc_t = mfma16(av, bt[t][u], acc_t);
+					if (ABS)
+						abs_t = mfma16(c16 == 0 ? 1.0 : 0.0, fabs(bt[t][u]), abs_t);
+				}
 			}
 		}
 		// direct: lane (c16 < 4) holds y_I[q + 4r][c16]; transposed: register 0 holds y_I[c16][q]

@@ -356,8 +356,8 @@ __global__ __launch_bounds__(NT) void k_ric_class(const blasfeo_amd_riccati_arg 
 		ric_sweep_lds<PS, NT>(args[list[t]], smem_d);
 }
 
-// one wave per instance: registers capped at 168 so that at least three
-// instances share each SIMD (the build reaches four).  Only classes of at most 40960 B use it, whose stages have
+// one wave per instance: registers capped at 168 so that three instances
+// share each SIMD.  Only classes of at most 40960 B use it, whose stages have
 // nu+nx <= 64 (the plan's rule), so the Cholesky keeps one panel row per lane.
 template <int PS>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3))) void k_ric_class1(
@@ -424,15 +424,15 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 	{
 		// built-in launch shape per class (threads per instance, resident
 		// instances per CU), from autotuner runs on the 8192 x N=20 batch
-		// (nx ~ U{8..64}, nu ~ U{2..16}) on MI355X: one wave and up to 16 per CU
-		// (4 per SIMD: k_ric_class1 needs 76 VGPRs + 32 AGPRs) for footprints up to
+		// (nx ~ U{8..64}, nu ~ U{2..16}) on MI355X: one wave and up to 12 per CU
+		// (3 per SIMD, the register cap of k_ric_class1) for footprints up to
 		// 24 KB, two waves and 4 per CU up to 40 KB, four waves and 2 per CU
 		// above.  The tuning knobs override both.
 		int per_cu, nt;
 		if (h_ric_cap[cls] <= 24576)
 		{
 			nt = 64;
-			per_cu = min(RIC_LDS_MAX / h_ric_cap[cls], 16);
+			per_cu = min(RIC_LDS_MAX / h_ric_cap[cls], 12);
 		}
 		else if (h_ric_cap[cls] <= 40960)
 		{

@@ -163,11 +163,12 @@ __device__ void chol_instance(T *P, int LDP, T *L11, T *invs,
 namespace bfa {
 
 // ---------------------------------------------------------------------------
-// In-LDS right-looking blocked Cholesky of an n x n matrix held in LDS and
-// addressed through M(r, c) (column-major, or 16x16 tiles; lower triangle
-// used, n <= NMAX <= 128), in place: M <- L,
+// In-LDS right-looking blocked Cholesky of the m x n lower trapezoid (m >= n)
+// of a matrix held in LDS and addressed through M(r, c) (column-major, or
+// 16x16 tiles; m <= NMAX <= 128), in place: M <- L (potrf_l_mn semantics:
+// L11 = chol(C11), L21 = C21 L11^{-T}),
 // invd[j] = 1/sqrt(pivot) with the zero-pivot rule above (L_jj = c*invd[j]).
-// Rows and columns n .. ceil16(n)-1 must hold zeros on entry.
+// Rows m .. ceil16(m)-1 and columns n .. ceil16(n)-1 must hold zeros on entry.
 // Per 16-column block: wave 0 factors the whole panel (diagonal block and the
 // rows below, two rows per lane, one when NMAX <= 64) in registers, broadcasting L(jj,j) with
 // v_readlane, so the panel needs no workgroup barrier; then the waves apply
@@ -176,19 +177,20 @@ namespace bfa {
 // Called by all NT threads of the workgroup; starts and ends with a barrier.
 // ---------------------------------------------------------------------------
 template <typename T, int NT, int NMAX, typename ACC>
-__device__ void chol_acc(const ACC M, int n, T *invd)
+__device__ void chol_acc(const ACC M, int m, int n, T *invd)
 {
 	typedef typename Vec4<T>::type V4;
 	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
 	const int w = wave_id();
-	const int nt = (n + 15) / 16;
+	const int nt = (n + 15) / 16;   // panels
+	const int mt = (m + 15) / 16;   // tile rows
 	__syncthreads();
 	for (int jb = 0; jb < nt; jb++)
 	{
 		const int c0 = jb * 16;
 		const int cw = min(16, n - c0);
-		const int nr = n - c0;
+		const int nr = m - c0;
 		if (w == 0)
 		{
 			T x0[16], x1[16];
@@ -241,13 +243,13 @@ __device__ void chol_acc(const ACC M, int n, T *invd)
 		{
 			int ntask = 0;
 			for (int bj = jb + 1; bj < nt; bj++)
-				ntask += (nt - bj + 3) / 4;
+				ntask += (mt - bj + 3) / 4;
 			for (int task = w; task < ntask; task += NW)
 			{
 				int bj = jb + 1, rem = task;
-				while (rem >= (nt - bj + 3) / 4)
+				while (rem >= (mt - bj + 3) / 4)
 				{
-					rem -= (nt - bj + 3) / 4;
+					rem -= (mt - bj + 3) / 4;
 					bj++;
 				}
 				const int g = bj + 4 * rem;
@@ -255,7 +257,7 @@ __device__ void chol_acc(const ACC M, int n, T *invd)
 #pragma unroll
 				for (int t = 0; t < 4; t++)
 				{
-					if (g + t < nt)
+					if (g + t < mt)
 #pragma unroll
 						for (int r = 0; r < 4; r++)
 							acc[t][r] = M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + (lane & 15));
@@ -268,7 +270,7 @@ __device__ void chol_acc(const ACC M, int n, T *invd)
 #pragma unroll
 					for (int t = 0; t < 4; t++)
 					{
-						if (g + t < nt)
+						if (g + t < mt)
 						{
 							const T a = M((g + t) * 16 + (lane & 15), kq);
 							acc[t] = mfma16(-a, b, acc[t]);
@@ -278,7 +280,7 @@ __device__ void chol_acc(const ACC M, int n, T *invd)
 #pragma unroll
 				for (int t = 0; t < 4; t++)
 				{
-					if (g + t < nt)
+					if (g + t < mt)
 #pragma unroll
 						for (int r = 0; r < 4; r++)
 							M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + (lane & 15)) = acc[t][r];
@@ -302,7 +304,7 @@ struct ColMajor
 template <typename T, int NT = NTL, int NMAX = 128>
 __device__ __forceinline__ void chol_lds(T *M, int ld, int n, T *invd)
 {
-	chol_acc<T, NT, NMAX>(ColMajor<T>{M, ld}, n, invd);
+	chol_acc<T, NT, NMAX>(ColMajor<T>{M, ld}, n, n, invd);
 }
 
 } // namespace bfa

@@ -25,6 +25,9 @@ template <typename T> struct FactArg;
 template <> struct FactArg<double> { typedef blasfeo_amd_dfact_arg type; };
 template <> struct FactArg<float> { typedef blasfeo_amd_sfact_arg type; };
 
+template <typename T> struct SPArg;
+template <> struct SPArg<double> { typedef blasfeo_amd_dsyrk_dpotrf_arg type; };
+
 // mode 0: potrf_l (use_dA = m), 1: potrf_l_mn (use_dA = 1)
 template <typename T, int PS, int MODE>
 __global__ __launch_bounds__(NTL) void k_potrf(const typename FactArg<T>::type *__restrict__ args, int mmax)
@@ -48,10 +51,11 @@ __global__ __launch_bounds__(NTL) void k_potrf(const typename FactArg<T>::type *
 		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? (MODE == 0 ? m : 1) : 0;
 }
 
-// potrf_l with the whole matrix in LDS (n <= 128): stage the lower triangle
-// of C, factor in place (chol_lds), stream L and dA out.  NT = 64 (one wave)
-// for n <= 32 so that several instances share a CU.
-template <typename T, int PS, int NT>
+// potrf_l (MN = false) and potrf_l_mn (MN = true) with the whole m x n lower
+// trapezoid in LDS (m <= 128): stage it from C, factor in place (chol_acc),
+// stream L and dA out.  NT = 64 (one wave) for m <= 32 so that several
+// instances share a CU.
+template <typename T, int PS, int NT, bool MN>
 __global__ __launch_bounds__(NT) void k_potrf_lds(const typename FactArg<T>::type *__restrict__ args, int np)
 {
 	extern __shared__ unsigned char smem_raw[];
@@ -60,17 +64,104 @@ __global__ __launch_bounds__(NT) void k_potrf_lds(const typename FactArg<T>::typ
 	T *invd = M + (size_t)np * LDM;
 	const typename FactArg<T>::type &a = args[blockIdx.x];
 	const int m = a.m;
-	if (m <= 0 || m > np)
+	int n = MN ? a.n : a.m;
+	if (m <= 0 || n <= 0 || m > np)
 		return;
-	const int mt = (m + 15) / 16;
-	g2s<T, PS, NT, false, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, mt * 16, m, m, M, LDM);
-	chol_lds<T, NT>(M, LDM, m, invd);
-	s2g<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, m, M, LDM, 0);
+	if (n > m)
+		n = m;
+	const int mt = (m + 15) / 16, nt = (n + 15) / 16;
+	g2s<T, PS, NT, false, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, nt * 16, m, n, M, LDM);
+	chol_acc<T, NT, 128>(ColMajor<T>{M, LDM}, m, n, invd);
+	s2g<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, n, M, LDM, 0);
 	T *dD = a.sD->dA;
-	for (int j = threadIdx.x; j < m; j += NT)
+	for (int j = threadIdx.x; j < n; j += NT)
 		dD[j] = invd[j];
 	if (threadIdx.x == 0)
-		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? m : 0;
+		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? (MN ? 1 : m) : 0;
+}
+
+// syrk_dpotrf_ln(_mn), m <= 128: D = chol(C + A B^T) on the m x n lower
+// trapezoid (blasfeo_hp_pm/d_lapack_lib4.c:1848-2077).  C's trapezoid is staged
+// in LDS, A B^T is added tile by tile on the matrix cores with A and B read
+// from global memory (a group of up to four row tiles shares B's operand), and
+// the sum is factored in place (chol_acc).
+template <typename T, int PS, int NT>
+__global__ __launch_bounds__(NT) void k_syrk_potrf_lds(const typename SPArg<T>::type *__restrict__ args, int np)
+{
+	typedef typename Vec4<T>::type V4;
+	constexpr int NW = NT / WAVE;
+	extern __shared__ unsigned char smem_raw[];
+	const int LDM = np + 1;
+	T *M = reinterpret_cast<T *>(smem_raw);
+	T *invd = M + (size_t)np * LDM;
+	const typename SPArg<T>::type &a = args[blockIdx.x];
+	const int m = a.m, k = a.k;
+	int n = a.n;
+	if (m <= 0 || n <= 0 || m > np || k < 0)
+		return;
+	if (n > m)
+		n = m;
+	const int lane = lane_id(), w = wave_id();
+	const int mt = (m + 15) / 16, nt = (n + 15) / 16;
+	const ColMajor<T> Ml{M, LDM};
+	g2s<T, PS, NT, false, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, nt * 16, m, n, M, LDM);
+	__syncthreads();
+	if (k > 0)
+	{
+		const PM<T, PS> gA = view<T, PS>(a.sA), gB = view<T, PS>(a.sB);
+		int ntask = 0;
+		for (int bj = 0; bj < nt; bj++)
+			ntask += (mt - bj + 3) / 4;
+		for (int task = w; task < ntask; task += NW)
+		{
+			int bj = 0, rem = task;
+			while (rem >= (mt - bj + 3) / 4)
+			{
+				rem -= (mt - bj + 3) / 4;
+				bj++;
+			}
+			const int g = bj + 4 * rem;
+			V4 acc[4];
+#pragma unroll
+			for (int t = 0; t < 4; t++)
+				acc[t] = V4{0, 0, 0, 0};
+			const int jr = bj * 16 + (lane & 15);     // row of B for this lane's operand
+#pragma unroll 1
+			for (int kk = 0; kk < k; kk += 4)
+			{
+				const int kq = kk + (lane >> 4);
+				const bool kok = kq < k;
+				const T bv = (kok && jr < n) ? gB.at(a.bi + jr, a.bj + kq) : T(0);
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+				{
+					if (g + t < mt)
+					{
+						const int ir = (g + t) * 16 + (lane & 15);
+						const T av = (kok && ir < m) ? gA.at(a.ai + ir, a.aj + kq) : T(0);
+						acc[t] = mfma16(av, bv, acc[t]);
+					}
+				}
+			}
+#pragma unroll
+			for (int t = 0; t < 4; t++)
+				if (g + t < mt)
+#pragma unroll
+					for (int r = 0; r < 4; r++)
+					{
+						const int ir = (g + t) * 16 + acc_row<T>(lane, r), jc = bj * 16 + (lane & 15);
+						if (ir < m && jc < n && ir >= jc)
+							Ml(ir, jc) += acc[t][r];
+					}
+		}
+	}
+	chol_acc<T, NT, 128>(Ml, m, n, invd);
+	s2g<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, n, M, LDM, 0);
+	T *dD = a.sD->dA;
+	for (int j = threadIdx.x; j < n; j += NT)
+		dD[j] = invd[j];
+	if (threadIdx.x == 0)
+		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -303,8 +394,6 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? n : 0;
 }
 
-template <typename T> struct SPArg;
-template <> struct SPArg<double> { typedef blasfeo_amd_dsyrk_dpotrf_arg type; };
 
 template <typename T, int PS>
 __global__ __launch_bounds__(NTL) void k_syrk_potrf(const typename SPArg<T>::type *__restrict__ args, int mmax)
@@ -546,30 +635,32 @@ static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args
 	int mmax;
 	if ((rc = fact_mmax_arg<T>(hint, name, &mmax)))
 		return rc;
-	if (MODE == 0 && mmax <= 128)
+	if (mmax <= 128)
 	{
 		const int np = (mmax + 15) / 16 * 16;
 		const size_t shm2 = sizeof(T) * ((size_t)np * (np + 1) + np);
 		if (np <= 32)
 		{
 			if (ps == 4)
-				hipLaunchKernelGGL((k_potrf_lds<T, 4, 64>), dim3(batch), dim3(64), shm2, st, args, np);
+				hipLaunchKernelGGL((k_potrf_lds<T, 4, 64, MODE == 1>), dim3(batch), dim3(64), shm2, st, args, np);
 			else if (ps == 8)
-				hipLaunchKernelGGL((k_potrf_lds<T, 8, 64>), dim3(batch), dim3(64), shm2, st, args, np);
+				hipLaunchKernelGGL((k_potrf_lds<T, 8, 64, MODE == 1>), dim3(batch), dim3(64), shm2, st, args, np);
 			else
-				hipLaunchKernelGGL((k_potrf_lds<T, 16, 64>), dim3(batch), dim3(64), shm2, st, args, np);
+				hipLaunchKernelGGL((k_potrf_lds<T, 16, 64, MODE == 1>), dim3(batch), dim3(64), shm2, st, args, np);
 		}
 		else
 		{
 			if (ps == 4)
-				hipLaunchKernelGGL((k_potrf_lds<T, 4, NTL>), dim3(batch), dim3(NTL), shm2, st, args, np);
+				hipLaunchKernelGGL((k_potrf_lds<T, 4, NTL, MODE == 1>), dim3(batch), dim3(NTL), shm2, st, args, np);
 			else if (ps == 8)
-				hipLaunchKernelGGL((k_potrf_lds<T, 8, NTL>), dim3(batch), dim3(NTL), shm2, st, args, np);
+				hipLaunchKernelGGL((k_potrf_lds<T, 8, NTL, MODE == 1>), dim3(batch), dim3(NTL), shm2, st, args, np);
 			else
-				hipLaunchKernelGGL((k_potrf_lds<T, 16, NTL>), dim3(batch), dim3(NTL), shm2, st, args, np);
+				hipLaunchKernelGGL((k_potrf_lds<T, 16, NTL, MODE == 1>), dim3(batch), dim3(NTL), shm2, st, args, np);
 		}
 		return check_launch(name);
 	}
+	if (MODE == 1)
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: m up to 128 supported (the LDS-resident form); got a bound of %d", name, mmax);
 	if (MODE == 0 && mmax <= PT_NMAX)
 	{
 		const size_t shm3 = potrf_tiles_lds_bytes<T>();
@@ -618,11 +709,24 @@ static int launch_syrk_potrf(int ps, int batch, const blasfeo_amd_dsyrk_dpotrf_a
 	int mmax;
 	if ((rc = fact_mmax_arg<double>(hint, name, &mmax)))
 		return rc;
-	const size_t shm = sizeof(double) * PW * ((mmax + 15) / 16 * 16 + 1);
-	if (ps == 4)
-		hipLaunchKernelGGL((k_syrk_potrf<double, 4>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+	if (mmax > 128)
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: m up to 128 supported (the LDS-resident form); got a bound of %d", name, mmax);
+	const int np = (mmax + 15) / 16 * 16;
+	const size_t shm = sizeof(double) * ((size_t)np * (np + 1) + np);
+	if (np <= 32)
+	{
+		if (ps == 4)
+			hipLaunchKernelGGL((k_syrk_potrf_lds<double, 4, 64>), dim3(batch), dim3(64), shm, st, args, np);
+		else
+			hipLaunchKernelGGL((k_syrk_potrf_lds<double, 8, 64>), dim3(batch), dim3(64), shm, st, args, np);
+	}
 	else
-		hipLaunchKernelGGL((k_syrk_potrf<double, 8>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+	{
+		if (ps == 4)
+			hipLaunchKernelGGL((k_syrk_potrf_lds<double, 4, NTL>), dim3(batch), dim3(NTL), shm, st, args, np);
+		else
+			hipLaunchKernelGGL((k_syrk_potrf_lds<double, 8, NTL>), dim3(batch), dim3(NTL), shm, st, args, np);
+	}
 	return check_launch(name);
 }
 
@@ -634,25 +738,14 @@ extern "C" {
 
 int blasfeo_amd_dpotrf_l_batched(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
 { return launch_potrf<double, 0>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dpotrf_l_batched"); }
-// Switched off: its first GPU run ended in an illegal-address fault whose
-// cause has not been found (every global access of the kernel was checked
-// against the operands' bounds on the host); it stays disabled until one is.
+// The LDS-resident forms (m <= 128).  The global-memory forms of these two
+// routines (k_potrf<T,PS,1> and k_syrk_potrf, 16-column panels through global
+// memory) each ended a GPU run in a memory fault whose cause was not found, so
+// they are not launched: larger bounds return BLASFEO_AMD_ETOOBIG.
 int blasfeo_amd_dpotrf_l_mn_batched(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
-{
-	(void)ps; (void)batch; (void)args; (void)hint; (void)s;
-	return set_error(BLASFEO_AMD_EINVAL, "blasfeo_amd_dpotrf_l_mn_batched: disabled in this build");
-}
-// Switched off: a GPU run ended in a memory-aperture fault at the
-// synchronisation after this kernel; every global access of the kernel was
-// checked against the operands' bounds on the host without finding the cause,
-// so it stays disabled until one is shown.  The fused Riccati sweep computes
-// the same step with its own kernels.
+{ return launch_potrf<double, 1>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dpotrf_l_mn_batched"); }
 int blasfeo_amd_dsyrk_dpotrf_ln_batched(int ps, int batch, const blasfeo_amd_dsyrk_dpotrf_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
-{
-	(void)ps; (void)batch; (void)args; (void)hint; (void)s;
-	(void)&launch_syrk_potrf;
-	return set_error(BLASFEO_AMD_EINVAL, "blasfeo_amd_dsyrk_dpotrf_ln_batched: disabled in this build");
-}
+{ return launch_syrk_potrf(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dsyrk_dpotrf_ln_batched"); }
 int blasfeo_amd_dgetrf_rp_batched(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
 { return launch_getrf<double>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dgetrf_rp_batched"); }
 int blasfeo_amd_spotrf_l_batched(int ps, int batch, const blasfeo_amd_sfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)

@@ -207,7 +207,7 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 		const int m = nx[N];
 		const int mt = (m + 15) / 16;
 		g2s_acc<double, PS, NT, true>(view<double, PS>(&RSQ[N]), 0, 0, mt * 16, mt * 16, m, m, M);
-		chol_acc<double, NT, NMAXC>(M, m, invd);
+		chol_acc<double, NT, NMAXC>(M, m, m, invd);
 		s2g_acc<double, PS, NT, true>(view<double, PS>(&L[N]), 0, 0, m, m, M, 0);
 		for (int j = threadIdx.x; j < m; j += NT)
 			L[N].dA[j] = invd[j];
@@ -232,7 +232,7 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 		__syncthreads();
 		ric_syrk_m<NT>(M, W, nkt, n1);
 		// L[k] = chol(M)
-		chol_acc<double, NT, NMAXC>(M, nk, invd);
+		chol_acc<double, NT, NMAXC>(M, nk, nk, invd);
 		s2g_acc<double, PS, NT, true>(view<double, PS>(&L[k]), 0, 0, nk, nk, M, 0);
 		for (int j = threadIdx.x; j < nk; j += NT)
 			L[k].dA[j] = invd[j];

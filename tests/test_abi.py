@@ -181,3 +181,13 @@ def test_tuning_knobs_without_gpu():
     assert at.ric_class(64, 48) == 5
     assert at.ric_class(65, 8) >= 5          # nu+nx > 64 never takes a one-wave class
     assert at.ric_class(60, 70) == 8 and at.ric_class(130, 10) == 8
+
+
+
+def test_lds_only_entries_reject_large_bounds():
+    """Trapezoid potrf and fused syrk+potrf run only in their LDS-resident
+    form (m <= 128); larger bounds are refused before any launch."""
+    L = bfa.lib()
+    h = bfa.HINT(200, 200, 0)
+    assert L.blasfeo_amd_dpotrf_l_mn_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
+    assert L.blasfeo_amd_dsyrk_dpotrf_ln_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4

@@ -55,8 +55,9 @@ def test_potrf(gpu, prec, ps, op):
     if prec == "s" and op == "potrf_l_mn":
         pytest.skip("no FP32 potrf_l_mn entry")
     if op == "potrf_l_mn":
-        pytest.skip("dpotrf_l_mn is switched off: its first GPU run faulted with an illegal "
-                    "address whose cause was not found on the host; kept off the GPU")
+        pytest.skip("sizes beyond 128 need the global-memory form of dpotrf_l_mn, which is not "
+                    "launched (its first GPU run faulted, cause not found); test_potrf_mn_lds "
+                    "covers the LDS-resident form")
     rng = np.random.default_rng(300 + ps)
     b = Batch(prec, ps, seed=1)
     recs = []
@@ -152,8 +153,9 @@ def test_potrf_inplace_and_failures(gpu, ps):
 
 @pytest.mark.parametrize("ps", [4, 8])
 def test_syrk_dpotrf_ln(gpu, ps):
-    pytest.skip("dsyrk_dpotrf_ln is switched off: a GPU run of this case ended in a memory-aperture "
-                "fault whose cause was not found on the host; kept off the GPU")
+    pytest.skip("this case ran the global-memory form of dsyrk_dpotrf_ln, which ended in a "
+                "memory-aperture fault whose cause was not found and is not launched; "
+                "test_syrk_dpotrf_ln_lds covers the LDS-resident form")
     rng = np.random.default_rng(400 + ps)
     b = Batch("d", ps, seed=3)
     recs = []
@@ -176,6 +178,76 @@ def test_syrk_dpotrf_ln(gpu, ps):
 
     g, gd, rr, rd = run_both(b, buf, bfa.DSYRK_DPOTRF_ARG, recs, "dsyrk_dpotrf_ln", ocall, (128, 128, 80))
     compare_buffers(b, g, rr, "dsyrk_dpotrf_ln ps=%d" % ps)
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_potrf_mn_lds(gpu, ps):
+    """dpotrf_l_mn with m <= 128: the m x n lower trapezoid factored in LDS
+    (L11 = chol(C11), L21 = C21 L11^{-T}), offsets, n = m and n < m, in place."""
+    rng = np.random.default_rng(720 + ps)
+    b = Batch("d", ps, seed=13)
+    recs = []
+    shapes = [(1, 1), (5, 3), (16, 16), (17, 9), (31, 31), (40, 17), (64, 33), (65, 64),
+              (100, 64), (127, 100), (128, 128), (128, 1), (90, 90)]
+    for r, (m, n) in enumerate(shapes):
+        ci, cj = (0, 0) if r % 2 == 0 else (int(rng.integers(0, 2 * ps)), int(rng.integers(0, 3)))
+        di, dj = (0, 0) if r % 3 == 0 else (int(rng.integers(0, 2 * ps)), int(rng.integers(0, 3)))
+        S = np.zeros((ci + m, cj + n))
+        S[ci:, cj:] = _spd(rng, m)[:, :n]
+        C = b.mat(ci + m, cj + n, "data", S)
+        if r == 12:
+            D, di, dj = C, ci, cj
+        else:
+            D = b.mat(di + m, dj + n, "const", -1.0)
+        recs.append(dict(m=m, n=n, ci=ci, cj=cj, di=di, dj=dj, sC=C, sD=D, ipiv=0))
+    buf = b.host()
+    f = oracle.fn("d", "potrf_l_mn")
+
+    def ocall(h, r):
+        f(ps, r["m"], r["n"], h.desc_addr(r["sC"]), r["ci"], r["cj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    g, gd, rr, rd = run_both(b, buf, bfa.DFACT_ARG, recs, "dpotrf_l_mn", ocall, (128, 128, 0))
+    compare_buffers(b, g, rr, "dpotrf_l_mn ps=%d" % ps)
+    assert (gd["use_dA"] == rd["use_dA"]).all()
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_syrk_dpotrf_ln_lds(gpu, ps):
+    """dsyrk_dpotrf_ln(_mn) with m <= 128: D = chol(C + A B^T) on the m x n lower
+    trapezoid, A B^T added on the matrix cores from global memory; k = 0,
+    A != B, offsets on every operand."""
+    rng = np.random.default_rng(740 + ps)
+    b = Batch("d", ps, seed=17)
+    recs = []
+    cases = [(1, 1, 1), (7, 7, 0), (16, 16, 4), (33, 20, 9), (48, 48, 33), (64, 64, 64),
+             (80, 80, 64), (100, 37, 5), (128, 128, 70), (120, 120, 1), (25, 25, 100)]
+    for r, (m, n, k) in enumerate(cases):
+        ai, bi, ci = (int(x) for x in rng.integers(0, 2 * ps, 3))
+        aj, bj, cj = (int(x) for x in rng.integers(0, 3, 3))
+        A = b.mat(ai + m, aj + max(k, 1))
+        same = r % 2 == 0
+        B = A if same else b.mat(ai + m, aj + max(k, 1))
+        if same:
+            bi, bj = ai, aj
+        else:
+            bi = ai  # B's rows are read at the same row offset as A's in this test
+        S = np.zeros((ci + m, cj + n))
+        S[ci:, cj:] = (m + 2 * k) * np.eye(m)[:, :n] + 0.01 * rng.standard_normal((m, n))
+        C = b.mat(ci + m, cj + n, "data", S)
+        D = b.mat(m, n, "const", -1.0)
+        recs.append(dict(m=m, n=n, k=k, ai=ai, aj=aj, bi=bi, bj=bj, ci=ci, cj=cj, di=0, dj=0,
+                         sA=A, sB=B, sC=C, sD=D))
+    buf = b.host()
+    f = oracle.fn("d", "syrk_potrf_ln_mn")
+
+    def ocall(h, r):
+        f(ps, r["m"], r["n"], r["k"], h.desc_addr(r["sA"]), r["ai"], r["aj"],
+          h.desc_addr(r["sB"]), r["bi"], r["bj"], h.desc_addr(r["sC"]), r["ci"], r["cj"],
+          h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    g, gd, rr, rd = run_both(b, buf, bfa.DSYRK_DPOTRF_ARG, recs, "dsyrk_dpotrf_ln", ocall, (128, 128, 100))
+    compare_buffers(b, g, rr, "dsyrk_dpotrf_ln ps=%d" % ps)
+    assert (gd["use_dA"] == rd["use_dA"]).all()
 
 
 def _getrf_case(b, rng, m, n, data, ci=0, cj=0, di=0, dj=0, inplace=False):

@@ -159,108 +159,63 @@ __device__ __forceinline__ void s2g(PM<T, PS> g, int r0, int c0, int Rv, int Cv,
 // A(i, c) (element (i, c) of the block).  With LOWER, g2s_acc skips elements
 // whose 16x16 tile lies above the diagonal (packed lower-tile storage has no
 // such tiles) and zero-fills the upper part of diagonal tiles.
-// Each lane walks its (panel, element) positions in order -- panels w, w + NW,
-// ..., elements lane, lane + 64, ... of each panel's run -- and issues the
-// loads of eight positions before storing any, so a wave keeps eight loads in
-// flight instead of waiting on each.  Out of line, so that the eight staged
-// values do not add to the registers of the (long) kernels that call it.
 template <typename T, int PS, int NT, bool LOWER, typename ACC>
-__device__ __attribute__((noinline)) void g2s_acc(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, const ACC A)
+__device__ __forceinline__ void g2s_acc(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, const ACC A)
 {
 	constexpr int NW = NT / WAVE;
-	constexpr int U = 8;
 	const int rr = r0 & (PS - 1);
 	const int np = (rr + R + PS - 1) / PS;
 	const int run = PS * C;
 	const T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
 	const int lane = threadIdx.x & (WAVE - 1);
-	int p = threadIdx.x / WAVE, e = lane;
-	while (p < np)
+	for (int p = threadIdx.x / WAVE; p < np; p += NW)
 	{
-		T v[U];
-		const int p0 = p, e0 = e;
-#pragma unroll
-		for (int u = 0; u < U; u++)
+		const T *src = base + (size_t)p * PS * g.cn;
+		for (int e = lane; e < run; e += WAVE)
 		{
-			const int c = e / PS;
-			const int i = p * PS + (e & (PS - 1)) - rr;
-			T x = T(0);
-			if (p < np && e < run && i >= 0 && i < R && i < Rv && c < Cv && (!LOWER || i >= c))
-				x = base[(size_t)p * PS * g.cn + e];
-			v[u] = x;
-			e += WAVE;
-			if (e >= run)
-			{
-				e = lane;
-				p += NW;
-			}
-		}
-		// the same positions again, for the stores
-		int pp = p0, ee = e0;
-#pragma unroll
-		for (int u = 0; u < U; u++)
-		{
-			const int c = ee / PS;
-			const int i = pp * PS + (ee & (PS - 1)) - rr;
-			if (pp < np && ee < run && i >= 0 && i < R && !(LOWER && (i >> 4) < (c >> 4)))
-				A(i, c) = v[u];
-			ee += WAVE;
-			if (ee >= run)
-			{
-				ee = lane;
-				pp += NW;
-			}
+			const int c = (unsigned)e / PS;
+			const int q = e & (PS - 1);
+			const int i = p * PS + q - rr;
+			if (i < 0 || i >= R)
+				continue;
+			if (LOWER && (i >> 4) < (c >> 4))
+				continue;
+			T v = T(0);
+			if (i < Rv && c < Cv && (!LOWER || i >= c))
+				v = src[e];
+			A(i, c) = v;
 		}
 	}
 }
 
-// Stores walk the positions the same way, eight LDS reads ahead of their
-// global stores.
 template <typename T, int PS, int NT, bool LOWER, typename ACC>
-__device__ __attribute__((noinline)) void s2g_acc(PM<T, PS> g, int r0, int c0, int Rv, int Cv, const ACC A, int diag)
+__device__ __forceinline__ void s2g_acc(PM<T, PS> g, int r0, int c0, int Rv, int Cv, const ACC A, int diag)
 {
 	constexpr int NW = NT / WAVE;
-	constexpr int U = 8;
 	const int rr = r0 & (PS - 1);
 	const int np = (rr + Rv + PS - 1) / PS;
 	const int run = PS * Cv;
 	T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
 	const int lane = threadIdx.x & (WAVE - 1);
-	int p = threadIdx.x / WAVE, e = lane;
-	while (p < np)
+	for (int p = threadIdx.x / WAVE; p < np; p += NW)
 	{
-		T v[U];
-		const int p0 = p, e0 = e;
-#pragma unroll
-		for (int u = 0; u < U; u++)
+		T *dst = base + (size_t)p * PS * g.cn;
+		int cend = run;
+		if (LOWER)
 		{
-			const int c = e / PS;
-			const int i = p * PS + (e & (PS - 1)) - rr;
-			T x = T(0);
-			if (p < np && e < run && i >= 0 && i < Rv && (!LOWER || i - c >= diag))
-				x = A(i, c);
-			v[u] = x;
-			e += WAVE;
-			if (e >= run)
-			{
-				e = lane;
-				p += NW;
-			}
+			const int last = min(p * PS + PS - 1 - rr, Rv - 1);
+			cend = min(run, (last - diag + 1) * PS);
 		}
-		int pp = p0, ee = e0;
-#pragma unroll
-		for (int u = 0; u < U; u++)
+		for (int e = lane; e < cend; e += WAVE)
 		{
-			const int c = ee / PS;
-			const int i = pp * PS + (ee & (PS - 1)) - rr;
-			if (pp < np && ee < run && i >= 0 && i < Rv && (!LOWER || i - c >= diag))
-				base[(size_t)pp * PS * g.cn + ee] = v[u];
-			ee += WAVE;
-			if (ee >= run)
-			{
-				ee = lane;
-				pp += NW;
-			}
+			const int c = (unsigned)e / PS;
+			const int q = e & (PS - 1);
+			const int i = p * PS + q - rr;
+			if (i < 0 || i >= Rv)
+				continue;
+			if (LOWER && i - c < diag)
+				continue;
+			dst[e] = A(i, c);
 		}
 	}
 }

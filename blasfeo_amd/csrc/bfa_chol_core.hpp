@@ -201,12 +201,31 @@ __device__ void chol_acc(const ACC M, int m, int n, T *invd)
 				x0[j] = r0ok ? M(c0 + lane, c0 + j) : T(0);
 				x1[j] = r1ok ? M(c0 + 64 + lane, c0 + j) : T(0);
 			}
-			if (cw == 16)
+			if (cw == 16 && !(NMAX > 64 && nr > 64))
 			{
-				// full panel, without per-column or per-lane guards: the rows
-				// above the diagonal (lanes < j of the diagonal tile) are scaled
-				// and updated too, but nothing below the diagonal reads them
-				// and they are never stored to D
+				// full panel of at most 64 rows, without per-column or per-lane
+				// guards: the rows above the diagonal (lanes < j of the diagonal
+				// tile) are scaled and updated too, but nothing below the
+				// diagonal reads them and they are never stored to D
+#pragma unroll
+				for (int j = 0; j < 16; j++)
+				{
+					const T c = readlane(x0[j], j);
+					const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
+					x0[j] = (lane == j) ? c * inv : x0[j] * inv;
+					if (lane == 0)
+						invd[c0 + j] = inv;
+#pragma unroll
+					for (int jj = j + 1; jj < 16; jj++)
+					{
+						const T l = readlane(x0[j], jj);
+						x0[jj] -= x0[j] * l;
+					}
+				}
+			}
+			else if (cw == 16)
+			{
+				// the same with a second row per lane (rows 64..127)
 #pragma unroll
 				for (int j = 0; j < 16; j++)
 				{

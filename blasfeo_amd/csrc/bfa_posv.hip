@@ -122,106 +122,95 @@ __device__ __forceinline__ void block_sum(double *v, double *red)
 	}
 }
 
-// Y = A V on the FP64 matrix cores, A symmetric from its stored lower
+// Y = A V in FP64 on the vector units, A symmetric from its stored lower
 // triangle in global memory (16x16 tiles), V = V[j*4 + c] in LDS, Y[i*4 + c]
 // and (ABS) the row sums of |A| in rowabs[i], both in LDS.  Wave w owns tile
 // rows and columns {w, 15 - w} (17 tiles of the direct products A_IJ V_J, 15
-// of the transposed products V_I^T A_IJ for nb = 16), so the two partial sums
-// of a row meet in one wave without atomics and the result does not depend
-// on timing.  The direct products read a tile as 16-row by 4-column slabs,
-// the transposed ones as 4-row by 16-column slabs, which are contiguous runs
-// of the panel-major storage.  Ends with a barrier.
+// of the transposed products V_K^T A_KI for nb = 16), so the two partial sums
+// of a row meet in one wave without atomics and the result does not depend on
+// timing.  Lane (i, q) = (l % 16, l / 16) takes row i and columns 4u + q of
+// every tile: the direct products read a tile as 16-row x 4-column slabs and
+// the transposed ones as 4-row x 16-column slabs (a contiguous 512-byte panel
+// run at ps = 4); partial sums stay in registers across the whole tile row
+// and are reduced over q once per row block.  With four right-hand sides the
+// vector units do no padded work, where 16x16x4 matrix-core products would
+// waste three quarters.  Ends with a barrier.
 template <int PS, bool ABS>
-__device__ void posv_matvec_mfma(const PM<double, PS> gA, int ai, int aj, int n, int nb,
+__device__ void posv_matvec(const PM<double, PS> gA, int ai, int aj, int n, int nb,
 	const double *V, double *Y, double *rowabs)
 {
 	const int lane = lane_id(), w = wave_id();
-	const int c16 = lane & 15, q = lane >> 4;
+	const int i = lane & 15, q = lane >> 4;
 	for (int sel = 0; sel < 2; sel++)
 	{
 		const int I = sel == 0 ? w : 2 * PV_NW - 1 - w;
 		if (I >= nb)
 			continue;
-		d4_t acc_d = d4_t{0, 0, 0, 0}, abs_d = d4_t{0, 0, 0, 0};
-		d4_t acc_t = d4_t{0, 0, 0, 0}, abs_t = d4_t{0, 0, 0, 0};
-		// direct: y_I += A_IJ v_J for J <= I (the diagonal tile completed
-		// symmetrically); four tiles per step, their 16 loads issued together
-		const int r = 16 * I + c16;
-		for (int J0 = 0; J0 <= I; J0 += 4)
+		double p[PV_R] = {0, 0, 0, 0};
+		double sa = 0.0;
+		const int r = 16 * I + i;
+		// direct: row r of A_IJ times V_J, J <= I (the diagonal tile completed symmetrically)
+		for (int J = 0; J <= I; J++)
 		{
-			double a[4][4];
+			double a[4];
 #pragma unroll
-			for (int t = 0; t < 4; t++)
+			for (int u = 0; u < 4; u++)
 			{
-				const int J = min(J0 + t, I);
-#pragma unroll
-				for (int u = 0; u < 4; u++)
-				{
-					const int cidx = 16 * J + 4 * u + q;
-					const bool up = cidx > r;
-					const int rr = min(up ? cidx : r, n - 1), cc = min(up ? r : cidx, n - 1);
-					const double e = gA.at(ai + rr, aj + cc);
-					a[t][u] = (J0 + t <= I && r < n && cidx < n) ? e : 0.0;
-				}
+				const int cidx = 16 * J + 4 * u + q;
+				const bool up = cidx > r;
+				const int rr = min(up ? cidx : r, n - 1), cc = min(up ? r : cidx, n - 1);
+				const double e = gA.at(ai + rr, aj + cc);
+				a[u] = (r < n && cidx < n) ? e : 0.0;
 			}
 #pragma unroll
-			for (int t = 0; t < 4; t++)
+			for (int u = 0; u < 4; u++)
 			{
-				const int J = min(J0 + t, I);
+				const double *v = V + (16 * J + 4 * u + q) * PV_R;
 #pragma unroll
-				for (int u = 0; u < 4; u++)
-				{
-					const double b = c16 < PV_R ? V[(16 * J + 4 * u + q) * PV_R + c16] : 0.0;
-					acc_d = mfma16(a[t][u], b, acc_d);
-					if (ABS)
-						abs_d = mfma16(fabs(a[t][u]), c16 == 0 ? 1.0 : 0.0, abs_d);
-				}
+				for (int c = 0; c < PV_R; c++)
+					p[c] += a[u] * v[c];
+				if (ABS)
+					sa += fabs(a[u]);
 			}
 		}
-		// transposed: y_I^T += v_K^T A_KI for K > I, four tiles per step
-		for (int K0 = I + 1; K0 < nb; K0 += 4)
+		// transposed: column r of A_KI (= row r of A_IK) times V_K, K > I
+		for (int K = I + 1; K < nb; K++)
 		{
-			double bt[4][4];
+			double bt[4];
 #pragma unroll
-			for (int t = 0; t < 4; t++)
+			for (int u = 0; u < 4; u++)
 			{
-				const int K = min(K0 + t, nb - 1);
-#pragma unroll
-				for (int u = 0; u < 4; u++)
-				{
-					const int rk = 16 * K + 4 * u + q, ci = 16 * I + c16;
-					const double e = gA.at(ai + min(rk, n - 1), aj + min(ci, n - 1));
-					bt[t][u] = (K0 + t < nb && rk < n && ci < n) ? e : 0.0;
-				}
+				const int rk = 16 * K + 4 * u + q;
+				const double e = gA.at(ai + min(rk, n - 1), aj + min(r, n - 1));
+				bt[u] = (rk < n && r < n) ? e : 0.0;
 			}
 #pragma unroll
-			for (int t = 0; t < 4; t++)
+			for (int u = 0; u < 4; u++)
 			{
-				const int K = min(K0 + t, nb - 1);
+				const double *v = V + (16 * K + 4 * u + q) * PV_R;
 #pragma unroll
-				for (int u = 0; u < 4; u++)
-				{
-					const double av = c16 < PV_R ? V[(16 * K + 4 * u + q) * PV_R + c16] : 0.0;
-					acc_t = mfma16(av, bt[t][u], acc_t);
-					if (ABS)
-						abs_t = mfma16(c16 == 0 ? 1.0 : 0.0, fabs(bt[t][u]), abs_t);
-				}
+				for (int c = 0; c < PV_R; c++)
+					p[c] += bt[u] * v[c];
+				if (ABS)
+					sa += fabs(bt[u]);
 			}
 		}
-		// direct: lane (c16 < 4) holds y_I[q + 4r][c16]; transposed: register 0 holds y_I[c16][q]
-		if (c16 < PV_R)
+		// reduce over q (lanes i, i + 16, i + 32, i + 48), fixed order
 #pragma unroll
-			for (int rg = 0; rg < 4; rg++)
-				Y[(16 * I + q + 4 * rg) * PV_R + c16] = acc_d[rg];
-		if (ABS && c16 == 0)
-#pragma unroll
-			for (int rg = 0; rg < 4; rg++)
-				rowabs[16 * I + q + 4 * rg] = abs_d[rg];
-		wave_sync();
-		Y[(16 * I + c16) * PV_R + q] += acc_t[0];
+		for (int c = 0; c < PV_R; c++)
+		{
+			p[c] += __shfl_xor(p[c], 16, WAVE);
+			p[c] += __shfl_xor(p[c], 32, WAVE);
+		}
+		if (ABS)
+		{
+			sa += __shfl_xor(sa, 16, WAVE);
+			sa += __shfl_xor(sa, 32, WAVE);
+		}
+		const double mine = q == 0 ? p[0] : (q == 1 ? p[1] : (q == 2 ? p[2] : p[3]));
+		Y[r * PV_R + q] = mine;
 		if (ABS && q == 0)
-			rowabs[16 * I + c16] += abs_t[0];
-		wave_sync();
+			rowabs[r] = sa;
 	}
 	__syncthreads();
 }
@@ -617,13 +606,13 @@ __global__ __launch_bounds__(PV_NT) void k_posv_mixed(const blasfeo_amd_dposv_ar
 			double qv[PV_R] = {0, 0, 0, 0};
 			if (anorm < 0.0)
 			{
-				posv_matvec_mfma<PS, true>(gA, ai, aj, n, nb, L.X, L.S, L.S + n16 * PV_R);
+				posv_matvec<PS, true>(gA, ai, aj, n, nb, L.X, L.S, L.S + n16 * PV_R);
 				double an = owner ? L.S[n16 * PV_R + i] : 0.0;
 				block_max<1>(&an, L.red);
 				anorm = an;
 			}
 			else
-				posv_matvec_mfma<PS, false>(gA, ai, aj, n, nb, L.X, L.S, nullptr);
+				posv_matvec<PS, false>(gA, ai, aj, n, nb, L.X, L.S, nullptr);
 			if (owner)
 #pragma unroll
 				for (int c = 0; c < PV_R; c++)
@@ -670,7 +659,7 @@ __global__ __launch_bounds__(PV_NT) void k_posv_mixed(const blasfeo_amd_dposv_ar
 					L.X[i * PV_R + c] = owner ? xr[c] : 0.0;
 			__syncthreads();
 			double ax[PV_R] = {0, 0, 0, 0};
-			posv_matvec_mfma<PS, false>(gA, ai, aj, n, nb, L.X, L.S, nullptr);
+			posv_matvec<PS, false>(gA, ai, aj, n, nb, L.X, L.S, nullptr);
 			if (owner)
 #pragma unroll
 				for (int c = 0; c < PV_R; c++)

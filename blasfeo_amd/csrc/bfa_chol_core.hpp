@@ -168,7 +168,8 @@ namespace bfa {
 // 16x16 tiles; m <= NMAX <= 128), in place: M <- L (potrf_l_mn semantics:
 // L11 = chol(C11), L21 = C21 L11^{-T}),
 // invd[j] = 1/sqrt(pivot) with the zero-pivot rule above (L_jj = c*invd[j]).
-// Rows m .. ceil16(m)-1 and columns n .. ceil16(n)-1 must hold zeros on entry.
+// Rows m .. ceil16(m)-1 and columns n .. ceil16(n)-1 must hold zeros on entry
+// (with PADCOLS the padding columns hold unspecified values on exit).
 // Per 16-column block: wave 0 factors the whole panel (diagonal block and the
 // rows below, two rows per lane, one when NMAX <= 64) in registers, broadcasting L(jj,j) with
 // v_readlane, so the panel needs no workgroup barrier; then the waves apply
@@ -176,8 +177,8 @@ namespace bfa {
 // block column of tiles at a time with independent accumulators.
 // Called by all NT threads of the workgroup; starts and ends with a barrier.
 // ---------------------------------------------------------------------------
-template <typename T, int NT, int NMAX, typename ACC>
-__device__ void chol_acc(const ACC M, int m, int n, T *invd)
+template <typename T, int NT, int NMAX, typename ACC, bool PADCOLS = true>
+__device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
 {
 	typedef typename Vec4<T>::type V4;
 	constexpr int NW = NT / WAVE;
@@ -201,73 +202,84 @@ __device__ void chol_acc(const ACC M, int m, int n, T *invd)
 				x0[j] = r0ok ? M(c0 + lane, c0 + j) : T(0);
 				x1[j] = r1ok ? M(c0 + 64 + lane, c0 + j) : T(0);
 			}
-			if (cw == 16 && !(NMAX > 64 && nr > 64))
+			// PADCOLS: the padding columns of a partial panel are factored too
+			// (unguarded); otherwise a partial panel takes the guarded form.
+			if (PADCOLS || cw == 16)
 			{
-				// full panel of at most 64 rows, without per-column or per-lane
-				// guards: the rows above the diagonal (lanes < j of the diagonal
-				// tile) are scaled and updated too, but nothing below the
-				// diagonal reads them and they are never stored to D
-#pragma unroll
-				for (int j = 0; j < 16; j++)
+				// No guards: the rows above the diagonal (lanes < j of the diagonal
+				// tile) are scaled and updated too, but nothing below the diagonal
+				// reads them and they are never stored to D.  Columns cw..15 of a
+				// partial panel are factored as well; they are padding columns,
+				// which no real column reads (column jj is updated only from
+				// columns j < jj) and no store returns, so only invd is guarded.
+				if (!(NMAX > 64 && nr > 64))
 				{
-					const T c = readlane(x0[j], j);
-					const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
-					x0[j] = (lane == j) ? c * inv : x0[j] * inv;
-					if (lane == 0)
-						invd[c0 + j] = inv;
 #pragma unroll
-					for (int jj = j + 1; jj < 16; jj++)
+					for (int j = 0; j < 16; j++)
 					{
-						const T l = readlane(x0[j], jj);
-						x0[jj] -= x0[j] * l;
+						const T c = readlane(x0[j], j);
+						const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
+						x0[j] = (lane == j) ? c * inv : x0[j] * inv;
+						if (lane == 0 && j < cw)
+							invd[c0 + j] = inv;
+#pragma unroll
+						for (int jj = j + 1; jj < 16; jj++)
+						{
+							const T l = readlane(x0[j], jj);
+							x0[jj] -= x0[j] * l;
+						}
 					}
 				}
-			}
-			else if (cw == 16)
-			{
-				// the same with a second row per lane (rows 64..127)
-#pragma unroll
-				for (int j = 0; j < 16; j++)
+				else
 				{
-					const T c = readlane(x0[j], j);
-					const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
-					x0[j] = (lane == j) ? c * inv : x0[j] * inv;
-					x1[j] *= inv;
-					if (lane == 0)
-						invd[c0 + j] = inv;
+					// the same with a second row per lane (rows 64..127)
 #pragma unroll
-					for (int jj = j + 1; jj < 16; jj++)
+					for (int j = 0; j < 16; j++)
 					{
-						const T l = readlane(x0[j], jj);
-						x0[jj] -= x0[j] * l;
-						x1[jj] -= x1[j] * l;
+						const T c = readlane(x0[j], j);
+						const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
+						x0[j] = (lane == j) ? c * inv : x0[j] * inv;
+						x1[j] *= inv;
+						if (lane == 0 && j < cw)
+							invd[c0 + j] = inv;
+#pragma unroll
+						for (int jj = j + 1; jj < 16; jj++)
+						{
+							const T l = readlane(x0[j], jj);
+							x0[jj] -= x0[j] * l;
+							x1[jj] -= x1[j] * l;
+						}
 					}
 				}
 			}
 			else
-#pragma unroll
-			for (int j = 0; j < 16; j++)
 			{
-				if (j < cw)
-				{
-					const T c = readlane(x0[j], j);
-					const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
-					if (lane > j)
-						x0[j] *= inv;
-					else if (lane == j)
-						x0[j] = c * inv;
-					x1[j] *= inv;
-					if (lane == 0)
-						invd[c0 + j] = inv;
+				// partial panel, per-column and per-lane guards (fewer
+				// registers than the unguarded form)
 #pragma unroll
-					for (int jj = j + 1; jj < 16; jj++)
+				for (int j = 0; j < 16; j++)
+				{
+					if (j < cw)
 					{
-						if (jj < cw)
+						const T c = readlane(x0[j], j);
+						const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
+						if (lane > j)
+							x0[j] *= inv;
+						else if (lane == j)
+							x0[j] = c * inv;
+						x1[j] *= inv;
+						if (lane == 0)
+							invd[c0 + j] = inv;
+#pragma unroll
+						for (int jj = j + 1; jj < 16; jj++)
 						{
-							const T l = readlane(x0[j], jj);
-							if (lane >= jj)
-								x0[jj] -= x0[j] * l;
-							x1[jj] -= x1[j] * l;
+							if (jj < cw)
+							{
+								const T l = readlane(x0[j], jj);
+								if (lane >= jj)
+									x0[jj] -= x0[j] * l;
+								x1[jj] -= x1[j] * l;
+							}
 						}
 					}
 				}

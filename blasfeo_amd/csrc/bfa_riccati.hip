@@ -179,6 +179,25 @@ __device__ __attribute__((noinline)) void ric_syrk_m(const TileLower M, const Ti
 	}
 }
 
+// The one-wave kernels call the in-LDS Cholesky out of line: inlined it
+// would raise them above the 168 registers that three waves per SIMD allow.
+// An LDS address rebuilt from its 32-bit offset behind an empty asm: the
+// callee then keeps the base in a register instead of resolving the kernel's
+// dynamic-LDS variable through the offset table at every access.
+__device__ __forceinline__ double *lds_opaque(double *p)
+{
+	unsigned off = (unsigned)(size_t)(__attribute__((address_space(3))) double *)p;
+	asm volatile("" : "+v"(off));
+	off = __builtin_amdgcn_readfirstlane(off);
+	return (double *)(__attribute__((address_space(3))) double *)(size_t)off;
+}
+
+template <int NT, int NMAXC>
+__device__ __attribute__((noinline)) void ric_chol_call(const TileLower M, int nk, double *invd)
+{
+	chol_acc<double, NT, NMAXC, TileLower, false>(TileLower{lds_opaque(M.p)}, nk, nk, lds_opaque(invd));
+}
+
 template <int PS, int NT>
 __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 {
@@ -202,44 +221,48 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 	const TileRect W{smem + (size_t)nlow * RT_SZ, nxt_max};
 	double *invd = smem + (size_t)(nlow + nkt_max * nxt_max) * RT_SZ;
 
-	// last stage: L[N] = chol(Q_N)
+	// s = -1 is the last stage, L[N] = chol(Q_N) (potrf_l); s >= 0 is stage
+	// k = N-1-s.  One call site of the in-LDS Cholesky.  Inlined, it addresses
+	// the kernel's dynamic LDS directly; out of line, every LDS access of the
+	// callee first loads the kernel's LDS offset from a table and waits for it.
+	for (int s = -1; s < N; s++)
 	{
-		const int m = nx[N];
-		const int mt = (m + 15) / 16;
-		g2s_acc<double, PS, NT, true>(view<double, PS>(&RSQ[N]), 0, 0, mt * 16, mt * 16, m, m, M);
-		chol_acc<double, NT, NMAXC>(M, m, m, invd);
-		s2g_acc<double, PS, NT, true>(view<double, PS>(&L[N]), 0, 0, m, m, M, 0);
-		for (int j = threadIdx.x; j < m; j += NT)
-			L[N].dA[j] = invd[j];
-		if (threadIdx.x == 0)
-			L[N].use_dA = m;
-	}
-	for (int s = 0; s < N; s++)
-	{
-		const int k = N - 1 - s;
-		const int nk = nu[k] + nx[k];
-		const int n1 = nx[k + 1];
-		const int u1 = nu[k + 1];
+		const bool last = s < 0;
+		const int k = last ? N : N - 1 - s;
+		const int nk = last ? nx[N] : nu[k] + nx[k];
 		const int nkt = (nk + 15) / 16;
-		const int n1t = (n1 + 15) / 16;
-		// stage BAbt[k] (nk x n1) into W; M still holds L[k+1]
-		g2s_acc<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W);
-		__syncthreads();
-		ric_trmm_w<NT>(M, W, nkt, n1, n1t, u1);
-		__syncthreads();
+		if (!last)
+		{
+			const int n1 = nx[k + 1];
+			const int u1 = nu[k + 1];
+			const int n1t = (n1 + 15) / 16;
+			// stage BAbt[k] (nk x n1) into W; M still holds L[k+1]
+			g2s_acc<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W);
+			__syncthreads();
+			ric_trmm_w<NT>(M, W, nkt, n1, n1t, u1);
+			__syncthreads();
+		}
 		// M <- lower(RSQ[k]) (L[k+1] is no longer needed)
 		g2s_acc<double, PS, NT, true>(view<double, PS>(&RSQ[k]), 0, 0, nkt * 16, nkt * 16, nk, nk, M);
-		__syncthreads();
-		ric_syrk_m<NT>(M, W, nkt, n1);
+		if (!last)
+		{
+			__syncthreads();
+			ric_syrk_m<NT>(M, W, nkt, nx[k + 1]);
+		}
 		// L[k] = chol(M)
-		chol_acc<double, NT, NMAXC>(M, nk, nk, invd);
+		if constexpr (NT == WAVE)
+			ric_chol_call<NT, NMAXC>(M, nk, invd);
+		else
+			chol_acc<double, NT, NMAXC>(M, nk, nk, invd);
 		s2g_acc<double, PS, NT, true>(view<double, PS>(&L[k]), 0, 0, nk, nk, M, 0);
 		for (int j = threadIdx.x; j < nk; j += NT)
 			L[k].dA[j] = invd[j];
 		if (threadIdx.x == 0)
 		{
-			L[k].use_dA = 1;
-			a.work->use_dA = 0;
+			// potrf_l marks the whole diagonal cached, syrk_dpotrf_ln one entry
+			L[k].use_dA = last ? nk : 1;
+			if (!last)
+				a.work->use_dA = 0;
 		}
 	}
 	__syncthreads();

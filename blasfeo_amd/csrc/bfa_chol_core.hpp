@@ -198,10 +198,19 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
 			const bool r0ok = lane < nr, r1ok = NMAX > 64 && lane + 64 < nr;
 #pragma unroll
 			for (int j = 0; j < 16; j++)
-			{
-				x0[j] = r0ok ? M(c0 + lane, c0 + j) : T(0);
-				x1[j] = r1ok ? M(c0 + 64 + lane, c0 + j) : T(0);
-			}
+				x0[j] = x1[j] = T(0);
+			// one guarded block per row set (per-element guards become one
+			// exec-mask branch per load)
+			if (r0ok)
+#pragma unroll
+				for (int j = 0; j < 16; j++)
+					x0[j] = M(c0 + lane, c0 + j);
+			if (r1ok)
+#pragma unroll
+				for (int j = 0; j < 16; j++)
+					x1[j] = M(c0 + 64 + lane, c0 + j);
+			// lane j keeps 1/sqrt(pivot j), stored once after the panel
+			T myinv = T(0);
 			// PADCOLS: the padding columns of a partial panel are factored too
 			// (unguarded); otherwise a partial panel takes the guarded form.
 			if (PADCOLS || cw == 16)
@@ -211,7 +220,8 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
 				// reads them and they are never stored to D.  Columns cw..15 of a
 				// partial panel are factored as well; they are padding columns,
 				// which no real column reads (column jj is updated only from
-				// columns j < jj) and no store returns, so only invd is guarded.
+				// columns j < jj) and no store returns; invd is stored for
+				// columns < cw only.
 				if (!(NMAX > 64 && nr > 64))
 				{
 #pragma unroll
@@ -220,8 +230,7 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
 						const T c = readlane(x0[j], j);
 						const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
 						x0[j] = (lane == j) ? c * inv : x0[j] * inv;
-						if (lane == 0 && j < cw)
-							invd[c0 + j] = inv;
+						myinv = (lane == j) ? inv : myinv;
 #pragma unroll
 						for (int jj = j + 1; jj < 16; jj++)
 						{
@@ -240,8 +249,7 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
 						const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
 						x0[j] = (lane == j) ? c * inv : x0[j] * inv;
 						x1[j] *= inv;
-						if (lane == 0 && j < cw)
-							invd[c0 + j] = inv;
+						myinv = (lane == j) ? inv : myinv;
 #pragma unroll
 						for (int jj = j + 1; jj < 16; jj++)
 						{
@@ -268,8 +276,7 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
 						else if (lane == j)
 							x0[j] = c * inv;
 						x1[j] *= inv;
-						if (lane == 0)
-							invd[c0 + j] = inv;
+						myinv = (lane == j) ? inv : myinv;
 #pragma unroll
 						for (int jj = j + 1; jj < 16; jj++)
 						{
@@ -284,6 +291,8 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
 					}
 				}
 			}
+			if (lane < cw)
+				invd[c0 + lane] = myinv;
 #pragma unroll
 			for (int j = 0; j < 16; j++)
 			{

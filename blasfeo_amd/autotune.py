@@ -26,10 +26,11 @@ _RT_SZ = 16 * 17
 
 
 def ric_footprint(nkmax, nxmax):
-    """LDS bytes of one instance in the resident Riccati sweep (the kernel's rule)."""
+    """LDS bytes of one instance in the resident Riccati sweep (the kernel's
+    rule: tiles, reciprocal pivots and the 16-byte work-queue slot)."""
     nkt = (nkmax + 15) // 16
     nxt = (nxmax + 15) // 16
-    return ((nkt * (nkt + 1) // 2 + nkt * nxt) * _RT_SZ + nkt * 16) * 8
+    return ((nkt * (nkt + 1) // 2 + nkt * nxt) * _RT_SZ + nkt * 16) * 8 + 16
 
 
 def ric_class(nkmax, nxmax):

@@ -38,6 +38,43 @@ int check_launch(const char *what)
 	return BLASFEO_AMD_OK;
 }
 
+// Whether a launch of kernel fn with dyn bytes of dynamic LDS fits the
+// device: its static LDS plus dyn at most the per-workgroup limit.  A launch
+// beyond it aborts the queue at dispatch instead of failing the launch call.
+int lds_check(const void *fn, size_t dyn, const char *what)
+{
+	static std::mutex mu;
+	static std::map<std::pair<int, const void *>, size_t> cache;  // static LDS per (device, kernel)
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess)
+		return set_error(BLASFEO_AMD_ELAUNCH, "%s: no device", what);
+	size_t stat = 0, limit = 0;
+	{
+		std::lock_guard<std::mutex> lock(mu);
+		auto key = std::make_pair(dev, fn);
+		auto it = cache.find(key);
+		if (it == cache.end())
+		{
+			hipFuncAttributes at;
+			if (hipFuncGetAttributes(&at, fn) != hipSuccess)
+				return check_launch(what);
+			it = cache.emplace(key, at.sharedSizeBytes).first;
+		}
+		stat = it->second;
+	}
+	// the per-block attribute can report less than a CU's LDS, which one
+	// workgroup may use whole on gfx950; take the larger of the two
+	int lb = 0, lm = 0;
+	if (hipDeviceGetAttribute(&lb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+		hipDeviceGetAttribute(&lm, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
+		return check_launch(what);
+	limit = (size_t)(lb > lm ? lb : lm);
+	if (stat + dyn > limit)
+		return set_error(BLASFEO_AMD_ELAUNCH, "%s: %zu bytes of LDS (%zu static + %zu dynamic) exceed the %zu-byte limit",
+			what, stat + dyn, stat, dyn, limit);
+	return BLASFEO_AMD_OK;
+}
+
 int num_cus()
 {
 	int dev = 0, n = 0;

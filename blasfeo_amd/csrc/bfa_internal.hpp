@@ -14,6 +14,7 @@ int check_launch(const char *what);
 
 // Number of compute units of the current device (cached per device).
 int num_cus();
+int lds_check(const void *fn, size_t dyn, const char *what);
 
 // Device scratch owned by the library for one (device, stream) pair, at least
 // `bytes` long.  Allocated on first use and grown when a larger size is asked

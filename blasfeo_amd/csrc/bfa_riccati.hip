@@ -29,6 +29,8 @@ namespace bfa {
 constexpr int RIC_LDS_MAX = 163840;   // LDS bytes a workgroup may use on gfx950
 constexpr int RIC_NCLASS = 8;         // LDS-resident footprint classes
 constexpr int RIC_GENERAL = RIC_NCLASS;
+constexpr int RIC_HEAD = 32;          // offset of the work-queue heads in the plan workspace
+static_assert(RIC_HEAD + RIC_NCLASS <= 64, "plan workspace header holds 64 ints");
 // class capacities in bytes: multiples of 1 KiB that divide the CU's LDS
 // into 20, 10, 6, 5, 4, 3, 2 and 1 resident instances
 __constant__ int c_ric_cap[RIC_NCLASS] = {8192, 16384, 24576, 32768, 40960, 54272, 81920, 163840};
@@ -64,13 +66,18 @@ struct TileRect
 	}
 };
 
+// The first RIC_QSLOT bytes of a class kernel's dynamic LDS hold the work-queue
+// slot; the class kernels declare no static LDS, so a launch asks for exactly
+// its class cap (at most the 160 KB of a CU).
+constexpr int RIC_QSLOT = 16;
+
 // LDS bytes of the resident form for an instance whose largest stage is
-// nkmax = max(nu+nx) and whose largest state is nxmax.
+// nkmax = max(nu+nx) and whose largest state is nxmax (queue slot included).
 __host__ __device__ __forceinline__ int ric_footprint(int nkmax, int nxmax)
 {
 	const int nkt = (nkmax + 15) / 16;
 	const int nxt = (nxmax + 15) / 16;
-	return ((nkt * (nkt + 1) / 2 + nkt * nxt) * RT_SZ + nkt * 16) * (int)sizeof(double);
+	return ((nkt * (nkt + 1) / 2 + nkt * nxt) * RT_SZ + nkt * 16) * (int)sizeof(double) + RIC_QSLOT;
 }
 
 // ---------------------------------------------------------------------------
@@ -323,9 +330,11 @@ __device__ void ric_sweep_general(const blasfeo_amd_riccati_arg &a, double *P, i
 // ---------------------------------------------------------------------------
 // plan and launches
 // ---------------------------------------------------------------------------
+// counts[0..RIC_NCLASS] (class sizes) and counts[RIC_HEAD + c] (the work-queue
+// head of class c)
 __global__ void k_ric_zero(int *counts)
 {
-	if (threadIdx.x < RIC_NCLASS + 1)
+	if (threadIdx.x < 64)
 		counts[threadIdx.x] = 0;
 }
 
@@ -369,14 +378,41 @@ __global__ void k_ric_plan(const blasfeo_amd_riccati_arg *__restrict__ args, int
 	lists[(size_t)cls * batch + slot] = i;
 }
 
+// Next instance of the class for this workgroup from the class's work-queue
+// head (one atomic per instance), so that a workgroup that drew short
+// instances takes more of them.  Every workgroup leaves once the head passes
+// the class size.  Several waves share the value through the LDS slot, which
+// is rewritten only after the sweep's last barrier, which every thread passes
+// after reading it.
+template <int NT>
+__device__ __forceinline__ int ric_next(int *head, int *slot)
+{
+	if constexpr (NT == WAVE)
+	{
+		int t = 0;
+		if (threadIdx.x == 0)
+			t = atomicAdd(head, 1);
+		return __shfl(t, 0, WAVE);
+	}
+	else
+	{
+		if (threadIdx.x == 0)
+			*slot = atomicAdd(head, 1);
+		__syncthreads();
+		return *slot;
+	}
+}
+
 template <int PS, int NT>
 __global__ __launch_bounds__(NT) void k_ric_class(const blasfeo_amd_riccati_arg *__restrict__ args,
-	const int *__restrict__ count, const int *__restrict__ list)
+	const int *__restrict__ count, const int *__restrict__ list, int *head)
 {
 	extern __shared__ double smem_d[];
+	int *slot = reinterpret_cast<int *>(smem_d);
+	double *work = smem_d + RIC_QSLOT / sizeof(double);
 	const int cnt = *count;
-	for (int t = blockIdx.x; t < cnt; t += gridDim.x)
-		ric_sweep_lds<PS, NT>(args[list[t]], smem_d);
+	for (int t = ric_next<NT>(head, slot); t < cnt; t = ric_next<NT>(head, slot))
+		ric_sweep_lds<PS, NT>(args[list[t]], work);
 }
 
 // one wave per instance: registers capped at 168 so that three instances
@@ -384,12 +420,14 @@ __global__ __launch_bounds__(NT) void k_ric_class(const blasfeo_amd_riccati_arg 
 // nu+nx <= 64 (the plan's rule), so the Cholesky keeps one panel row per lane.
 template <int PS>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3))) void k_ric_class1(
-	const blasfeo_amd_riccati_arg *__restrict__ args, const int *__restrict__ count, const int *__restrict__ list)
+	const blasfeo_amd_riccati_arg *__restrict__ args, const int *__restrict__ count, const int *__restrict__ list,
+	int *head)
 {
 	extern __shared__ double smem_d[];
+	double *work = smem_d + RIC_QSLOT / sizeof(double);
 	const int cnt = *count;
-	for (int t = blockIdx.x; t < cnt; t += gridDim.x)
-		ric_sweep_lds<PS, WAVE>(args[list[t]], smem_d);
+	for (int t = ric_next<WAVE>(head, nullptr); t < cnt; t = ric_next<WAVE>(head, nullptr))
+		ric_sweep_lds<PS, WAVE>(args[list[t]], work);
 }
 
 template <int PS>
@@ -409,15 +447,56 @@ __global__ __launch_bounds__(NT3) void k_ric_general(const blasfeo_amd_riccati_a
 }
 
 template <int PS, int NT>
-static void launch_class(int cls, int grid, const blasfeo_amd_riccati_arg *args, const int *counts, const int *lists,
-	int batch, hipStream_t st)
+static int launch_class(int cls, int grid, const blasfeo_amd_riccati_arg *args, int *counts, const int *lists,
+	int batch, hipStream_t st, const char *name)
 {
+	const void *fn = NT == WAVE ? reinterpret_cast<const void *>(&k_ric_class1<PS>)
+	                            : reinterpret_cast<const void *>(&k_ric_class<PS, NT>);
+	if (const int rc = lds_check(fn, h_ric_cap[cls], name))
+		return rc;
 	if (NT == WAVE)
 		hipLaunchKernelGGL((k_ric_class1<PS>), dim3(grid), dim3(WAVE), h_ric_cap[cls], st, args,
-			counts + cls, lists + (size_t)cls * batch);
+			counts + cls, lists + (size_t)cls * batch, counts + RIC_HEAD + cls);
 	else
 		hipLaunchKernelGGL((k_ric_class<PS, NT>), dim3(grid), dim3(NT), h_ric_cap[cls], st, args,
-			counts + cls, lists + (size_t)cls * batch);
+			counts + cls, lists + (size_t)cls * batch, counts + RIC_HEAD + cls);
+	return check_launch(name);
+}
+
+// Helper streams (default priority) on which the class launches run side by
+// side with the caller's stream: forked from and joined back to it with
+// events, so the call stays ordered on the caller's stream, and a capture of
+// that stream captures the helpers' launches as parallel branches.  Created
+// per device on the first call (outside any capture), kept for the process.
+constexpr int RIC_AUX = 3;
+struct RicFork
+{
+	hipStream_t aux[RIC_AUX];
+	hipEvent_t fork;
+	hipEvent_t join[RIC_AUX];
+	std::mutex mu;
+};
+
+static RicFork *ric_fork_state()
+{
+	static std::mutex gmu;
+	static std::map<int, RicFork *> per_dev;
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess)
+		return nullptr;
+	std::lock_guard<std::mutex> lock(gmu);
+	auto it = per_dev.find(dev);
+	if (it != per_dev.end())
+		return it->second;
+	RicFork *f = new RicFork;
+	bool ok = hipEventCreateWithFlags(&f->fork, hipEventDisableTiming) == hipSuccess;
+	for (int i = 0; i < RIC_AUX && ok; i++)
+		ok = hipStreamCreateWithFlags(&f->aux[i], hipStreamNonBlocking) == hipSuccess &&
+			hipEventCreateWithFlags(&f->join[i], hipEventDisableTiming) == hipSuccess;
+	if (!ok)
+		return nullptr;
+	per_dev[dev] = f;
+	return f;
 }
 
 template <int PS>
@@ -435,27 +514,53 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 	if (rc)
 		return rc;
 	const int cus = num_cus() > 0 ? num_cus() : 256;
-	// general class first (largest instances), then resident classes largest first
+	RicFork *fk = ric_fork_state();
+	if (!fk)
+		return set_error(BLASFEO_AMD_ELAUNCH, "%s: cannot create the helper streams", name);
+	std::lock_guard<std::mutex> lock(fk->mu);
+	hipStream_t ss[1 + RIC_AUX] = {st};
+	if (hipEventRecord(fk->fork, st) != hipSuccess)
+		return check_launch(name);
+	for (int i = 0; i < RIC_AUX; i++)
+	{
+		ss[1 + i] = fk->aux[i];
+		if (hipStreamWaitEvent(fk->aux[i], fk->fork, 0) != hipSuccess)
+			return check_launch(name);
+	}
+	// general class first (largest instances), then resident classes largest
+	// first, dealt round-robin over the caller's and the helper streams
+	int q = 0;
 	{
 		const size_t shm = sizeof(double) * PW * ((mmax + 15) / 16 * 16 + 1);
 		const int grid = min(batch, cus);
-		hipLaunchKernelGGL((k_ric_general<PS>), dim3(grid), dim3(NT3), shm, st, args,
-			counts + RIC_GENERAL, lists + (size_t)RIC_GENERAL * batch, mmax);
+		rc = lds_check(reinterpret_cast<const void *>(&k_ric_general<PS>), shm, name);
+		if (!rc)
+		{
+			hipLaunchKernelGGL((k_ric_general<PS>), dim3(grid), dim3(NT3), shm, ss[q++ % (1 + RIC_AUX)], args,
+				counts + RIC_GENERAL, lists + (size_t)RIC_GENERAL * batch, mmax);
+			rc = check_launch(name);
+		}
 	}
 	(void)maxfoot;
-	for (int cls = RIC_NCLASS - 1; cls >= 0; cls--)
+	for (int cls = RIC_NCLASS - 1; cls >= 0 && !rc; cls--)
 	{
 		// built-in launch shape per class (threads per instance, resident
 		// instances per CU), from autotuner runs on the 8192 x N=20 batch
-		// (nx ~ U{8..64}, nu ~ U{2..16}) on MI355X: one wave and up to 12 per CU
-		// (3 per SIMD, the register cap of k_ric_class1) for footprints up to
-		// 24 KB, two waves and 4 per CU up to 40 KB, four waves and 2 per CU
-		// above.  The tuning knobs override both.
+		// (nx ~ U{8..64}, nu ~ U{2..16}) on MI355X: two waves per instance up
+		// to 24 KB (4 per CU up to 8 KB, 8 above: fewer resident instances
+		// spread a small class over more CUs), two waves and 4 per CU up to
+		// 40 KB except four waves and 3 per CU at 32 KB, four waves and 2 per
+		// CU above.  The tuning knobs override both.
 		int per_cu, nt;
 		if (h_ric_cap[cls] <= 24576)
 		{
-			nt = 64;
-			per_cu = min(RIC_LDS_MAX / h_ric_cap[cls], 12);
+			nt = 128;
+			per_cu = h_ric_cap[cls] <= 8192 ? 4 : min(RIC_LDS_MAX / h_ric_cap[cls], 8);
+		}
+		else if (h_ric_cap[cls] == 32768)
+		{
+			nt = 256;
+			per_cu = 3;
 		}
 		else if (h_ric_cap[cls] <= 40960)
 		{
@@ -472,16 +577,23 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 		if (const int v = tune_value(TUNE_RIC_THREADS, cls))
 			nt = v;
 		const int grid = min(batch, cus * per_cu);
+		hipStream_t sc = ss[q++ % (1 + RIC_AUX)];
 		if (nt == 64)
-			launch_class<PS, 64>(cls, grid, args, counts, lists, batch, st);
+			rc = launch_class<PS, 64>(cls, grid, args, counts, lists, batch, sc, name);
 		else if (nt == 128)
-			launch_class<PS, 128>(cls, grid, args, counts, lists, batch, st);
+			rc = launch_class<PS, 128>(cls, grid, args, counts, lists, batch, sc, name);
 		else
-			launch_class<PS, 256>(cls, grid, args, counts, lists, batch, st);
-		if ((rc = check_launch(name)))
-			return rc;
+			rc = launch_class<PS, 256>(cls, grid, args, counts, lists, batch, sc, name);
+		if (rc)
+			break;
 	}
-	return check_launch(name);
+	// join back whatever was launched (also after a launch error, so that
+	// the caller's stream never runs ahead of the helpers)
+	for (int i = 0; i < RIC_AUX; i++)
+		if (hipEventRecord(fk->join[i], fk->aux[i]) != hipSuccess ||
+			hipStreamWaitEvent(st, fk->join[i], 0) != hipSuccess)
+			return check_launch(name);
+	return rc ? rc : check_launch(name);
 }
 
 } // namespace bfa

@@ -177,7 +177,7 @@ def test_tuning_knobs_without_gpu():
     assert at.get_knob("ric_per_cu", 3) == 0 and at.get_knob("ric_threads", 2) == 0
     # footprints of the resident form: largest bench stage (nu+nx = 80, nx = 64)
     # fits two per CU, nu+nx = 64, nx = 48 three per CU
-    assert at.ric_footprint(80, 64) == 76800 and at.ric_class(80, 64) == 6
+    assert at.ric_footprint(80, 64) == 76816 and at.ric_class(80, 64) == 6
     assert at.ric_class(64, 48) == 5
     assert at.ric_class(65, 8) >= 5          # nu+nx > 64 never takes a one-wave class
     assert at.ric_class(60, 70) == 8 and at.ric_class(130, 10) == 8

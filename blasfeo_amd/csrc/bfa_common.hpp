@@ -83,41 +83,69 @@ __device__ __forceinline__ PM<T, PS> view(const typename MatOf<T>::type *s)
 	return v;
 }
 
-// Cooperative copy of the R x C block at absolute (r0, c0) of a panel-major
-// matrix into LDS, element (i,c) stored at lds[c*ld + i] (TRANS=false) or
-// lds[i*ld + c] (TRANS=true).  Elements with i >= Rv or c >= Cv (and, with
-// LOWER, those above the diagonal, i < c) are written as zero and never read
-// from memory.  The waves take the covering panels in turn and the lanes of a
-// wave walk one panel's contiguous run of C*PS elements, so a wave reads
-// contiguous memory and no index needs a runtime division.
-template <typename T, int PS, int NT, bool TRANS, bool LOWER = false>
-__device__ __forceinline__ void g2s(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, T *lds, int ld)
+// Walks the elements of the R x C block at absolute (r0, c0) of a panel-major
+// matrix that this thread copies: the waves take the covering panels in turn
+// and the lanes of a wave walk one panel's contiguous run of C*PS elements, so
+// a wave reads contiguous memory and no index needs a runtime division.  The
+// elements go eight at a time: a group's global loads are all issued before
+// any of its stores, so a thread keeps up to eight loads in flight instead of
+// waiting for each.  Elements with i >= Rv or c >= Cv (and, with LOWER, those
+// above the diagonal, i < c) are passed on as zero and never read; with
+// SKIPTILES, elements whose 16x16 tile lies above the diagonal are skipped.
+// put(i, c, v) stores element (i, c) of the block.
+template <typename T, int PS, int NT, bool LOWER, bool SKIPTILES, typename PUT>
+__device__ __forceinline__ void g2s_walk(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, PUT put)
 {
 	constexpr int NW = NT / WAVE;
+	constexpr int B = 8;
 	const int rr = r0 & (PS - 1);
 	const int np = (rr + R + PS - 1) / PS;
 	const int run = PS * C;
 	const T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
 	const int lane = threadIdx.x & (WAVE - 1);
-	for (int p = threadIdx.x / WAVE; p < np; p += NW)
+	int p = threadIdx.x / WAVE, e = lane;
+	while (p < np)
 	{
-		const T *src = base + (size_t)p * PS * g.cn;
-		for (int e = lane; e < run; e += WAVE)
+		T v[B];
+		int ii[B], cc[B];
+		bool ok[B];
+#pragma unroll
+		for (int b = 0; b < B; b++)
 		{
 			const int c = (unsigned)e / PS;
-			const int q = e & (PS - 1);
-			const int i = p * PS + q - rr;
-			if (i < 0 || i >= R)
-				continue;
-			T v = T(0);
-			if (i < Rv && c < Cv && (!LOWER || i >= c))
-				v = src[e];
-			if (TRANS)
-				lds[i * ld + c] = v;
-			else
-				lds[c * ld + i] = v;
+			const int i = p * PS + (e & (PS - 1)) - rr;
+			ok[b] = p < np && e < run && i >= 0 && i < R && (!SKIPTILES || (i >> 4) >= (c >> 4));
+			ii[b] = i;
+			cc[b] = c;
+			const bool ld = ok[b] && i < Rv && c < Cv && (!LOWER || i >= c);
+			v[b] = ld ? base[(size_t)p * PS * g.cn + e] : T(0);
+			e += WAVE;
+			if (e >= run)
+			{
+				e = lane;
+				p += NW;
+			}
 		}
+#pragma unroll
+		for (int b = 0; b < B; b++)
+			if (ok[b])
+				put(ii[b], cc[b], v[b]);
 	}
+}
+
+// Cooperative copy of the R x C block at absolute (r0, c0) of a panel-major
+// matrix into LDS, element (i,c) stored at lds[c*ld + i] (TRANS=false) or
+// lds[i*ld + c] (TRANS=true); elements outside Rv x Cv (and, with LOWER, above
+// the diagonal) are stored as zero.
+template <typename T, int PS, int NT, bool TRANS, bool LOWER = false>
+__device__ __forceinline__ void g2s(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, T *lds, int ld)
+{
+	g2s_walk<T, PS, NT, LOWER, false>(g, r0, c0, R, C, Rv, Cv, [=](int i, int c, T v) {
+		if (TRANS)
+			lds[i * ld + c] = v;
+		else
+			lds[c * ld + i] = v;
+	});
 }
 
 // Cooperative copy of LDS (lds[c*ld + i]) back to the Rv x Cv block at
@@ -162,30 +190,7 @@ __device__ __forceinline__ void s2g(PM<T, PS> g, int r0, int c0, int Rv, int Cv,
 template <typename T, int PS, int NT, bool LOWER, typename ACC>
 __device__ __forceinline__ void g2s_acc(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, const ACC A)
 {
-	constexpr int NW = NT / WAVE;
-	const int rr = r0 & (PS - 1);
-	const int np = (rr + R + PS - 1) / PS;
-	const int run = PS * C;
-	const T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
-	const int lane = threadIdx.x & (WAVE - 1);
-	for (int p = threadIdx.x / WAVE; p < np; p += NW)
-	{
-		const T *src = base + (size_t)p * PS * g.cn;
-		for (int e = lane; e < run; e += WAVE)
-		{
-			const int c = (unsigned)e / PS;
-			const int q = e & (PS - 1);
-			const int i = p * PS + q - rr;
-			if (i < 0 || i >= R)
-				continue;
-			if (LOWER && (i >> 4) < (c >> 4))
-				continue;
-			T v = T(0);
-			if (i < Rv && c < Cv && (!LOWER || i >= c))
-				v = src[e];
-			A(i, c) = v;
-		}
-	}
+	g2s_walk<T, PS, NT, LOWER, LOWER>(g, r0, c0, R, C, Rv, Cv, [=](int i, int c, T v) { A(i, c) = v; });
 }
 
 template <typename T, int PS, int NT, bool LOWER, typename ACC>

@@ -85,16 +85,28 @@ __host__ __device__ __forceinline__ int ric_footprint(int nkmax, int nxmax)
 //   M (packed lower tiles): L[k+1] at the start of stage k, then
 //     RSQ[k] + W W^T, then L[k]
 //   W (tile rectangle): BAbt[k], then W = BAbt[k] * L[k+1](nu1:, nu1:)
-// The two matrix-core stages of a Riccati step are kept out of line: inlined
-// into the stage loop, their address arithmetic is hoisted across the whole
-// sweep and the one-wave kernel needs 248 VGPRs + 64 AGPRs (one wave per
-// SIMD); out of line three instances share a SIMD.
+// The two matrix-core stages of a Riccati step are inlined into the kernels
+// of two and four waves per instance.  The one-wave kernel calls them out of
+// line (inlined, their address arithmetic is hoisted across the whole sweep
+// and it needs 248 VGPRs + 64 AGPRs, one wave per SIMD), with the LDS base
+// passed opaquely (see lds_opaque).
+// An LDS address rebuilt from its 32-bit offset behind an empty asm: the
+// callee then keeps the base in a register instead of resolving the kernel's
+// dynamic-LDS variable through the offset table at every access.
+__device__ __forceinline__ double *lds_opaque(double *p)
+{
+	unsigned off = (unsigned)(size_t)(__attribute__((address_space(3))) double *)p;
+	asm volatile("" : "+v"(off));
+	off = __builtin_amdgcn_readfirstlane(off);
+	return (double *)(__attribute__((address_space(3))) double *)(size_t)off;
+}
+
 // ---------------------------------------------------------------------------
 // W <- W * Lxx (trmm_rlnn), Lxx = L[k+1](u1:u1+n1, u1:u1+n1), held in M.
 // 16-row strips per wave, the strip's 16 x 64 result in four accumulators;
 // only this wave reads its strip, so it is overwritten in place.
 template <int NT>
-__device__ __attribute__((noinline)) void ric_trmm_w(const TileLower M, const TileRect W, int nkt, int n1, int n1t, int u1)
+__device__ __forceinline__ void ric_trmm_w(const TileLower M, const TileRect W, int nkt, int n1, int n1t, int u1)
 {
 	typedef d4_t V4;
 	constexpr int NW = NT / WAVE;
@@ -134,7 +146,7 @@ __device__ __attribute__((noinline)) void ric_trmm_w(const TileLower M, const Ti
 // M += W W^T on the lower tiles (syrk_ln): (block column bj, group of up to
 // 4 tiles bi = g..g+3) per wave, the B operand shared by the group.
 template <int NT>
-__device__ __attribute__((noinline)) void ric_syrk_m(const TileLower M, const TileRect W, int nkt, int n1)
+__device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, int nkt, int n1)
 {
 	typedef d4_t V4;
 	constexpr int NW = NT / WAVE;
@@ -188,15 +200,16 @@ __device__ __attribute__((noinline)) void ric_syrk_m(const TileLower M, const Ti
 
 // The one-wave kernels call the in-LDS Cholesky out of line: inlined it
 // would raise them above the 168 registers that three waves per SIMD allow.
-// An LDS address rebuilt from its 32-bit offset behind an empty asm: the
-// callee then keeps the base in a register instead of resolving the kernel's
-// dynamic-LDS variable through the offset table at every access.
-__device__ __forceinline__ double *lds_opaque(double *p)
+template <int NT>
+__device__ __attribute__((noinline)) void ric_trmm_call(const TileLower M, const TileRect W, int nkt, int n1, int n1t, int u1)
 {
-	unsigned off = (unsigned)(size_t)(__attribute__((address_space(3))) double *)p;
-	asm volatile("" : "+v"(off));
-	off = __builtin_amdgcn_readfirstlane(off);
-	return (double *)(__attribute__((address_space(3))) double *)(size_t)off;
+	ric_trmm_w<NT>(TileLower{lds_opaque(M.p)}, TileRect{lds_opaque(W.p), W.nct}, nkt, n1, n1t, u1);
+}
+
+template <int NT>
+__device__ __attribute__((noinline)) void ric_syrk_call(const TileLower M, const TileRect W, int nkt, int n1)
+{
+	ric_syrk_m<NT>(TileLower{lds_opaque(M.p)}, TileRect{lds_opaque(W.p), W.nct}, nkt, n1);
 }
 
 template <int NT, int NMAXC>
@@ -246,7 +259,10 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 			// stage BAbt[k] (nk x n1) into W; M still holds L[k+1]
 			g2s_acc<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W);
 			__syncthreads();
-			ric_trmm_w<NT>(M, W, nkt, n1, n1t, u1);
+			if constexpr (NT == WAVE)
+				ric_trmm_call<NT>(M, W, nkt, n1, n1t, u1);
+			else
+				ric_trmm_w<NT>(M, W, nkt, n1, n1t, u1);
 			__syncthreads();
 		}
 		// M <- lower(RSQ[k]) (L[k+1] is no longer needed)
@@ -254,7 +270,10 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 		if (!last)
 		{
 			__syncthreads();
-			ric_syrk_m<NT>(M, W, nkt, nx[k + 1]);
+			if constexpr (NT == WAVE)
+				ric_syrk_call<NT>(M, W, nkt, nx[k + 1]);
+			else
+				ric_syrk_m<NT>(M, W, nkt, nx[k + 1]);
 		}
 		// L[k] = chol(M)
 		if constexpr (NT == WAVE)
@@ -404,7 +423,7 @@ __device__ __forceinline__ int ric_next(int *head, int *slot)
 }
 
 template <int PS, int NT>
-__global__ __launch_bounds__(NT) void k_ric_class(const blasfeo_amd_riccati_arg *__restrict__ args,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_ric_class(const blasfeo_amd_riccati_arg *__restrict__ args,
 	const int *__restrict__ count, const int *__restrict__ list, int *head)
 {
 	extern __shared__ double smem_d[];

@@ -275,7 +275,7 @@ def run_riccati(args, torch, bfa):
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "algorithmic_bytes": algo_bytes, "flops_per_launch": flops,
-                     "kernel": "k_ric_class<ps,256> + k_ric_class<ps,128> + k_ric_class1<ps> (the launches of one batched call)"},
+                     "kernel": "k_ric_class<ps,256> + k_ric_class<ps,128> (the class launches of one batched call, run side by side)"},
         "cpu_baseline": cpu,
         "checks": checks,
     }

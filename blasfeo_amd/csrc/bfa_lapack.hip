@@ -255,20 +255,30 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 				const int ar = c0 + 16 * r + (lane & 15);
 				const bool aok = ar < n;
 
-				for (int kk = 0; kk < c0; kk += 4)
+				// 16 columns of L per step (c0 is a multiple of 64): all of a
+				// step's loads are issued before its products, so a wave waits
+				// for memory once per 16 columns instead of once per 4
+				for (int kk = 0; kk < c0; kk += 16)
 				{
-					const int kq = kk + (lane >> 4);
-					const T av = aok ? gD.at(di + ar, dj + kq) : T(0);
+					T av[4], bv[4][4];
 #pragma unroll
-					for (int jj = 0; jj < 4; jj++)
+					for (int u = 0; u < 4; u++)
 					{
-						if (jj < C && jj <= r)
+						const int kq = kk + 4 * u + (lane >> 4);
+						av[u] = aok ? gD.at(di + ar, dj + kq) : T(0);
+#pragma unroll
+						for (int jj = 0; jj < 4; jj++)
 						{
 							const int br = c0 + 16 * jj + (lane & 15);
-							const T bv = br < n ? gD.at(di + br, dj + kq) : T(0);
-							acc[jj] = mfma16(-av, bv, acc[jj]);
+							bv[u][jj] = (jj < C && jj <= r && br < n) ? gD.at(di + br, dj + kq) : T(0);
 						}
 					}
+#pragma unroll
+					for (int u = 0; u < 4; u++)
+#pragma unroll
+						for (int jj = 0; jj < 4; jj++)
+							if (jj < C && jj <= r)
+								acc[jj] = mfma16(-av[u], bv[u][jj], acc[jj]);
 				}
 #pragma unroll
 				for (int jj = 0; jj < 4; jj++)

@@ -299,25 +299,26 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 #pragma unroll
 				for (int j = 0; j < 16; j++)
 					x[j] = lane < 16 ? D[j * 16 + lane] : T(0);
+				// no per-lane guards: the upper triangle of the tile (lanes < j)
+				// is updated too, but only the lower triangle is read (the
+				// row solve, L11 below its diagonal) or written to D
+				T myinv = T(0);
 #pragma unroll
 				for (int j = 0; j < 16; j++)
 				{
 					const T c = readlane(x[j], j);
 					const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
-					if (lane > j)
-						x[j] *= inv;
-					else if (lane == j)
-						x[j] = c * inv;
-					if (lane == 0)
-						invs[j] = inv;
+					x[j] = (lane == j) ? c * inv : x[j] * inv;
+					myinv = (lane == j) ? inv : myinv;
 #pragma unroll
 					for (int jn = j + 1; jn < 16; jn++)
 					{
 						const T l = readlane(x[j], jn);
-						if (lane >= jn)
-							x[jn] -= x[j] * l;
+						x[jn] -= x[j] * l;
 					}
 				}
+				if (lane < 16)
+					invs[lane] = myinv;
 				if (lane < 16)
 #pragma unroll
 					for (int j = 0; j < 16; j++)

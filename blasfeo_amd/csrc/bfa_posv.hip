@@ -227,6 +227,8 @@ __device__ __forceinline__ void posv_diag(float *FKK, int *flag)
 		x[j] = lane < 16 ? (double)FKK[j * 16 + lane] : 0.0;
 	double iv[16];
 	bool bad = false;
+	// no per-lane guards: the upper triangle (lanes < j) is updated too, but
+	// only the lower triangle is read below
 #pragma unroll
 	for (int j = 0; j < 16; j++)
 	{
@@ -235,16 +237,12 @@ __device__ __forceinline__ void posv_diag(float *FKK, int *flag)
 		bad = bad || !ok;
 		const double inv = ok ? rsqrt_fast(c) : 0.0;
 		iv[j] = inv;
-		if (lane > j)
-			x[j] *= inv;
-		else if (lane == j)
-			x[j] = c * inv;
+		x[j] = (lane == j) ? c * inv : x[j] * inv;
 #pragma unroll
 		for (int jj = j + 1; jj < 16; jj++)
 		{
 			const double l = readlane(x[j], jj);
-			if (lane >= jj)
-				x[jj] -= x[j] * l;
+			x[jj] -= x[j] * l;
 		}
 	}
 	// Z = L_KK^{-1}: lane c computes column c by forward substitution

@@ -438,6 +438,7 @@ __global__ __launch_bounds__(NTL) void k_getrf(const typename FactArg<T>::type *
 	extern __shared__ unsigned char smem_raw[];
 	__shared__ T invs[PW];
 	__shared__ int pivs[PW];
+	__shared__ T Lb[16 * 17];   // the unit-lower block L(r0:r0+16, r0:r0+16) of the solve
 	const int LDP = (mmax + 15) / 16 * 16 + 1;
 	T *P = reinterpret_cast<T *>(smem_raw);                              // [PW][LDP]
 	int *perm = reinterpret_cast<int *>(smem_raw + sizeof(T) * PW * LDP);  // [mmax]
@@ -465,21 +466,42 @@ __global__ __launch_bounds__(NTL) void k_getrf(const typename FactArg<T>::type *
 	{
 		const int cw = min(PW, n - c0);
 		__syncthreads();
-		// gather the panel through the current row permutation
-		for (int e = threadIdx.x; e < PW * mt * 16; e += NTL)
+		// gather the panel through the current row permutation, eight
+		// elements per thread at a time (all loads issued before the stores)
+		for (int e0 = threadIdx.x; e0 < PW * mt * 16; e0 += 8 * NTL)
 		{
-			const int c = e / (mt * 16);
-			const int i = e - c * (mt * 16);
-			T v = T(0);
-			if (i < m && c < cw)
-				v = gC.at(ci + perm[i], cj + c0 + c);
-			P[c * LDP + i] = v;
+			T v[8];
+#pragma unroll
+			for (int u = 0; u < 8; u++)
+			{
+				const int e = e0 + u * NTL;
+				const int c = e / (mt * 16);
+				const int i = e - c * (mt * 16);
+				v[u] = (e < PW * mt * 16 && i < m && c < cw) ? gC.at(ci + perm[i], cj + c0 + c) : T(0);
+			}
+#pragma unroll
+			for (int u = 0; u < 8; u++)
+			{
+				const int e = e0 + u * NTL;
+				const int c = e / (mt * 16);
+				const int i = e - c * (mt * 16);
+				if (e < PW * mt * 16)
+					P[c * LDP + i] = v[u];
+			}
 		}
 		// apply the finished unit-lower panels, 16 rows at a time
 		const int rbmax = min(c0, p);
 		for (int r0 = 0; r0 < rbmax; r0 += 16)
 		{
 			const int rh = min(16, rbmax - r0);
+			// stage the strictly lower part of L(r0:r0+rh, r0:r0+rh), one
+			// element per thread, so that the solve reads it from LDS
+			if (threadIdx.x < 256)
+			{
+				const int r = threadIdx.x >> 4, l = threadIdx.x & 15;
+				if (l < r && r < rh)
+					Lb[r * 17 + l] = gD.at(di + r0 + r, dj + r0 + l);
+			}
 			__syncthreads();
 			// unit-lower solve of rows r0..r0+rh: one lane per panel column
 			if (w == 0 && lane < cw)
@@ -496,7 +518,7 @@ __global__ __launch_bounds__(NTL) void k_getrf(const typename FactArg<T>::type *
 						T c = x[r];
 #pragma unroll
 						for (int l = 0; l < r; l++)
-							c -= gD.at(di + r0 + r, dj + r0 + l) * x[l];
+							c -= Lb[r * 17 + l] * x[l];
 						x[r] = c;
 					}
 				}
@@ -513,14 +535,20 @@ __global__ __launch_bounds__(NTL) void k_getrf(const typename FactArg<T>::type *
 				V4 acc = V4{0, 0, 0, 0};
 				const int ir = rs + rt * 16 + (lane & 15);
 				const bool iok = ir < m;
+				// the tile's four loads issued before its products
+				T av[4];
 #pragma unroll
-				for (int kk = 0; kk < 16; kk += 4)
+				for (int u = 0; u < 4; u++)
 				{
-					const int kq = kk + (lane >> 4);
-					const bool kok = kq < rh;
-					T av = (iok && kok) ? gD.at(di + ir, dj + r0 + kq) : T(0);
-					T bv = kok ? P[(lane & 15) * LDP + r0 + kq] : T(0);
-					acc = mfma16(av, bv, acc);
+					const int kq = 4 * u + (lane >> 4);
+					av[u] = (iok && kq < rh) ? gD.at(di + ir, dj + r0 + kq) : T(0);
+				}
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+				{
+					const int kq = 4 * u + (lane >> 4);
+					const T bv = kq < rh ? P[(lane & 15) * LDP + r0 + kq] : T(0);
+					acc = mfma16(av[u], bv, acc);
 				}
 #pragma unroll
 				for (int r = 0; r < 4; r++)

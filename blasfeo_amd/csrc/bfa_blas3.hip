@@ -11,7 +11,7 @@
 namespace bfa {
 
 template <typename T, int PS, int OP>
-__global__ __launch_bounds__(NT3) void k_gemm(const typename GemmArg<T>::type *__restrict__ args)
+__global__ __launch_bounds__(NT3) __attribute__((amdgpu_waves_per_eu(3))) void k_gemm(const typename GemmArg<T>::type *__restrict__ args)
 {
 	__shared__ T As[KC * LDT];
 	__shared__ T Bs[KC * LDT];

@@ -432,7 +432,7 @@ __global__ __launch_bounds__(NTL) void k_syrk_potrf(const typename SPArg<T>::typ
 // LU with partial row pivoting (blasfeo_ref/x_lapack_ref.c:1156-1536 semantics)
 // ---------------------------------------------------------------------------
 template <typename T, int PS>
-__global__ __launch_bounds__(NTL) void k_getrf(const typename FactArg<T>::type *__restrict__ args, int mmax)
+__global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k_getrf(const typename FactArg<T>::type *__restrict__ args, int mmax)
 {
 	typedef typename Vec4<T>::type V4;
 	extern __shared__ unsigned char smem_raw[];

@@ -567,9 +567,9 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 		// instances per CU), from autotuner runs on the 8192 x N=20 batch
 		// (nx ~ U{8..64}, nu ~ U{2..16}) on MI355X: two waves per instance up
 		// to 24 KB (4 per CU up to 8 KB, 8 above: fewer resident instances
-		// spread a small class over more CUs), two waves and 4 per CU up to
-		// 40 KB except four waves and 3 per CU at 32 KB, four waves and 2 per
-		// CU above.  The tuning knobs override both.
+		// spread a small class over more CUs), four waves and 3 per CU at
+		// 32 KB, four waves and 2 per CU above.  The tuning knobs override
+		// both.
 		int per_cu, nt;
 		if (h_ric_cap[cls] <= 24576)
 		{
@@ -580,11 +580,6 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 		{
 			nt = 256;
 			per_cu = 3;
-		}
-		else if (h_ric_cap[cls] <= 40960)
-		{
-			nt = 128;
-			per_cu = 4;
 		}
 		else
 		{

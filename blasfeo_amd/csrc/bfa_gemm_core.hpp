@@ -170,9 +170,29 @@ __device__ __forceinline__ void gemm_core(T *As, T *Bs, int m, int n, int k, T a
 			}
 		}
 
-		// epilogue: D = beta*C + alpha*acc (trmm: alpha*acc), element-wise
+		// epilogue: D = beta*C + alpha*acc (trmm: alpha*acc), element-wise,
+		// a row half (x) of the wave's quarter at a time: its 8 elements of C
+		// are all loaded before any of D is stored (C may be D: in place,
+		// each element is read before it is written either way), so the
+		// loads overlap instead of each waiting behind the previous store.
 #pragma unroll
 		for (int x = 0; x < 2; x++)
+		{
+			T cv[2][4];
+#pragma unroll
+			for (int y = 0; y < 2; y++)
+			{
+				const int jl = wc + 16 * y + (lane & 15);
+#pragma unroll
+				for (int r = 0; r < 4; r++)
+				{
+					const int il = wr + 16 * x + acc_row<T>(lane, r);
+					const int gi = i0 + il, gj = j0 + jl;
+					const bool need = OP != OP_TRMM_RLNN && beta != T(0) && il < mv && jl < nv &&
+						!((OP == OP_SYRK || OP == OP_SYRK_MN) && gi < gj);
+					cv[y][r] = need ? gC.at(ci + gi, cj + gj) : T(0);
+				}
+			}
 #pragma unroll
 			for (int y = 0; y < 2; y++)
 			{
@@ -188,10 +208,11 @@ __device__ __forceinline__ void gemm_core(T *As, T *Bs, int m, int n, int k, T a
 						continue;
 					T val = alpha * acc[x][y][r];
 					if (OP != OP_TRMM_RLNN && beta != T(0))
-						val = beta * gC.at(ci + gi, cj + gj) + val;
+						val = beta * cv[y][r] + val;
 					gD.at(di + gi, dj + gj) = val;
 				}
 			}
+		}
 	}
 }
 

@@ -216,24 +216,40 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 		const int C = min(4, R);         // tile columns of the panel
 		const int c0 = 16 * J0;          // first column of the panel
 		__syncthreads();
-		// 1. stage the panel (lower part; identity beyond n)
-		for (int t = w; t < R * 4; t += PT_NW)
+		// 1. stage the panel (lower part; identity beyond n), four tiles per
+		// wave at a time: their 16 loads per lane are issued before any store
+		for (int t0 = w; t0 < R * 4; t0 += 4 * PT_NW)
 		{
-			const int r = t >> 2, jj = t & 3;
-			if (jj >= C || r < jj)
-				continue;
-			T *PT = P + t * 256;
-			const int i = lane & 15;
-			const int gr = c0 + 16 * r + i;
+			T v[4][4];
 #pragma unroll
-			for (int k = 0; k < 4; k++)
+			for (int u = 0; u < 4; u++)
 			{
-				const int j = 4 * k + (lane >> 4);
-				const int gc = c0 + 16 * jj + j;
-				T v = (gr == gc) ? T(1) : T(0);
-				if (gr < n && gc < n && gr >= gc)
-					v = gC.at(ci + gr, cj + gc);
-				PT[j * 16 + i] = v;
+				const int t = t0 + u * PT_NW;
+				const int r = t >> 2, jj = t & 3;
+				const int i = lane & 15;
+				const int gr = c0 + 16 * r + i;
+#pragma unroll
+				for (int k = 0; k < 4; k++)
+				{
+					const int j = 4 * k + (lane >> 4);
+					const int gc = c0 + 16 * jj + j;
+					const bool live = t < R * 4 && jj < C && r >= jj;
+					v[u][k] = (live && gr < n && gc < n && gr >= gc) ? gC.at(ci + gr, cj + gc)
+					                                                  : ((gr == gc) ? T(1) : T(0));
+				}
+			}
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+			{
+				const int t = t0 + u * PT_NW;
+				const int r = t >> 2, jj = t & 3;
+				if (t >= R * 4 || jj >= C || r < jj)
+					continue;
+				T *PT = P + t * 256;
+				const int i = lane & 15;
+#pragma unroll
+				for (int k = 0; k < 4; k++)
+					PT[(4 * k + (lane >> 4)) * 16 + i] = v[u][k];
 			}
 		}
 		__syncthreads();

@@ -85,3 +85,43 @@ def test_riccati_graph_replay(gpu):
             g.replay()
         torch.cuda.synchronize()
         assert torch.equal(dev.outbuf, ref), "replay %d differs from the eager sweep" % rep
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_riccati_tuned_shapes(gpu, ps):
+    """The launch shapes the tuning knobs select besides the defaults: one
+    wave per instance (k_ric_class1 and k_ric_class<ps,64>: out-of-line stage
+    calls, guarded partial panels) for the classes up to 40 KB, and four waves
+    at one instance per CU for the small classes; every stage's factor and dA
+    against the oracle at the 1e-11 rule."""
+    import torch
+    from blasfeo_amd import autotune
+    rng = np.random.default_rng(660 + ps)
+    # nu + nx <= 64 throughout, so every instance lands in a class up to 40 KB
+    nxs = list(rng.integers(1, 49, 20)) + [48, 1, 16]
+    nus = list(rng.integers(1, 17, 20)) + [16, 1, 15]
+    rb = RiccatiBatch(nxs, nus, N=5, ps=ps)
+    inp = rb.fill_inputs(rng)
+    host = HostRiccati(rb, inp)
+    oracle.lib().or_dric_trf_batch(ps, rb.batch, host.argv.ctypes.data)
+    shapes = [(64, 12), (64, 3), (256, 1)]
+    try:
+        for nt, pc in shapes:
+            autotune.reset()
+            for c in range(5):
+                autotune.set_knob("ric_threads", c, nt)
+                autotune.set_knob("ric_per_cu", c, pc)
+            dev = DeviceRiccati(rb, inp)
+            print("riccati ps=%d threads=%d per_cu=%d" % (ps, nt, pc), flush=True)
+            dev.run()
+            torch.cuda.synchronize()
+            gbuf = dev.outbuf.cpu().numpy()
+            for i in range(rb.batch):
+                for k in range(rb.N + 1):
+                    g = rb.out.get(gbuf, rb.i_L[i][k])
+                    r = host.L(i, k)
+                    assert_close(np.tril(g), np.tril(r), 1e-11, "threads %d instance %d stage %d" % (nt, i, k))
+                    assert_close(rb.out.dview(gbuf, rb.i_L[i][k]), rb.out.dview(host.outbuf, rb.i_L[i][k]),
+                                 1e-11, "threads %d instance %d stage %d dA" % (nt, i, k))
+    finally:
+        autotune.reset()

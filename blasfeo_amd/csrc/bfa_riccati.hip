@@ -144,9 +144,13 @@ __device__ __forceinline__ void ric_trmm_w(const TileLower M, const TileRect W, 
 }
 
 // M += W W^T on the lower tiles (syrk_ln): (block column bj, group of up to
-// 4 tiles bi = g..g+3) per wave, the B operand shared by the group.
-template <int NT>
-__device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, int nkt, int n1)
+// 4 tiles bi = g..g+3) per wave, the B operand shared by the group.  FROMG:
+// M = lower(gR) + W W^T instead, the accumulators started from the nk x nk
+// lower triangle of gR in global memory (zero elsewhere), so that M needs no
+// separate staging pass.
+template <int NT, int PS = 4, bool FROMG = false>
+__device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, int nkt, int n1,
+	const PM<double, PS> gR = PM<double, PS>{}, int nk = 0)
 {
 	typedef d4_t V4;
 	constexpr int NW = NT / WAVE;
@@ -171,7 +175,13 @@ __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, 
 			if (g + t < nkt)
 #pragma unroll
 				for (int r = 0; r < 4; r++)
-					acc[t][r] = M((g + t) * 16 + acc_row<double>(lane, r), bj * 16 + (lane & 15));
+				{
+					const int row = (g + t) * 16 + acc_row<double>(lane, r), col = bj * 16 + (lane & 15);
+					if (FROMG)
+						acc[t][r] = (row < nk && col < nk && row >= col) ? gR.at(row, col) : 0.0;
+					else
+						acc[t][r] = M(row, col);
+				}
 		}
 		for (int l0 = 0; l0 < n1; l0 += 4)
 		{
@@ -265,15 +275,19 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 				ric_trmm_w<NT>(M, W, nkt, n1, n1t, u1);
 			__syncthreads();
 		}
-		// M <- lower(RSQ[k]) (L[k+1] is no longer needed)
-		g2s_acc<double, PS, NT, true>(view<double, PS>(&RSQ[k]), 0, 0, nkt * 16, nkt * 16, nk, nk, M);
+		// M <- lower(RSQ[k]) + W W^T (L[k+1] is no longer needed); the
+		// multi-wave kernels start the syrk accumulators from RSQ[k] itself
+		if (last || NT == WAVE)
+			g2s_acc<double, PS, NT, true>(view<double, PS>(&RSQ[k]), 0, 0, nkt * 16, nkt * 16, nk, nk, M);
 		if (!last)
 		{
-			__syncthreads();
 			if constexpr (NT == WAVE)
+			{
+				__syncthreads();
 				ric_syrk_call<NT>(M, W, nkt, nx[k + 1]);
+			}
 			else
-				ric_syrk_m<NT>(M, W, nkt, nx[k + 1]);
+				ric_syrk_m<NT, PS, true>(M, W, nkt, nx[k + 1], view<double, PS>(&RSQ[k]), nk);
 		}
 		// L[k] = chol(M)
 		if constexpr (NT == WAVE)

@@ -175,10 +175,18 @@ namespace bfa {
 // v_readlane, so the panel needs no workgroup barrier; then the waves apply
 // the rank-16 trailing update to the lower tiles on the matrix cores, one
 // block column of tiles at a time with independent accumulators.
+// While wave 0 factors the first panel, the other waves call side(): work of
+// the caller's that touches nothing the factorisation reads or writes (a
+// no-op by default; with one wave, or no panel, every wave calls it once).
 // Called by all NT threads of the workgroup; starts and ends with a barrier.
 // ---------------------------------------------------------------------------
-template <typename T, int NT, int NMAX, typename ACC, bool PADCOLS = true>
-__device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
+struct NoSide
+{
+	__device__ __forceinline__ void operator()() const {}
+};
+
+template <typename T, int NT, int NMAX, typename ACC, bool PADCOLS = true, typename SIDE = NoSide>
+__device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, const SIDE &side = SIDE{})
 {
 	typedef typename Vec4<T>::type V4;
 	constexpr int NW = NT / WAVE;
@@ -187,6 +195,10 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
 	const int nt = (n + 15) / 16;   // panels
 	const int mt = (m + 15) / 16;   // tile rows
 	__syncthreads();
+	// no first panel (n = 0): every wave calls side() here; with one wave it
+	// is called after the factorisation
+	if (nt == 0)
+		side();
 	for (int jb = 0; jb < nt; jb++)
 	{
 		const int c0 = jb * 16;
@@ -302,6 +314,8 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
 					M(c0 + 64 + lane, c0 + j) = x1[j];
 			}
 		}
+		else if (jb == 0)
+			side();
 		__syncthreads();
 		// trailing update: tasks (block column bj > jb, group of up to 4 tiles
 		// bi = g..g+3 >= bj) spread over the waves, independent accumulators
@@ -354,6 +368,10 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd)
 		}
 		__syncthreads();
 	}
+	if (NW == 1 && nt > 0)
+		side();
+	if (nt == 0 || NW == 1)
+		__syncthreads();
 }
 
 

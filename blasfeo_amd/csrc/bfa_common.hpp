@@ -93,17 +93,20 @@ __device__ __forceinline__ PM<T, PS> view(const typename MatOf<T>::type *s)
 // above the diagonal, i < c) are passed on as zero and never read; with
 // SKIPTILES, elements whose 16x16 tile lies above the diagonal are skipped.
 // put(i, c, v) stores element (i, c) of the block.
-template <typename T, int PS, int NT, bool LOWER, bool SKIPTILES, typename PUT>
+template <typename T, int PS, int NT, bool LOWER, bool SKIPTILES, typename PUT, int W0 = 0>
 __device__ __forceinline__ void g2s_walk(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, PUT put)
 {
-	constexpr int NW = NT / WAVE;
+	// waves W0 .. NT/WAVE - 1 take part (all of them by default)
+	constexpr int NW = NT / WAVE - W0;
 	constexpr int B = 8;
 	const int rr = r0 & (PS - 1);
 	const int np = (rr + R + PS - 1) / PS;
 	const int run = PS * C;
 	const T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
 	const int lane = threadIdx.x & (WAVE - 1);
-	int p = threadIdx.x / WAVE, e = lane;
+	int p = (int)(threadIdx.x / WAVE) - W0, e = lane;
+	if (p < 0)
+		return;
 	while (p < np)
 	{
 		T v[B];
@@ -187,10 +190,11 @@ __device__ __forceinline__ void s2g(PM<T, PS> g, int r0, int c0, int Rv, int Cv,
 // A(i, c) (element (i, c) of the block).  With LOWER, g2s_acc skips elements
 // whose 16x16 tile lies above the diagonal (packed lower-tile storage has no
 // such tiles) and zero-fills the upper part of diagonal tiles.
-template <typename T, int PS, int NT, bool LOWER, typename ACC>
+template <typename T, int PS, int NT, bool LOWER, typename ACC, int W0 = 0>
 __device__ __forceinline__ void g2s_acc(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, const ACC A)
 {
-	g2s_walk<T, PS, NT, LOWER, LOWER>(g, r0, c0, R, C, Rv, Cv, [=](int i, int c, T v) { A(i, c) = v; });
+	auto put = [=](int i, int c, T v) { A(i, c) = v; };
+	g2s_walk<T, PS, NT, LOWER, LOWER, decltype(put), W0>(g, r0, c0, R, C, Rv, Cv, put);
 }
 
 template <typename T, int PS, int NT, bool LOWER, typename ACC>

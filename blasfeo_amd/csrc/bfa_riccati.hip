@@ -266,8 +266,11 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 			const int n1 = nx[k + 1];
 			const int u1 = nu[k + 1];
 			const int n1t = (n1 + 15) / 16;
-			// stage BAbt[k] (nk x n1) into W; M still holds L[k+1]
-			g2s_acc<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W);
+			// stage BAbt[k] (nk x n1) into W; M still holds L[k+1].  The
+			// multi-wave kernels loaded it during the previous stage's
+			// Cholesky (its first panel, on the waves other than 0)
+			if constexpr (NT == WAVE)
+				g2s_acc<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W);
 			__syncthreads();
 			if constexpr (NT == WAVE)
 				ric_trmm_call<NT>(M, W, nkt, n1, n1t, u1);
@@ -289,11 +292,23 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 			else
 				ric_syrk_m<NT, PS, true>(M, W, nkt, nx[k + 1], view<double, PS>(&RSQ[k]), nk);
 		}
-		// L[k] = chol(M)
+		// L[k] = chol(M); W is free (the syrk has read it), so the waves
+		// other than 0 load the next stage's BAbt into it meanwhile
 		if constexpr (NT == WAVE)
 			ric_chol_call<NT, NMAXC>(M, nk, invd);
 		else
-			chol_acc<double, NT, NMAXC>(M, nk, nk, invd);
+		{
+			const int kn = k - 1;   // next stage (k = N for the last stage: kn = N - 1)
+			auto side = [&]() {
+				if (kn >= 0)
+				{
+					const int nkn = nu[kn] + nx[kn], n1n = nx[kn + 1];
+					g2s_acc<double, PS, NT, false, TileRect, 1>(view<double, PS>(&BAbt[kn]), 0, 0,
+						(nkn + 15) / 16 * 16, (n1n + 15) / 16 * 16, nkn, n1n, W);
+				}
+			};
+			chol_acc<double, NT, NMAXC, TileLower, true, decltype(side)>(M, nk, nk, invd, side);
+		}
 		s2g_acc<double, PS, NT, true>(view<double, PS>(&L[k]), 0, 0, nk, nk, M, 0);
 		for (int j = threadIdx.x; j < nk; j += NT)
 			L[k].dA[j] = invd[j];

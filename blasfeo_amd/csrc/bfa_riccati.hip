@@ -268,10 +268,15 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 			const int n1t = (n1 + 15) / 16;
 			// stage BAbt[k] (nk x n1) into W; M still holds L[k+1].  The
 			// multi-wave kernels loaded it during the previous stage's
-			// Cholesky (its first panel, on the waves other than 0)
+			// Cholesky (its first panel, on the waves other than 0), so W and
+			// M are complete at the Cholesky's barriers; without a barrier
+			// here the previous stage's stores of L (which only read M)
+			// overlap the trmm (which only reads M and writes W)
 			if constexpr (NT == WAVE)
+			{
 				g2s_acc<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W);
-			__syncthreads();
+				__syncthreads();
+			}
 			if constexpr (NT == WAVE)
 				ric_trmm_call<NT>(M, W, nkt, n1, n1t, u1);
 			else

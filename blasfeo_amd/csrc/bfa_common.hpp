@@ -229,6 +229,44 @@ __device__ __forceinline__ void s2g_acc(PM<T, PS> g, int r0, int c0, int Rv, int
 	}
 }
 
+// A DPP move of a double (both 32-bit halves with the same control): lanes
+// whose source lane is outside the pattern, or whose row is not in ROWMASK,
+// get `old`.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d(double old, double v)
+{
+	const long long o = __builtin_bit_cast(long long, old), x = __builtin_bit_cast(long long, v);
+	const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, ROWMASK, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, ROWMASK, 0xf, false);
+	return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Wave-wide sum and max of a double with DPP moves (quad permutes, row
+// shifts by 4 and 8, row broadcasts of lanes 15 and 31): the result is in
+// lane 63, in a fixed order.  Register to register, where a shuffle goes
+// through the LDS crossbar.  wave_max_d: for values >= 0 or NaN-free inputs
+// (the identity used is the lane's own value).
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+	v += dpp_d<0xb1, 0xf>(0.0, v);   // quad_perm 1,0,3,2
+	v += dpp_d<0x4e, 0xf>(0.0, v);   // quad_perm 2,3,0,1
+	v += dpp_d<0x114, 0xf>(0.0, v);  // row_shr:4
+	v += dpp_d<0x118, 0xf>(0.0, v);  // row_shr:8
+	v += dpp_d<0x142, 0xa>(0.0, v);  // row_bcast:15 into rows 1, 3
+	v += dpp_d<0x143, 0xc>(0.0, v);  // row_bcast:31 into rows 2, 3
+	return v;
+}
+__device__ __forceinline__ double wave_max_d(double v)
+{
+	v = fmax(v, dpp_d<0xb1, 0xf>(v, v));
+	v = fmax(v, dpp_d<0x4e, 0xf>(v, v));
+	v = fmax(v, dpp_d<0x114, 0xf>(v, v));
+	v = fmax(v, dpp_d<0x118, 0xf>(v, v));
+	v = fmax(v, dpp_d<0x142, 0xa>(v, v));
+	v = fmax(v, dpp_d<0x143, 0xc>(v, v));
+	return v;
+}
+
 // wave-wide reductions via cross-lane moves
 __device__ __forceinline__ double shfl_xor_d(double v, int mask)
 {

@@ -78,10 +78,9 @@ __device__ __forceinline__ void block_max(double *v, double *red)
 	const int lane = lane_id(), w = wave_id();
 #pragma unroll
 	for (int q = 0; q < NV; q++)
-		for (int m = 1; m < WAVE; m <<= 1)
-			v[q] = fmax(v[q], __shfl_xor(v[q], m, WAVE));
+		v[q] = wave_max_d(v[q]);
 	__syncthreads();
-	if (lane == 0)
+	if (lane == WAVE - 1)
 #pragma unroll
 		for (int q = 0; q < NV; q++)
 			red[w * 12 + q] = v[q];
@@ -104,10 +103,9 @@ __device__ __forceinline__ void block_sum(double *v, double *red)
 	const int lane = lane_id(), w = wave_id();
 #pragma unroll
 	for (int q = 0; q < NV; q++)
-		for (int m = 1; m < WAVE; m <<= 1)
-			v[q] += __shfl_xor(v[q], m, WAVE);
+		v[q] = wave_sum_d(v[q]);
 	__syncthreads();
-	if (lane == 0)
+	if (lane == WAVE - 1)
 #pragma unroll
 		for (int q = 0; q < NV; q++)
 			red[w * 12 + q] = v[q];

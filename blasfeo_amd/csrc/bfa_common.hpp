@@ -273,23 +273,53 @@ __device__ __forceinline__ double shfl_xor_d(double v, int mask)
 	return __shfl_xor(v, mask, WAVE);
 }
 
+// DPP move of a 32-bit value (see dpp_d).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i(int old, int v)
+{
+	return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWMASK, 0xf, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_t(float old, float v)
+{
+	return __builtin_bit_cast(float, dpp_i<CTRL, ROWMASK>(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v)));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_t(double old, double v)
+{
+	return dpp_d<CTRL, ROWMASK>(old, v);
+}
+
+template <typename T, int CTRL, int ROWMASK>
+__device__ __forceinline__ void argmax_step(T &best, int &idx)
+{
+	// a lane without a source in the pattern compares with itself
+	const T ob = dpp_t<CTRL, ROWMASK>(best, best);
+	const int oi = dpp_i<CTRL, ROWMASK>(idx, idx);
+	if (ob > best || (ob == best && oi < idx))
+	{
+		best = ob;
+		idx = oi;
+	}
+}
+
 // Wave-wide argmax of |v| with the reference's pivot rule: a candidate wins
 // only if strictly larger (so NaN never wins), ties go to the smaller row
-// index.  Returns the winning row index (uniform across the wave).
+// index.  Each lane passes its own candidate, which is never NaN (a lane's
+// scan only takes strictly larger values), so the rule is a total order and
+// the winner does not depend on the reduction tree: DPP moves (quad
+// permutes, row shifts, row broadcasts) bring it to lane 63, register to
+// register.  Returns the winning row index (uniform across the wave).
 template <typename T>
 __device__ __forceinline__ int wave_argmax_abs(T best, int idx)
 {
-	for (int m = 1; m < WAVE; m <<= 1)
-	{
-		T ob = __shfl_xor(best, m, WAVE);
-		int oi = __shfl_xor(idx, m, WAVE);
-		if (ob > best || (ob == best && oi < idx))
-		{
-			best = ob;
-			idx = oi;
-		}
-	}
-	return __builtin_amdgcn_readfirstlane(idx);
+	argmax_step<T, 0xb1, 0xf>(best, idx);
+	argmax_step<T, 0x4e, 0xf>(best, idx);
+	argmax_step<T, 0x114, 0xf>(best, idx);
+	argmax_step<T, 0x118, 0xf>(best, idx);
+	argmax_step<T, 0x142, 0xa>(best, idx);
+	argmax_step<T, 0x143, 0xc>(best, idx);
+	return __builtin_amdgcn_readlane(idx, WAVE - 1);
 }
 
 // 1/sqrt(c) for c > 0 from the hardware estimate refined by one Newton step

@@ -309,9 +309,10 @@ __device__ __forceinline__ void argmax_step(T &best, int &idx)
 // scan only takes strictly larger values), so the rule is a total order and
 // the winner does not depend on the reduction tree: DPP moves (quad
 // permutes, row shifts, row broadcasts) bring it to lane 63, register to
-// register.  Returns the winning row index (uniform across the wave).
+// register.  Returns the winning row index (uniform across the wave);
+// wave_argmax_abs_to_last leaves the winner (value and index) in lane 63.
 template <typename T>
-__device__ __forceinline__ int wave_argmax_abs(T best, int idx)
+__device__ __forceinline__ void wave_argmax_abs_to_last(T &best, int &idx)
 {
 	argmax_step<T, 0xb1, 0xf>(best, idx);
 	argmax_step<T, 0x4e, 0xf>(best, idx);
@@ -319,6 +320,11 @@ __device__ __forceinline__ int wave_argmax_abs(T best, int idx)
 	argmax_step<T, 0x118, 0xf>(best, idx);
 	argmax_step<T, 0x142, 0xa>(best, idx);
 	argmax_step<T, 0x143, 0xc>(best, idx);
+}
+template <typename T>
+__device__ __forceinline__ int wave_argmax_abs(T best, int idx)
+{
+	wave_argmax_abs_to_last(best, idx);
 	return __builtin_amdgcn_readlane(idx, WAVE - 1);
 }
 

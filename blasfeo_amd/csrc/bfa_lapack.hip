@@ -455,6 +455,8 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 	__shared__ T invs[PW];
 	__shared__ int pivs[PW];
 	__shared__ T Lb[16 * 17];   // the unit-lower block L(r0:r0+16, r0:r0+16) of the solve
+	__shared__ T wbest[NTL / WAVE];   // per-wave pivot candidates
+	__shared__ int widx[NTL / WAVE];
 	const int LDP = (mmax + 15) / 16 * 16 + 1;
 	T *P = reinterpret_cast<T *>(smem_raw);                              // [PW][LDP]
 	int *perm = reinterpret_cast<int *>(smem_raw + sizeof(T) * PW * LDP);  // [mmax]
@@ -577,7 +579,75 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 		__syncthreads();
 		// factor the panel rows c0..m-1 (only columns that have a pivot row)
 		const int cf = min(cw, p - c0);
-		if (w == 0 && cf > 0)
+		if (cf > 0 && m - c0 > WAVE)
+		{
+			// all waves, rows striped over the threads: per column a wave
+			// argmax, the waves' winners combined through LDS (the same
+			// total order, so the same pivot), the swap, then the rank-1
+			// update of the rows below; three barriers per column
+			for (int j = 0; j < cf; j++)
+			{
+				const int jr = c0 + j;
+				T best = T(0);
+				int bi = jr;
+				for (int i = jr + threadIdx.x; i < m; i += NTL)
+				{
+					const T v = P[j * LDP + i];
+					const T av = v > T(0) ? v : -v;
+					if (av > best)
+					{
+						best = av;
+						bi = i;
+					}
+				}
+				wave_argmax_abs_to_last<T>(best, bi);
+				if (lane == WAVE - 1)
+				{
+					wbest[w] = best;
+					widx[w] = bi;
+				}
+				__syncthreads();
+				T b = wbest[0];
+				int ip = widx[0];
+				for (int ww = 1; ww < NTL / WAVE; ww++)
+					if (wbest[ww] > b || (wbest[ww] == b && widx[ww] < ip))
+					{
+						b = wbest[ww];
+						ip = widx[ww];
+					}
+				if (threadIdx.x == 0)
+					pivs[j] = ip;
+				if (ip != jr)
+				{
+					if (threadIdx.x < PW)
+					{
+						const T t0 = P[threadIdx.x * LDP + jr];
+						P[threadIdx.x * LDP + jr] = P[threadIdx.x * LDP + ip];
+						P[threadIdx.x * LDP + ip] = t0;
+					}
+					if (threadIdx.x == 0)
+					{
+						const int t1 = perm[jr];
+						perm[jr] = perm[ip];
+						perm[ip] = t1;
+					}
+				}
+				__syncthreads();
+				const T piv = P[j * LDP + jr];
+				const T inv = piv != T(0) ? T(1) / piv : T(0);
+				if (threadIdx.x == 0)
+					invs[j] = inv;
+				for (int i = jr + 1 + threadIdx.x; i < m; i += NTL)
+				{
+					const T l = P[j * LDP + i] * inv;
+					P[j * LDP + i] = l;
+					for (int jj = j + 1; jj < cw; jj++)
+						P[jj * LDP + i] -= l * P[jj * LDP + jr];
+				}
+				__syncthreads();
+			}
+		}
+		else if (w == 0 && cf > 0)
 		{
 			for (int j = 0; j < cf; j++)
 			{

@@ -548,31 +548,49 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 			// rows below: P(r0+16.., :) -= L(r0+16.., r0:r0+16) * P(r0:r0+16, :)
 			const int rs = r0 + 16;
 			const int ntile = (m - rs + 15) / 16;
-			for (int rt = w; rt < ntile; rt += NTL / WAVE)
+			// the right operand (the solved rows r0..r0+16) is the same for
+			// every tile: read once
+			T bv[4];
+#pragma unroll
+			for (int u = 0; u < 4; u++)
 			{
-				V4 acc = V4{0, 0, 0, 0};
-				const int ir = rs + rt * 16 + (lane & 15);
-				const bool iok = ir < m;
-				// the tile's four loads issued before its products
-				T av[4];
+				const int kq = 4 * u + (lane >> 4);
+				bv[u] = kq < rh ? P[(lane & 15) * LDP + r0 + kq] : T(0);
+			}
+			// up to four of the wave's row tiles at a time: all their loads
+			// are issued before any product
+			for (int rt0 = w; rt0 < ntile; rt0 += 4 * (NTL / WAVE))
+			{
+				T av[4][4];
 #pragma unroll
-				for (int u = 0; u < 4; u++)
+				for (int t = 0; t < 4; t++)
 				{
-					const int kq = 4 * u + (lane >> 4);
-					av[u] = (iok && kq < rh) ? gD.at(di + ir, dj + r0 + kq) : T(0);
+					const int rt = rt0 + t * (NTL / WAVE);
+					const int ir = rs + rt * 16 + (lane & 15);
+					const bool iok = rt < ntile && ir < m;
+#pragma unroll
+					for (int u = 0; u < 4; u++)
+					{
+						const int kq = 4 * u + (lane >> 4);
+						av[t][u] = (iok && kq < rh) ? gD.at(di + ir, dj + r0 + kq) : T(0);
+					}
 				}
 #pragma unroll
-				for (int u = 0; u < 4; u++)
+				for (int t = 0; t < 4; t++)
 				{
-					const int kq = 4 * u + (lane >> 4);
-					const T bv = kq < rh ? P[(lane & 15) * LDP + r0 + kq] : T(0);
-					acc = mfma16(av[u], bv, acc);
-				}
+					const int rt = rt0 + t * (NTL / WAVE);
+					if (rt >= ntile)
+						break;
+					V4 acc = V4{0, 0, 0, 0};
 #pragma unroll
-				for (int r = 0; r < 4; r++)
-				{
-					const int il = rs + rt * 16 + acc_row<T>(lane, r);
-					P[(lane & 15) * LDP + il] -= acc[r];
+					for (int u = 0; u < 4; u++)
+						acc = mfma16(av[t][u], bv[u], acc);
+#pragma unroll
+					for (int r = 0; r < 4; r++)
+					{
+						const int il = rs + rt * 16 + acc_row<T>(lane, r);
+						P[(lane & 15) * LDP + il] -= acc[r];
+					}
 				}
 			}
 		}

@@ -267,11 +267,6 @@ __device__ __forceinline__ double wave_max_d(double v)
 	return v;
 }
 
-// wave-wide reductions via cross-lane moves
-__device__ __forceinline__ double shfl_xor_d(double v, int mask)
-{
-	return __shfl_xor(v, mask, WAVE);
-}
 
 // DPP move of a 32-bit value (see dpp_d).
 template <int CTRL, int ROWMASK>

@@ -9,12 +9,12 @@
 // Launch model (size-bucketed work queue):
 //   1. a plan kernel gives every instance an LDS footprint class from its
 //      largest stage and appends its index to that class's list;
-//   2. one persistent launch per class, largest first, sized so that its LDS
-//      footprint fills the CU (e.g. five 32 KB instances per CU), pulls
-//      instances from the class list.  Small classes run one wave per
-//      instance, so a CU holds many instances and their latency chains
-//      overlap; the largest classes use 2 or 4 waves per instance so that all
-//      four SIMDs of a CU are busy;
+//   2. one persistent launch per class, largest first, the launches spread
+//      over the caller's stream and helper streams so that the classes run
+//      side by side; the workgroups of a launch pull instances from the
+//      class's atomic work queue.  Two or four waves per instance by default
+//      (one wave when the tuning knobs ask for it), a few instances per CU so
+//      that their latency chains overlap;
 //   3. instances too large for LDS (nx > 64 or footprint > 160 KB) take the
 //      general sweep, which keeps only a 16-column panel in LDS.
 #include <mutex>
@@ -468,9 +468,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
 		ric_sweep_lds<PS, NT>(args[list[t]], work);
 }
 
-// one wave per instance: registers capped at 168 so that three instances
-// share each SIMD.  Only classes of at most 40960 B use it, whose stages have
-// nu+nx <= 64 (the plan's rule), so the Cholesky keeps one panel row per lane.
+// one wave per instance (selected by the tuning knobs, not by default):
+// registers capped at 168 so that three instances share each SIMD.  Only
+// classes of at most 40960 B can use it, whose stages have nu+nx <= 64 (the
+// plan's rule), so the Cholesky keeps one panel row per lane.
 template <int PS>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3))) void k_ric_class1(
 	const blasfeo_amd_riccati_arg *__restrict__ args, const int *__restrict__ count, const int *__restrict__ list,

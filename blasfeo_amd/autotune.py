@@ -60,6 +60,7 @@ def reset():
         set_knob("ric_per_cu", c, 0)
         set_knob("ric_threads", c, 0)
     set_knob("gemm_grid_y", 0, 0)
+    set_knob("potrf_bw", 0, 0)
 
 
 def apply(tuning):
@@ -70,6 +71,8 @@ def apply(tuning):
         set_knob("ric_threads", c, v)
     if "gemm_grid_y" in tuning:
         set_knob("gemm_grid_y", 0, tuning["gemm_grid_y"])
+    if "potrf_bw" in tuning:
+        set_knob("potrf_bw", 0, tuning["potrf_bw"])
 
 
 def save(path, tuning):

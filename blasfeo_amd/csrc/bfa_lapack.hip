@@ -165,38 +165,40 @@ __global__ __launch_bounds__(NT) void k_syrk_potrf_lds(const typename SPArg<T>::
 }
 
 // ---------------------------------------------------------------------------
-// potrf_l for 128 < n <= 256: left-looking over 64-column block panels held in
-// LDS as 16x16 tiles (col-major inside a tile, tile (r, jj) of the panel at
-// (4r + jj) * 256), one 8-wave workgroup per instance.
+// potrf_l for 128 < n <= 256: left-looking over block panels of 16 BW columns
+// held in LDS as 16x16 tiles (col-major inside a tile, tile (r, jj) of the
+// panel at (BW r + jj) * 256), one workgroup of NTH threads per instance: BW =
+// 4 (128 KB, one instance per CU, 8 waves) or BW = 2 (64 KB, two instances per
+// CU, 4 waves each, whose serial chains overlap).
 //   1. the panel's lower part is staged from C (identity beyond n);
 //   2. the earlier block columns are applied on the matrix cores,
 //      P(r, jj) -= L(rows r, 0:c0) L(rows jj, 0:c0)^T, with L read from D
-//      (one wave per tile row, the A operand shared by the row's 4 tiles);
+//      (one wave per tile row, the A operand shared by the row's BW tiles);
 //   3. the panel is factored right-looking in LDS: wave 0 factors the
 //      diagonal tile in registers (as chol_lds), every thread then solves one
 //      row below it against L_jj with the reciprocal diagonal, and the
 //      panel's remaining tile columns get a rank-16 matrix-core update;
 //   4. L and the reciprocal diagonal are written to D.
 // Compared with the 16-column left-looking form this reads the earlier
-// columns once per 64 columns instead of once per 16.
+// columns once per 16 BW columns instead of once per 16.
 // ---------------------------------------------------------------------------
-constexpr int PT_NT = 512;
-constexpr int PT_NW = PT_NT / WAVE;
 constexpr int PT_NMAX = 256;
 
-template <typename T>
+template <typename T, int BW>
 __host__ __device__ inline size_t potrf_tiles_lds_bytes()
 {
-	return sizeof(T) * (64 * 256 + 16 * 17 + 16);
+	return sizeof(T) * (16 * BW * 256 + 16 * 17 + 16);
 }
 
-template <typename T, int PS>
-__global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>::type *__restrict__ args, int nmax)
+template <typename T, int PS, int BW, int NTH>
+__global__ __launch_bounds__(NTH) void k_potrf_tiles(const typename FactArg<T>::type *__restrict__ args, int nmax)
 {
+	constexpr int PT_NT = NTH;
+	constexpr int PT_NW = NTH / WAVE;
 	typedef typename Vec4<T>::type V4;
 	extern __shared__ unsigned char smem_raw[];
 	T *P = reinterpret_cast<T *>(smem_raw);
-	T *L11 = P + 64 * 256;
+	T *L11 = P + 16 * BW * 256;
 	T *invs = L11 + 16 * 17;
 	const typename FactArg<T>::type &a = args[blockIdx.x];
 	const int n = a.m;
@@ -210,22 +212,22 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 	const int ci = a.ci, cj = a.cj, di = a.di, dj = a.dj;
 	T *dD = a.sD->dA;
 
-	for (int J0 = 0; J0 < nb; J0 += 4)
+	for (int J0 = 0; J0 < nb; J0 += BW)
 	{
 		const int R = nb - J0;           // tile rows of the panel
-		const int C = min(4, R);         // tile columns of the panel
+		const int C = min(BW, R);        // tile columns of the panel
 		const int c0 = 16 * J0;          // first column of the panel
 		__syncthreads();
 		// 1. stage the panel (lower part; identity beyond n), four tiles per
 		// wave at a time: their 16 loads per lane are issued before any store
-		for (int t0 = w; t0 < R * 4; t0 += 4 * PT_NW)
+		for (int t0 = w; t0 < R * BW; t0 += 4 * PT_NW)
 		{
 			T v[4][4];
 #pragma unroll
 			for (int u = 0; u < 4; u++)
 			{
 				const int t = t0 + u * PT_NW;
-				const int r = t >> 2, jj = t & 3;
+				const int r = t / BW, jj = t % BW;
 				const int i = lane & 15;
 				const int gr = c0 + 16 * r + i;
 #pragma unroll
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 				{
 					const int j = 4 * k + (lane >> 4);
 					const int gc = c0 + 16 * jj + j;
-					const bool live = t < R * 4 && jj < C && r >= jj;
+					const bool live = t < R * BW && jj < C && r >= jj;
 					v[u][k] = (live && gr < n && gc < n && gr >= gc) ? gC.at(ci + gr, cj + gc)
 					                                                  : ((gr == gc) ? T(1) : T(0));
 				}
@@ -242,8 +244,8 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 			for (int u = 0; u < 4; u++)
 			{
 				const int t = t0 + u * PT_NW;
-				const int r = t >> 2, jj = t & 3;
-				if (t >= R * 4 || jj >= C || r < jj)
+				const int r = t / BW, jj = t % BW;
+				if (t >= R * BW || jj >= C || r < jj)
 					continue;
 				T *PT = P + t * 256;
 				const int i = lane & 15;
@@ -258,32 +260,32 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 		{
 			for (int r = w; r < R; r += PT_NW)
 			{
-				V4 acc[4];
+				V4 acc[BW];
 #pragma unroll
-				for (int jj = 0; jj < 4; jj++)
+				for (int jj = 0; jj < BW; jj++)
 				{
 					acc[jj] = V4{0, 0, 0, 0};
 					if (jj < C && jj <= r)
 #pragma unroll
 						for (int q = 0; q < 4; q++)
-							acc[jj][q] = P[(r * 4 + jj) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)];
+							acc[jj][q] = P[(r * BW + jj) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)];
 				}
 				const int ar = c0 + 16 * r + (lane & 15);
 				const bool aok = ar < n;
 
-				// 16 columns of L per step (c0 is a multiple of 64): all of a
+				// 16 columns of L per step (c0 is a multiple of 16 BW): all of a
 				// step's loads are issued before its products, so a wave waits
 				// for memory once per 16 columns instead of once per 4
 				for (int kk = 0; kk < c0; kk += 16)
 				{
-					T av[4], bv[4][4];
+					T av[4], bv[4][BW];
 #pragma unroll
 					for (int u = 0; u < 4; u++)
 					{
 						const int kq = kk + 4 * u + (lane >> 4);
 						av[u] = aok ? gD.at(di + ar, dj + kq) : T(0);
 #pragma unroll
-						for (int jj = 0; jj < 4; jj++)
+						for (int jj = 0; jj < BW; jj++)
 						{
 							const int br = c0 + 16 * jj + (lane & 15);
 							bv[u][jj] = (jj < C && jj <= r && br < n) ? gD.at(di + br, dj + kq) : T(0);
@@ -292,23 +294,23 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 #pragma unroll
 					for (int u = 0; u < 4; u++)
 #pragma unroll
-						for (int jj = 0; jj < 4; jj++)
+						for (int jj = 0; jj < BW; jj++)
 							if (jj < C && jj <= r)
 								acc[jj] = mfma16(-av[u], bv[u][jj], acc[jj]);
 				}
 #pragma unroll
-				for (int jj = 0; jj < 4; jj++)
+				for (int jj = 0; jj < BW; jj++)
 					if (jj < C && jj <= r)
 #pragma unroll
 						for (int q = 0; q < 4; q++)
-							P[(r * 4 + jj) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)] = acc[jj][q];
+							P[(r * BW + jj) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)] = acc[jj][q];
 			}
 			__syncthreads();
 		}
 		// 3. factor the panel
 		for (int jj = 0; jj < C; jj++)
 		{
-			T *D = P + (jj * 4 + jj) * 256;
+			T *D = P + (jj * BW + jj) * 256;
 			if (w == 0)
 			{
 				T x[16];
@@ -351,7 +353,7 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 				{
 					asm volatile("" ::: "memory");
 					const int r = jj + 1 + (q >> 4), i = q & 15;
-					T *row = P + (r * 4 + jj) * 256 + i;
+					T *row = P + (r * BW + jj) * 256 + i;
 					T x[16];
 #pragma unroll
 					for (int j = 0; j < 16; j++)
@@ -382,16 +384,16 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 					V4 acc;
 #pragma unroll
 					for (int q = 0; q < 4; q++)
-						acc[q] = P[(r * 4 + j2) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)];
+						acc[q] = P[(r * BW + j2) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)];
 #pragma unroll
 					for (int kk = 0; kk < 16; kk += 4)
 					{
 						const int o = (kk + (lane >> 4)) * 16 + (lane & 15);
-						acc = mfma16(-P[(r * 4 + jj) * 256 + o], P[(j2 * 4 + jj) * 256 + o], acc);
+						acc = mfma16(-P[(r * BW + jj) * 256 + o], P[(j2 * BW + jj) * 256 + o], acc);
 					}
 #pragma unroll
 					for (int q = 0; q < 4; q++)
-						P[(r * 4 + j2) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)] = acc[q];
+						P[(r * BW + j2) * 256 + (lane & 15) * 16 + acc_row<T>(lane, q)] = acc[q];
 				}
 			}
 			if (threadIdx.x < 16 && c0 + 16 * jj + threadIdx.x < n)
@@ -399,9 +401,9 @@ __global__ __launch_bounds__(PT_NT) void k_potrf_tiles(const typename FactArg<T>
 			__syncthreads();
 		}
 		// 4. write L (lower part, rows and columns < n)
-		for (int t = w; t < R * 4; t += PT_NW)
+		for (int t = w; t < R * BW; t += PT_NW)
 		{
-			const int r = t >> 2, jj = t & 3;
+			const int r = t / BW, jj = t % BW;
 			if (jj >= C || r < jj)
 				continue;
 			const T *PT = P + t * 256;
@@ -806,13 +808,31 @@ static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args
 		return set_error(BLASFEO_AMD_ETOOBIG, "%s: m up to 128 supported (the LDS-resident form); got a bound of %d", name, mmax);
 	if (MODE == 0 && mmax <= PT_NMAX)
 	{
-		const size_t shm3 = potrf_tiles_lds_bytes<T>();
-		if (ps == 4)
-			hipLaunchKernelGGL((k_potrf_tiles<T, 4>), dim3(batch), dim3(PT_NT), shm3, st, args, mmax);
-		else if (ps == 8)
-			hipLaunchKernelGGL((k_potrf_tiles<T, 8>), dim3(batch), dim3(PT_NT), shm3, st, args, mmax);
+		// block panels of 32 columns (two 4-wave instances per CU, whose
+		// chains overlap: the potrf + trsm solve sequence for 2048 x n = 256
+		// takes 1.65 instead of 1.90 ms on MI355X) or of 64 (one 8-wave
+		// instance per CU); the tuning knob potrf_bw overrides
+		const int bw = tune_value(TUNE_POTRF_BW, 0) == 4 ? 4 : 2;
+		if (bw == 4)
+		{
+			const size_t shm3 = potrf_tiles_lds_bytes<T, 4>();
+			if (ps == 4)
+				hipLaunchKernelGGL((k_potrf_tiles<T, 4, 4, 512>), dim3(batch), dim3(512), shm3, st, args, mmax);
+			else if (ps == 8)
+				hipLaunchKernelGGL((k_potrf_tiles<T, 8, 4, 512>), dim3(batch), dim3(512), shm3, st, args, mmax);
+			else
+				hipLaunchKernelGGL((k_potrf_tiles<T, 16, 4, 512>), dim3(batch), dim3(512), shm3, st, args, mmax);
+		}
 		else
-			hipLaunchKernelGGL((k_potrf_tiles<T, 16>), dim3(batch), dim3(PT_NT), shm3, st, args, mmax);
+		{
+			const size_t shm3 = potrf_tiles_lds_bytes<T, 2>();
+			if (ps == 4)
+				hipLaunchKernelGGL((k_potrf_tiles<T, 4, 2, 256>), dim3(batch), dim3(256), shm3, st, args, mmax);
+			else if (ps == 8)
+				hipLaunchKernelGGL((k_potrf_tiles<T, 8, 2, 256>), dim3(batch), dim3(256), shm3, st, args, mmax);
+			else
+				hipLaunchKernelGGL((k_potrf_tiles<T, 16, 2, 256>), dim3(batch), dim3(256), shm3, st, args, mmax);
+		}
 		return check_launch(name);
 	}
 	const size_t shm = sizeof(T) * PW * ((mmax + 15) / 16 * 16 + 1);

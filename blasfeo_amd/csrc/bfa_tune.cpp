@@ -13,6 +13,7 @@ constexpr int NCLASS = 8;
 std::atomic<int> g_ric_per_cu[NCLASS];
 std::atomic<int> g_ric_threads[NCLASS];
 std::atomic<int> g_gemm_grid_y{0};
+std::atomic<int> g_potrf_bw{0};
 
 std::atomic<int> *slot(const char *name, int index)
 {
@@ -24,6 +25,8 @@ std::atomic<int> *slot(const char *name, int index)
 		return (index >= 0 && index < NCLASS) ? &g_ric_threads[index] : nullptr;
 	if (!std::strcmp(name, "gemm_grid_y"))
 		return index == 0 ? &g_gemm_grid_y : nullptr;
+	if (!std::strcmp(name, "potrf_bw"))
+		return index == 0 ? &g_potrf_bw : nullptr;
 	return nullptr;
 }
 
@@ -33,6 +36,9 @@ bool valid(const char *name, int index, int value)
 		return true;
 	if (!std::strcmp(name, "ric_per_cu") || !std::strcmp(name, "gemm_grid_y"))
 		return value >= 1 && value <= 64;
+	// potrf_bw: tile columns per block panel of potrf_l for 128 < n <= 256
+	if (!std::strcmp(name, "potrf_bw"))
+		return value == 2 || value == 4;
 	// ric_threads: one wave only for the classes whose stages have nu+nx <= 64
 	return value == 128 || value == 256 || (value == 64 && index <= 4);
 }
@@ -48,6 +54,8 @@ int tune_value(TuneKnob knob, int index)
 		return (index >= 0 && index < NCLASS) ? g_ric_threads[index].load() : 0;
 	case TUNE_GEMM_GRID_Y:
 		return g_gemm_grid_y.load();
+	case TUNE_POTRF_BW:
+		return g_potrf_bw.load();
 	}
 	return 0;
 }

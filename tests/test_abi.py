@@ -173,7 +173,12 @@ def test_tuning_knobs_without_gpu():
         at.set_knob("gemm_grid_y", 0, 65)
     with pytest.raises(RuntimeError):
         at.set_knob("nonsense", 0, 1)
+    at.set_knob("potrf_bw", 0, 2)
+    assert at.get_knob("potrf_bw", 0) == 2
+    with pytest.raises(RuntimeError):
+        at.set_knob("potrf_bw", 0, 3)          # 2 or 4 tile columns per block panel
     at.reset()
+    assert at.get_knob("potrf_bw", 0) == 0
     assert at.get_knob("ric_per_cu", 3) == 0 and at.get_knob("ric_threads", 2) == 0
     # footprints of the resident form: largest bench stage (nu+nx = 80, nx = 64)
     # fits two per CU, nu+nx = 64, nx = 48 three per CU

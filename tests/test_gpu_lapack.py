@@ -88,12 +88,23 @@ def test_potrf(gpu, prec, ps, op):
     assert (gd["use_dA"] == rd["use_dA"]).all()
 
 
+@pytest.mark.parametrize("bw", [4, 2])
 @pytest.mark.parametrize("prec", ["d", "s"])
 @pytest.mark.parametrize("ps", [4, 8])
-def test_potrf_tiles(gpu, prec, ps):
-    """potrf_l with a size bound in (128, 256]: 64-column block panels factored
-    in LDS as 16x16 tiles, earlier block columns applied from D; sizes around
-    the tile and panel edges, offsets, in place, a non-SPD pivot."""
+def test_potrf_tiles(gpu, prec, ps, bw):
+    """potrf_l with a size bound in (128, 256]: block panels of 16 bw columns
+    (the tuning knob potrf_bw: 64 columns, one instance per CU, or 32, two)
+    factored in LDS as 16x16 tiles, earlier block columns applied from D;
+    sizes around the tile and panel edges, offsets, in place, a non-SPD pivot."""
+    from blasfeo_amd import autotune
+    autotune.set_knob("potrf_bw", 0, bw)
+    try:
+        _potrf_tiles_case(prec, ps)
+    finally:
+        autotune.set_knob("potrf_bw", 0, 0)
+
+
+def _potrf_tiles_case(prec, ps):
     rng = np.random.default_rng(370 + ps)
     b = Batch(prec, ps, seed=9)
     recs = []

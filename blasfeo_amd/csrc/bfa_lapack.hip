@@ -168,8 +168,9 @@ __global__ __launch_bounds__(NT) void k_syrk_potrf_lds(const typename SPArg<T>::
 // potrf_l for 128 < n <= 256: left-looking over block panels of 16 BW columns
 // held in LDS as 16x16 tiles (col-major inside a tile, tile (r, jj) of the
 // panel at (BW r + jj) * 256), one workgroup of NTH threads per instance: BW =
-// 4 (128 KB, one instance per CU, 8 waves) or BW = 2 (64 KB, two instances per
-// CU, 4 waves each, whose serial chains overlap).
+// 4 (128 KB, one instance per CU, 8 waves), 2 (64 KB, two per CU, 4 waves
+// each) or 1 (32 KB, four per CU): more resident instances overlap their
+// serial chains, at the cost of reading the earlier columns more often.
 //   1. the panel's lower part is staged from C (identity beyond n);
 //   2. the earlier block columns are applied on the matrix cores,
 //      P(r, jj) -= L(rows r, 0:c0) L(rows jj, 0:c0)^T, with L read from D
@@ -808,12 +809,24 @@ static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args
 		return set_error(BLASFEO_AMD_ETOOBIG, "%s: m up to 128 supported (the LDS-resident form); got a bound of %d", name, mmax);
 	if (MODE == 0 && mmax <= PT_NMAX)
 	{
-		// block panels of 32 columns (two 4-wave instances per CU, whose
-		// chains overlap: the potrf + trsm solve sequence for 2048 x n = 256
-		// takes 1.65 instead of 1.90 ms on MI355X) or of 64 (one 8-wave
-		// instance per CU); the tuning knob potrf_bw overrides
-		const int bw = tune_value(TUNE_POTRF_BW, 0) == 4 ? 4 : 2;
-		if (bw == 4)
+		// block panels of 16 columns (four 4-wave instances per CU, whose
+		// chains overlap), 32 (two per CU) or 64 (one 8-wave instance per
+		// CU): the potrf + trsm solve sequence for 2048 x n = 256 takes 1.31,
+		// 1.65 and 1.90 ms on MI355X.  16 by default, the tuning knob
+		// potrf_bw overrides
+		const int knob = tune_value(TUNE_POTRF_BW, 0);
+		const int bw = knob == 4 ? 4 : (knob == 2 ? 2 : 1);
+		if (bw == 1)
+		{
+			const size_t shm3 = potrf_tiles_lds_bytes<T, 1>();
+			if (ps == 4)
+				hipLaunchKernelGGL((k_potrf_tiles<T, 4, 1, 256>), dim3(batch), dim3(256), shm3, st, args, mmax);
+			else if (ps == 8)
+				hipLaunchKernelGGL((k_potrf_tiles<T, 8, 1, 256>), dim3(batch), dim3(256), shm3, st, args, mmax);
+			else
+				hipLaunchKernelGGL((k_potrf_tiles<T, 16, 1, 256>), dim3(batch), dim3(256), shm3, st, args, mmax);
+		}
+		else if (bw == 4)
 		{
 			const size_t shm3 = potrf_tiles_lds_bytes<T, 4>();
 			if (ps == 4)

@@ -38,7 +38,7 @@ bool valid(const char *name, int index, int value)
 		return value >= 1 && value <= 64;
 	// potrf_bw: tile columns per block panel of potrf_l for 128 < n <= 256
 	if (!std::strcmp(name, "potrf_bw"))
-		return value == 2 || value == 4;
+		return value == 1 || value == 2 || value == 4;
 	// ric_threads: one wave only for the classes whose stages have nu+nx <= 64
 	return value == 128 || value == 256 || (value == 64 && index <= 4);
 }

@@ -176,7 +176,7 @@ def test_tuning_knobs_without_gpu():
     at.set_knob("potrf_bw", 0, 2)
     assert at.get_knob("potrf_bw", 0) == 2
     with pytest.raises(RuntimeError):
-        at.set_knob("potrf_bw", 0, 3)          # 2 or 4 tile columns per block panel
+        at.set_knob("potrf_bw", 0, 3)          # 1, 2 or 4 tile columns per block panel
     at.reset()
     assert at.get_knob("potrf_bw", 0) == 0
     assert at.get_knob("ric_per_cu", 3) == 0 and at.get_knob("ric_threads", 2) == 0

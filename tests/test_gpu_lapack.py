@@ -88,13 +88,14 @@ def test_potrf(gpu, prec, ps, op):
     assert (gd["use_dA"] == rd["use_dA"]).all()
 
 
-@pytest.mark.parametrize("bw", [4, 2])
+@pytest.mark.parametrize("bw", [4, 2, 1])
 @pytest.mark.parametrize("prec", ["d", "s"])
 @pytest.mark.parametrize("ps", [4, 8])
 def test_potrf_tiles(gpu, prec, ps, bw):
     """potrf_l with a size bound in (128, 256]: block panels of 16 bw columns
-    (the tuning knob potrf_bw: 64 columns, one instance per CU, or 32, two)
-    factored in LDS as 16x16 tiles, earlier block columns applied from D;
+    (the tuning knob potrf_bw: 64, 32 or 16 columns, one, two or four
+    instances per CU) factored in LDS as 16x16 tiles, earlier block columns
+    applied from D;
     sizes around the tile and panel edges, offsets, in place, a non-SPD pivot."""
     from blasfeo_amd import autotune
     autotune.set_knob("potrf_bw", 0, bw)

@@ -127,7 +127,10 @@ const char *blasfeo_amd_last_error(void);
  *                  of the class's persistent launch (1..64);
  *   "ric_threads", index = class 0..7: threads per instance (64, 128 or 256;
  *                  64 only for classes 0..4);
- *   "gemm_grid_y", index 0: most workgroups per gemm/syrk/trmm instance (1..64).
+ *   "gemm_grid_y", index 0: most workgroups per gemm/syrk/trmm instance (1..64);
+ *   "potrf_bw",    index 0: tile columns per block panel of potrf_l for
+ *                  128 < n <= 256 (1, 2 or 4: 16-, 32- or 64-column panels,
+ *                  four, two or one instance per CU; built-in 1).
  * blasfeo_amd_tune_set returns BLASFEO_AMD_OK or BLASFEO_AMD_EINVAL;
  * blasfeo_amd_tune_get returns the value (0 = built-in) or BLASFEO_AMD_EINVAL. */
 int blasfeo_amd_tune_set(const char *name, int index, int value);

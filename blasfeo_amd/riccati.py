@@ -10,10 +10,13 @@ them, as the reference example does (:737-747):
     RSQ  = [R S; S^T Q] (nu+nx) x (nu+nx)      (lower triangle used)
     QN   = Q            nx x nx                 (last stage)
 Outputs: L[0..N-1] (nu+nx) x (nu+nx), L[N] nx x nx, plus one work matrix.
+
+``RiccatiSolve`` adds the vectors of the solve on those factors
+(d_back_ric_trs_libstr, :203-267) for ``blasfeo_amd_d_back_ric_trs_batched``.
 """
 import numpy as np
 
-from . import DMAT, Layout, RICCATI_ARG
+from . import DMAT, DVEC, Layout, RICCATI_ARG, RICCATI_TRS_ARG
 
 
 class RiccatiBatch:
@@ -103,6 +106,94 @@ class RiccatiBatch:
         return a
 
 
+class RiccatiSolve:
+    """The vectors of the Riccati solve for every instance of a RiccatiBatch,
+    packed into one byte buffer (each vector on a 64-byte boundary, its size
+    rounded as blasfeo_memsize_dvec does, auxiliary/d_aux_lib4.c).
+
+    Per instance the descriptors are stored in this order:
+        b[0..N-1] (nx), rq[0..N] (nu+nx; nx on stage N), Pb[0..N-1] (nx),
+        ux[0..N] (nu+nx; nx on stage N), pi[0..N-1] (nx), work (nx)
+    so that each kind is a contiguous descriptor array, as the solve's
+    argument record expects.  b and rq are inputs, the rest outputs.
+    """
+
+    KINDS = ("b", "rq", "Pb", "ux", "pi", "work")
+
+    def __init__(self, rb):
+        self.rb = rb
+        N, ps = rb.N, rb.ps
+        self.per = 5 * N + 3
+        self.start = {"b": 0, "rq": N, "Pb": 2 * N + 1, "ux": 3 * N + 1, "pi": 4 * N + 2,
+                      "work": 5 * N + 2}
+        nx = rb.nxs[:, None]
+        nk = (rb.nxs + rb.nus)[:, None]
+        sz = np.zeros((rb.batch, self.per), dtype=np.int64)
+        st = self.start
+        sz[:, st["b"]:st["b"] + N] = nx
+        sz[:, st["rq"]:st["rq"] + N] = nk
+        sz[:, st["rq"] + N] = rb.nxs
+        sz[:, st["Pb"]:st["Pb"] + N] = nx
+        sz[:, st["ux"]:st["ux"] + N] = nk
+        sz[:, st["ux"] + N] = rb.nxs
+        sz[:, st["pi"]:st["pi"] + N] = nx
+        sz[:, st["work"]] = rb.nxs
+        self.sizes = sz.reshape(-1)
+        self.pm = -(-self.sizes // ps) * ps
+        self.memsize = -(-(self.pm * 8) // 64) * 64
+        self.offs = np.concatenate([[0], np.cumsum(self.memsize)[:-1]]).astype(np.int64)
+        self.nbytes = int(self.offs[-1] + self.memsize[-1]) if len(self.offs) else 0
+
+    def index(self, i, kind, k=0):
+        return i * self.per + self.start[kind] + k
+
+    def table(self, base):
+        d = np.zeros(len(self.sizes), dtype=DVEC)
+        addr = np.uint64(int(base)) + self.offs.astype(np.uint64)
+        d["mem"] = addr
+        d["pa"] = addr
+        d["m"] = self.sizes
+        d["pm"] = self.pm
+        d["memsize"] = self.memsize
+        return d
+
+    def fill_inputs(self, rng):
+        """b ~ N(0,1) and rq ~ N(0,1) on every stage (host buffer)."""
+        buf = np.zeros(self.nbytes, dtype=np.uint8)
+        f = buf.view(np.float64)
+        sel = np.zeros((self.rb.batch, self.per), dtype=bool)
+        N = self.rb.N
+        sel[:, self.start["b"]:self.start["b"] + N] = True
+        sel[:, self.start["rq"]:self.start["rq"] + N + 1] = True
+        sel = sel.reshape(-1)
+        m = self.sizes[sel]
+        first = self.offs[sel] // 8
+        idx = np.repeat(first - np.concatenate([[0], np.cumsum(m)[:-1]]), m) + np.arange(int(m.sum()))
+        f[idx] = rng.standard_normal(len(idx))
+        return buf
+
+    def vec(self, buf, i, kind, k=0):
+        j = self.index(i, kind, k)
+        o = int(self.offs[j])
+        return np.asarray(buf)[o: o + int(self.sizes[j]) * 8].view(np.float64).copy()
+
+    def args(self, tab, addr_babt, addr_L, addr_nx, addr_nu):
+        """Argument records for descriptor table address ``tab`` and the
+        factorisation's BAbt / L / nx / nu arrays (RiccatiBatch.tables)."""
+        rb, N = self.rb, self.rb.N
+        a = np.zeros(rb.batch, dtype=RICCATI_TRS_ARG)
+        idx = np.arange(rb.batch, dtype=np.uint64)
+        a["N"] = N
+        a["nx"] = addr_nx + idx * 4 * (N + 1)
+        a["nu"] = addr_nu + idx * 4 * (N + 1)
+        a["BAbt"] = addr_babt + idx * DMAT.itemsize * max(N, 1)
+        a["L"] = addr_L + idx * DMAT.itemsize * (N + 1)
+        base = np.uint64(int(tab)) + idx * np.uint64(DVEC.itemsize * self.per)
+        for kind in self.KINDS:
+            a[kind] = base + np.uint64(DVEC.itemsize * self.start[kind])
+        return a
+
+
 class DeviceRiccati:
     """A RiccatiBatch resident in GPU memory, ready to launch."""
 
@@ -137,6 +228,30 @@ class DeviceRiccati:
     def L(self, i, k):
         buf = self.outbuf.cpu().numpy()
         return self.rb.out.get(buf, self.rb.i_L[i][k])
+
+
+class DeviceRiccatiSolve:
+    """The solve vectors of a RiccatiSolve in GPU memory, on the factors of a
+    DeviceRiccati (whose sweep must have run first on the same stream)."""
+
+    def __init__(self, dev, rs, host_buf):
+        import torch
+        from .device import to_device
+        self.dev, self.rs = dev, rs
+        self.buf = torch.from_numpy(np.ascontiguousarray(host_buf)).to("cuda")
+        self.tab = to_device(rs.table(self.buf.data_ptr()))
+        babt, rsq, L, work, nx, nu = dev.t_dev
+        self.args = to_device(rs.args(self.tab.data_ptr(), babt.data_ptr(), L.data_ptr(),
+                                      nx.data_ptr(), nu.data_ptr()))
+
+    def run(self, stream=0):
+        from . import call
+        rb = self.rs.rb
+        call("d_back_ric_trs", rb.ps, rb.batch, self.args.data_ptr(),
+             (rb.max_nk, rb.max_nx, 0), stream)
+
+    def host(self):
+        return self.buf.cpu().numpy()
 
 
 class HostRiccati:

@@ -342,6 +342,39 @@ struct blasfeo_amd_riccati_arg
  * examples/example_d_riccati_recursion.c:120-138). */
 int blasfeo_amd_d_back_ric_trf_batched(int ps, int batch, const struct blasfeo_amd_riccati_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 
+/* One instance of d_back_ric_trs_libstr(N, nx, nu, hsBAbt, hsb, hsrq, hsL, hsPb,
+ * hsux, hspi, hswork_vec) (examples/example_d_riccati_recursion.c:203-267):
+ * the backward and forward substitutions on the factors L[0..N] of
+ * blasfeo_amd_d_back_ric_trf_batched.  Device arrays of descriptors:
+ * BAbt[N] as above, b[N] (nx[k+1]), rq[N+1] (nu[k]+nx[k]), L[N+1], and the
+ * outputs Pb[N] (nx[k+1]), ux[N+1] (nu[k]+nx[k]) and pi[N] (nx[k+1]).
+ * L[k]->dA / use_dA follow blasfeo_dtrsv_lnn(_mn) and blasfeo_dtrsv_ltn(_mn)
+ * (blasfeo_hp_pm/d_blas2_lib4.c:1275-1330): the cached reciprocal diagonal is
+ * used when use_dA == 1, else it is formed, stored and use_dA set to 1.
+ * work (one vector of max nx) is accepted for the reference's argument list
+ * and not written: the kernel keeps that vector on chip. */
+struct blasfeo_amd_riccati_trs_arg
+	{
+	int N;
+	const int *nx;
+	const int *nu;
+	struct blasfeo_dmat *BAbt;
+	struct blasfeo_dvec *b;
+	struct blasfeo_dvec *rq;
+	struct blasfeo_dmat *L;
+	struct blasfeo_dvec *Pb;
+	struct blasfeo_dvec *ux;
+	struct blasfeo_dvec *pi;
+	struct blasfeo_dvec *work;
+	};
+
+/* Batched Riccati solve, one wavefront per instance, both sweeps in one launch
+ * (the trmv_ltn / trmv_lnn / gemv_n / trsv_lnn_mn backward chain and the
+ * trsv_ltn_mn / gemv_t / trmv forward chain of
+ * examples/example_d_riccati_recursion.c:211-264).  hint->max_m bounds
+ * nu[k]+nx[k] and hint->max_n bounds nx[k], both <= 128. */
+int blasfeo_amd_d_back_ric_trs_batched(int ps, int batch, const struct blasfeo_amd_riccati_trs_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Cholesky solve A X = B with A symmetric positive definite                */
 /* ------------------------------------------------------------------------ */

@@ -26,6 +26,10 @@ _SIGS = {
     "potrf_l": "ipiipii", "potrf_l_mn": "iipiipii",
     "syrk_potrf_ln": "iipiipiipiipii", "syrk_potrf_ln_mn": "iiipiipiipiipii",
     "getrf_rp": "iipiipiip", "ric_trf": "ippppp",
+    "gemv_n": "iiRpiipiRpipi", "gemv_t": "iiRpiipiRpipi",
+    "trmv_lnn": "ipiipipi", "trmv_ltn": "ipiipipi",
+    "trsv_lnn_mn": "iipiipipi", "trsv_ltn_mn": "iipiipipi",
+    "ric_trs": "ippppppppppp",
 }
 
 
@@ -46,7 +50,7 @@ def lib():
                 f.argtypes = [_I] + [{"i": _I, "p": _V, "R": real}[c] for c in sig]
                 f.restype = None
         for name in ("dgemm_nt", "dgemm_nn", "dsyrk_ln", "dpotrf_l", "dgetrf_rp",
-                     "sgetrf_rp", "dric_trf", "dposv"):
+                     "sgetrf_rp", "dric_trf", "dric_trs", "dposv"):
             f = getattr(L, "or_%s_batch" % name)
             f.argtypes = [_I, _I, _V]
             f.restype = None

@@ -27,21 +27,25 @@
 
 #define REAL double
 #define MAT struct blasfeo_dmat
+#define VEC struct blasfeo_dvec
 #define PFX(name) or_d##name
 #define SQRT sqrt
 #include "oracle_impl.h"
 #undef REAL
 #undef MAT
+#undef VEC
 #undef PFX
 #undef SQRT
 
 #define REAL float
 #define MAT struct blasfeo_smat
+#define VEC struct blasfeo_svec
 #define PFX(name) or_s##name
 #define SQRT sqrtf
 #include "oracle_impl.h"
 #undef REAL
 #undef MAT
+#undef VEC
 #undef PFX
 #undef SQRT
 
@@ -96,6 +100,13 @@ void or_dric_trf_batch(int ps, int batch, const struct blasfeo_amd_riccati_arg *
 #pragma omp parallel for schedule(dynamic, 4)
 	for(int i = 0; i < batch; i++)
 		or_dric_trf(ps, a[i].N, a[i].nx, a[i].nu, a[i].BAbt, a[i].RSQrq, a[i].L, a[i].work);
+	}
+
+void or_dric_trs_batch(int ps, int batch, const struct blasfeo_amd_riccati_trs_arg *a)
+	{
+#pragma omp parallel for schedule(dynamic, 4)
+	for(int i = 0; i < batch; i++)
+		or_dric_trs(ps, a[i].N, a[i].nx, a[i].nu, a[i].BAbt, a[i].b, a[i].rq, a[i].L, a[i].Pb, a[i].ux, a[i].pi, a[i].work);
 	}
 
 /* The FP64 Cholesky solve a reference caller writes:

@@ -332,4 +332,326 @@ void PFX(ric_trf)(int ps, int N, const int *nx, const int *nu, MAT *BAbt, MAT *R
 		}
 	}
 
+/* ---- level 2 (blasfeo_ref/x_blas2_ref.c), used by the Riccati solve ----
+ * Rows (gemv_n, trmv, trsv) or columns (gemv_t) are taken in pairs; inside a
+ * pair each partial sum adds two products at a time (a*x0 + b*x1, rounded,
+ * then accumulated), and an odd last row or column accumulates one product
+ * at a time, as the reference's loops do. */
+
+/* z <= beta*y + alpha*A*x, A m x n; blasfeo_ref/x_blas2_ref.c:39-98 */
+void PFX(gemv_n)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, VEC *sx, int xi, REAL beta, VEC *sy, int yi, VEC *sz, int zi)
+	{
+	REAL *x = sx->pa + xi, *y = sy->pa + yi, *z = sz->pa + zi;
+	for(int i = 0; i < m; i++)
+		z[i] = beta == 0 ? (REAL)0.0 : beta * y[i];
+	int i = 0;
+	for(; i < m-1; i += 2)
+		{
+		REAL y0 = 0, y1 = 0;
+		int j = 0;
+		for(; j < n-1; j += 2)
+			{
+			y0 += EL(sA, ai+i, aj+j) * x[j] + EL(sA, ai+i, aj+j+1) * x[j+1];
+			y1 += EL(sA, ai+i+1, aj+j) * x[j] + EL(sA, ai+i+1, aj+j+1) * x[j+1];
+			}
+		if(j < n)
+			{
+			y0 += EL(sA, ai+i, aj+j) * x[j];
+			y1 += EL(sA, ai+i+1, aj+j) * x[j];
+			}
+		z[i] += alpha * y0;
+		z[i+1] += alpha * y1;
+		}
+	for(; i < m; i++)
+		{
+		REAL y0 = 0;
+		for(int j = 0; j < n; j++)
+			y0 += EL(sA, ai+i, aj+j) * x[j];
+		z[i] += alpha * y0;
+		}
+	}
+
+/* z <= beta*y + alpha*A^T*x, A m x n; blasfeo_ref/x_blas2_ref.c:146-205 */
+void PFX(gemv_t)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, VEC *sx, int xi, REAL beta, VEC *sy, int yi, VEC *sz, int zi)
+	{
+	REAL *x = sx->pa + xi, *y = sy->pa + yi, *z = sz->pa + zi;
+	for(int j = 0; j < n; j++)
+		z[j] = beta == 0 ? (REAL)0.0 : beta * y[j];
+	int j = 0;
+	for(; j < n-1; j += 2)
+		{
+		REAL y0 = 0, y1 = 0;
+		int i = 0;
+		for(; i < m-1; i += 2)
+			{
+			y0 += EL(sA, ai+i, aj+j) * x[i] + EL(sA, ai+i+1, aj+j) * x[i+1];
+			y1 += EL(sA, ai+i, aj+j+1) * x[i] + EL(sA, ai+i+1, aj+j+1) * x[i+1];
+			}
+		if(i < m)
+			{
+			y0 += EL(sA, ai+i, aj+j) * x[i];
+			y1 += EL(sA, ai+i, aj+j+1) * x[i];
+			}
+		z[j] += alpha * y0;
+		z[j+1] += alpha * y1;
+		}
+	for(; j < n; j++)
+		{
+		REAL y0 = 0;
+		for(int i = 0; i < m; i++)
+			y0 += EL(sA, ai+i, aj+j) * x[i];
+		z[j] += alpha * y0;
+		}
+	}
+
+/* z <= A*x, A lower m x m, non-unit; bottom-up so z may alias x;
+ * blasfeo_ref/x_blas2_ref.c:470-530 (the _mn form with n = m) */
+void PFX(trmv_lnn)(int ps, int m, MAT *sA, int ai, int aj, VEC *sx, int xi, VEC *sz, int zi)
+	{
+	REAL *x = sx->pa + xi, *z = sz->pa + zi;
+	int n = m;
+	if(n % 2 != 0)
+		{
+		int i = n-1;
+		REAL y0 = x[i];
+		y0 *= EL(sA, ai+i, aj+i);
+		for(int j = 0; j < i; j++)
+			y0 += EL(sA, ai+i, aj+j) * x[j];
+		z[i] = y0;
+		n -= 1;
+		}
+	for(int i = n-2; i >= 0; i -= 2)
+		{
+		REAL y0 = x[i], y1 = x[i+1];
+		y1 *= EL(sA, ai+i+1, aj+i+1);
+		y1 += EL(sA, ai+i+1, aj+i) * y0;
+		y0 *= EL(sA, ai+i, aj+i);
+		for(int j = 0; j < i-1; j += 2)
+			{
+			y0 += EL(sA, ai+i, aj+j) * x[j] + EL(sA, ai+i, aj+j+1) * x[j+1];
+			y1 += EL(sA, ai+i+1, aj+j) * x[j] + EL(sA, ai+i+1, aj+j+1) * x[j+1];
+			}
+		z[i] = y0;
+		z[i+1] = y1;
+		}
+	}
+
+/* z <= A^T*x, A lower m x m, non-unit; top-down so z may alias x;
+ * blasfeo_ref/x_blas2_ref.c:535-590 (the _mn form with n = m) */
+void PFX(trmv_ltn)(int ps, int m, MAT *sA, int ai, int aj, VEC *sx, int xi, VEC *sz, int zi)
+	{
+	REAL *x = sx->pa + xi, *z = sz->pa + zi;
+	int j = 0;
+	for(; j < m-1; j += 2)
+		{
+		REAL y0 = x[j], y1 = x[j+1];
+		y0 *= EL(sA, ai+j, aj+j);
+		y0 += EL(sA, ai+j+1, aj+j) * y1;
+		y1 *= EL(sA, ai+j+1, aj+j+1);
+		int i = j+2;
+		for(; i < m-1; i += 2)
+			{
+			y0 += EL(sA, ai+i, aj+j) * x[i] + EL(sA, ai+i+1, aj+j) * x[i+1];
+			y1 += EL(sA, ai+i, aj+j+1) * x[i] + EL(sA, ai+i+1, aj+j+1) * x[i+1];
+			}
+		for(; i < m; i++)
+			{
+			y0 += EL(sA, ai+i, aj+j) * x[i];
+			y1 += EL(sA, ai+i, aj+j+1) * x[i];
+			}
+		z[j] = y0;
+		z[j+1] = y1;
+		}
+	for(; j < m; j++)
+		{
+		REAL y0 = x[j];
+		y0 *= EL(sA, ai+j, aj+j);
+		for(int i = j+1; i < m; i++)
+			y0 += EL(sA, ai+i, aj+j) * x[i];
+		z[j] = y0;
+		}
+	}
+
+/* Reciprocal diagonal used by trsv: the cache when ai = aj = 0 and use_dA = 1,
+ * else formed as 1/a_jj (stored; use_dA = 1 at offset 0, 0 elsewhere);
+ * blasfeo_ref/x_blas2_ref.c:773-787, blasfeo_hp_pm/d_blas2_lib4.c:1300-1316 */
+static REAL *PFX(trsv_diag)(int ps, int n, MAT *sA, int ai, int aj)
+	{
+	REAL *dA = sA->dA;
+	if(ai == 0 && aj == 0)
+		{
+		if(sA->use_dA != 1)
+			{
+			for(int i = 0; i < n; i++)
+				dA[i] = (REAL)1.0 / EL(sA, ai+i, aj+i);
+			sA->use_dA = 1;
+			}
+		}
+	else
+		{
+		for(int i = 0; i < n; i++)
+			dA[i] = (REAL)1.0 / EL(sA, ai+i, aj+i);
+		sA->use_dA = 0;
+		}
+	return dA;
+	}
+
+/* z <= solution of A(0:n,0:n) z = x(0:n), then z(n:m) = x(n:m) - A(n:m,0:n) z(0:n);
+ * A lower m x n, non-unit; blasfeo_ref/x_blas2_ref.c:734-840 (trsv_lnn is n = m,
+ * :990-1050, the same loops) */
+void PFX(trsv_lnn_mn)(int ps, int m, int n, MAT *sA, int ai, int aj, VEC *sx, int xi, VEC *sz, int zi)
+	{
+	if(m == 0 || n == 0)
+		return;
+	REAL *dA = PFX(trsv_diag)(ps, n, sA, ai, aj);
+	REAL *x = sx->pa + xi, *z = sz->pa + zi;
+	int i = 0;
+	for(; i < n-1; i += 2)
+		{
+		REAL y0 = x[i], y1 = x[i+1];
+		int j = 0;
+		for(; j < i-1; j += 2)
+			{
+			y0 -= EL(sA, ai+i, aj+j) * z[j] + EL(sA, ai+i, aj+j+1) * z[j+1];
+			y1 -= EL(sA, ai+i+1, aj+j) * z[j] + EL(sA, ai+i+1, aj+j+1) * z[j+1];
+			}
+		y0 *= dA[i];
+		y1 -= EL(sA, ai+i+1, aj+j) * y0;
+		y1 *= dA[i+1];
+		z[i] = y0;
+		z[i+1] = y1;
+		}
+	for(; i < n; i++)
+		{
+		REAL y0 = x[i];
+		for(int j = 0; j < i; j++)
+			y0 -= EL(sA, ai+i, aj+j) * z[j];
+		y0 *= dA[i];
+		z[i] = y0;
+		}
+	for(; i < m-1; i += 2)
+		{
+		REAL y0 = x[i], y1 = x[i+1];
+		int j = 0;
+		for(; j < n-1; j += 2)
+			{
+			y0 -= EL(sA, ai+i, aj+j) * z[j] + EL(sA, ai+i, aj+j+1) * z[j+1];
+			y1 -= EL(sA, ai+i+1, aj+j) * z[j] + EL(sA, ai+i+1, aj+j+1) * z[j+1];
+			}
+		if(j < n)
+			{
+			y0 -= EL(sA, ai+i, aj+j) * z[j];
+			y1 -= EL(sA, ai+i+1, aj+j) * z[j];
+			}
+		z[i] = y0;
+		z[i+1] = y1;
+		}
+	for(; i < m; i++)
+		{
+		REAL y0 = x[i];
+		for(int j = 0; j < n; j++)
+			y0 -= EL(sA, ai+i, aj+j) * z[j];
+		z[i] = y0;
+		}
+	}
+
+/* z(0:n) <= solution of A(0:n,0:n)^T z(0:n) = x(0:n) - A(n:m,0:n)^T x(n:m),
+ * z(n:m) = x(n:m); A lower m x n, non-unit; blasfeo_ref/x_blas2_ref.c:888-985
+ * (trsv_ltn is n = m, :1115-1175, the same arithmetic) */
+void PFX(trsv_ltn_mn)(int ps, int m, int n, MAT *sA, int ai, int aj, VEC *sx, int xi, VEC *sz, int zi)
+	{
+	if(m == 0)
+		return;
+	REAL *dA = PFX(trsv_diag)(ps, n, sA, ai, aj);
+	REAL *x = sx->pa + xi, *z = sz->pa + zi;
+	if(x != z)
+		for(int i = 0; i < m; i++)
+			z[i] = x[i];
+	int j;
+	if(n % 2 != 0)
+		{
+		j = n-1;
+		REAL y0 = z[j];
+		for(int i = j+1; i < m; i++)
+			y0 -= EL(sA, ai+i, aj+j) * z[i];
+		y0 *= dA[j];
+		z[j] = y0;
+		j -= 2;
+		}
+	else
+		j = n-2;
+	for(; j >= 0; j -= 2)
+		{
+		REAL y0 = z[j], y1 = z[j+1];
+		int i = j+2;
+		for(; i < m-1; i += 2)
+			{
+			y0 -= EL(sA, ai+i, aj+j) * z[i] + EL(sA, ai+i+1, aj+j) * z[i+1];
+			y1 -= EL(sA, ai+i, aj+j+1) * z[i] + EL(sA, ai+i+1, aj+j+1) * z[i+1];
+			}
+		if(i < m)
+			{
+			y0 -= EL(sA, ai+i, aj+j) * z[i];
+			y1 -= EL(sA, ai+i, aj+j+1) * z[i];
+			}
+		y1 *= dA[j+1];
+		y0 -= EL(sA, ai+j+1, aj+j) * y1;
+		y0 *= dA[j];
+		z[j] = y0;
+		z[j+1] = y1;
+		}
+	}
+
+/* blasfeo_dveccp, blasfeo_daxpy (z = alpha*x + y), blasfeo_dvecsc;
+ * blasfeo_ref/x_blas1_ref.c */
+static void PFX(veccp)(int m, VEC *sx, int xi, VEC *sz, int zi)
+	{
+	for(int i = 0; i < m; i++)
+		sz->pa[zi+i] = sx->pa[xi+i];
+	}
+static void PFX(axpy)(int m, REAL alpha, VEC *sx, int xi, VEC *sy, int yi, VEC *sz, int zi)
+	{
+	for(int i = 0; i < m; i++)
+		sz->pa[zi+i] = sy->pa[yi+i] + alpha * sx->pa[xi+i];
+	}
+static void PFX(vecsc)(int m, REAL alpha, VEC *sx, int xi)
+	{
+	for(int i = 0; i < m; i++)
+		sx->pa[xi+i] *= alpha;
+	}
+
+/* Riccati solve d_back_ric_trs_libstr, examples/example_d_riccati_recursion.c:203-267:
+ * backward substitution (Pb, ux), then forward substitution (ux, pi).  The
+ * stage-0 steps of the reference's loops (full trsv on L[0], all of ux[0]
+ * negated) are the k = 0 cases here. */
+void PFX(ric_trs)(int ps, int N, const int *nx, const int *nu, MAT *BAbt, VEC *b, VEC *rq, MAT *L, VEC *Pb, VEC *ux, VEC *pi, VEC *work)
+	{
+	if(N < 0)
+		return;
+	PFX(veccp)(nu[N]+nx[N], &rq[N], 0, &ux[N], 0);
+	for(int k = N-1; k >= 0; k--)
+		{
+		int nk = nu[k] + nx[k];
+		PFX(trmv_ltn)(ps, nx[k+1], &L[k+1], nu[k+1], nu[k+1], &b[k], 0, &Pb[k], 0);
+		PFX(trmv_lnn)(ps, nx[k+1], &L[k+1], nu[k+1], nu[k+1], &Pb[k], 0, &Pb[k], 0);
+		PFX(veccp)(nk, &rq[k], 0, &ux[k], 0);
+		PFX(veccp)(nx[k+1], &Pb[k], 0, work, 0);
+		PFX(axpy)(nx[k+1], (REAL)1.0, &ux[k+1], nu[k+1], work, 0, work, 0);
+		PFX(gemv_n)(ps, nk, nx[k+1], (REAL)1.0, &BAbt[k], 0, 0, work, 0, (REAL)1.0, &ux[k], 0, &ux[k], 0);
+		PFX(trsv_lnn_mn)(ps, nk, k == 0 ? nk : nu[k], &L[k], 0, 0, &ux[k], 0, &ux[k], 0);
+		}
+	for(int k = 0; k < N; k++)
+		{
+		int nk = nu[k] + nx[k];
+		PFX(veccp)(nx[k+1], &ux[k+1], nu[k+1], &pi[k], 0);
+		PFX(vecsc)(k == 0 ? nk : nu[k], (REAL)-1.0, &ux[k], 0);
+		PFX(trsv_ltn_mn)(ps, nk, k == 0 ? nk : nu[k], &L[k], 0, 0, &ux[k], 0, &ux[k], 0);
+		PFX(gemv_t)(ps, nk, nx[k+1], (REAL)1.0, &BAbt[k], 0, 0, &ux[k], 0, (REAL)1.0, &b[k], 0, &ux[k+1], nu[k+1]);
+		PFX(veccp)(nx[k+1], &ux[k+1], nu[k+1], work, 0);
+		PFX(trmv_ltn)(ps, nx[k+1], &L[k+1], nu[k+1], nu[k+1], work, 0, work, 0);
+		PFX(trmv_lnn)(ps, nx[k+1], &L[k+1], nu[k+1], nu[k+1], work, 0, work, 0);
+		PFX(axpy)(nx[k+1], (REAL)1.0, work, 0, &pi[k], 0, &pi[k], 0);
+		}
+	}
+
 #undef EL

@@ -153,7 +153,7 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="riccati: time replays of the batched sweep captured in a HIP graph")
     ap.add_argument("--workload", default="riccati",
-                    choices=("riccati", "potrf32", "gemm_syrk", "getrf", "sgetrf", "posv256"),
+                    choices=("riccati", "ric_trs", "potrf32", "gemm_syrk", "getrf", "sgetrf", "posv256"),
                     help="riccati (default, the headline); the others measure the remaining "
                          "batched configurations")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -184,6 +184,8 @@ def main():
 
     if args.workload == "posv256":
         return run_posv(args, torch, bfa)
+    if args.workload == "ric_trs":
+        return run_ric_trs(args, torch, bfa)
     if args.workload != "riccati":
         return run_other(args, torch, bfa)
     return run_riccati(args, torch, bfa)
@@ -281,6 +283,149 @@ def run_riccati(args, torch, bfa):
     }
     print(json.dumps(out), flush=True)
 
+
+
+# ---------------------------------------------------------------------------
+# the Riccati solve on the factors of the same workload
+# ---------------------------------------------------------------------------
+def riccati_trs_bytes(nxs, nus, N):
+    """Algorithmic HBM bytes of one solve launch, 8 bytes per double.  Each of
+    the two sweeps reads the parts of the factors it uses: the trailing
+    nx x nx lower block of L[k+1] for every stage, the first nu columns
+    (lower part) of L[k] on stages 1..N-1 and all of L[0]; BAbt once per
+    instance (time-invariant, the stages re-read it from cache); plus its
+    vectors: backward rq/ux of stage N, and per stage b, rq, Pb, ux;
+    forward per stage ux[k] in and out, the state part of ux[k+1] in and
+    out, b and pi."""
+    nx = np.asarray(nxs, dtype=np.float64)
+    nu = np.asarray(nus, dtype=np.float64)
+    nk = nx + nu
+    mats = N * nx * (nx + 1) / 2 + (N - 1) * (nu * nk - nu * (nu - 1) / 2) + nk * (nk + 1) / 2 + nk * nx
+    vecs = 2 * nx + N * (2 * nx + 2 * nk) + N * (4 * nx + 2 * nk)
+    return float(8 * (2 * mats + vecs).sum())
+
+
+def _trs_subset(rb, inp, gbuf, Ltab, rs, sin, n):
+    """The first n instances in host memory with the GPU's factors (bytes and
+    use_dA flags) and the same solve inputs, ready for the oracle's solve."""
+    from blasfeo_amd.riccati import RiccatiBatch, RiccatiSolve, HostRiccati
+    sub = RiccatiBatch(rb.nxs[:n], rb.nus[:n], rb.N, rb.ps)
+    buf = sub.inp.host_buffer()
+    for i in range(n):
+        for a, b in ((sub.i_babt[i], rb.i_babt[i]), (sub.i_rsq[i], rb.i_rsq[i]), (sub.i_qn[i], rb.i_qn[i])):
+            sub.inp.view(buf, a)[:] = rb.inp.view(inp, b)
+    h = HostRiccati(sub, buf)
+    for i in range(n):
+        for k in range(rb.N + 1):
+            a, b = sub.i_L[i][k], rb.i_L[i][k]
+            sub.out.view(h.outbuf, a)[:] = rb.out.view(gbuf, b)
+            sub.out.dview(h.outbuf, a)[:] = rb.out.dview(gbuf, b)
+    h.t[2]["use_dA"] = Ltab["use_dA"][: n * (rb.N + 1)]
+    srs = RiccatiSolve(sub)
+    sbuf = np.zeros(srs.nbytes, dtype=np.uint8)
+    for i in range(n):
+        for kind, ks in (("b", range(rb.N)), ("rq", range(rb.N + 1))):
+            for k in ks:
+                j, jj = srs.index(i, kind, k), rs.index(i, kind, k)
+                o, oo, m = int(srs.offs[j]), int(rs.offs[jj]), int(srs.sizes[j]) * 8
+                sbuf[o:o + m] = sin[oo:oo + m]
+    tab = srs.table(sbuf.ctypes.data)
+    babt, rsq, L, work, nx, nu = h.t
+    argv = srs.args(tab.ctypes.data, babt.ctypes.data, L.ctypes.data, nx.ctypes.data, nu.ctypes.data)
+    return sub, srs, sbuf, (h, tab, argv)
+
+
+def trs_checks(rb, inp, rs, sin, gbuf, Ltab, gsol, n, nchk=64):
+    """Checks of a measured solve: the oracle's solve of the first n instances
+    on the GPU's factors, then for the first nchk instances the KKT backward
+    errors of both solutions and their largest difference (1e-11 rule).
+    Returns (checks, the oracle's argument records, objects to keep alive)."""
+    import oracle
+    from blasfeo_amd.riccati import trs_backward_error
+    sub, srs, sbuf, keep = _trs_subset(rb, inp, gbuf, Ltab, rs, sin, n)
+    argv = keep[2]
+    oracle.lib().or_dric_trs_batch(rb.ps, n, argv.ctypes.data)
+    # the recursion amplifies rounding over the horizon (the oracle's own
+    # KKT backward error reaches 1e5-1e6 eps on some instances at N = 20),
+    # so the full-size bar is accuracy: the GPU's backward error within
+    # 10x the oracle's (+100 eps) on the same instance and factors
+    nchk = min(nchk, n)
+    worst, gb, ob, ok = 0.0, [], [], True
+    for i in range(nchk):
+        for kind, ks in (("ux", range(rb.N + 1)), ("pi", range(rb.N)), ("Pb", range(rb.N))):
+            for k in ks:
+                g = rs.vec(gsol, i, kind, k)
+                r = srs.vec(sbuf, i, kind, k)
+                d = np.abs(g - r)
+                rel = np.where(d > 1e-11, d / np.maximum(np.abs(g) + np.abs(r), 1e-300), 0.0)
+                worst = max(worst, float(rel.max()) if rel.size else 0.0)
+        gb.append(trs_backward_error(rb, inp, rs, sin, gsol, i))
+        ob.append(trs_backward_error(sub, keep[0].inbuf, srs, sbuf, sbuf, i))
+        ok = ok and gb[-1] <= 10 * ob[-1] + 100
+    checks = {"instances": nchk, "max_kkt_backward_error_over_eps": max(gb),
+              "median_kkt_backward_error_over_eps": float(np.median(gb)),
+              "oracle_max_kkt_backward_error_over_eps": max(ob),
+              "oracle_median_kkt_backward_error_over_eps": float(np.median(ob)),
+              "max_rel_diff_vs_oracle_on_gpu_factors": worst, "pass": bool(ok)}
+    log("checked %d instances: KKT backward error max %.3g / median %.3g eps (oracle %.3g / %.3g); "
+        "max rel diff vs oracle %.1e" % (nchk, max(gb), np.median(gb), max(ob), np.median(ob), worst))
+    return checks, argv, (sub, srs, sbuf, keep)
+
+
+def run_ric_trs(args, torch, bfa):
+    """The Riccati solve (d_back_ric_trs_libstr) on the factors of the
+    headline workload: the factorisation runs once, untimed; each step is one
+    batched solve launch."""
+    from blasfeo_amd.riccati import RiccatiBatch, DeviceRiccati, RiccatiSolve, DeviceRiccatiSolve
+    from blasfeo_amd import DMAT
+    import oracle
+    rng = np.random.default_rng(args.seed)
+    nxs = rng.integers(8, 65, args.batch)
+    nus = rng.integers(2, 17, args.batch)
+    log("building %d Riccati instances (N=%d) and their solve vectors" % (args.batch, args.horizon))
+    rb = RiccatiBatch(nxs, nus, args.horizon, args.ps)
+    inp = rb.fill_inputs(rng)
+    rs = RiccatiSolve(rb)
+    sin = rs.fill_inputs(rng)
+    dev = DeviceRiccati(rb, inp)
+    ds = DeviceRiccatiSolve(dev, rs, sin)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        dev.run(sh)
+    torch.cuda.synchronize()
+    log("timing %d solve launches" % args.steps)
+    ms = time_steps(torch, stream, lambda: ds.run(sh), args.steps, args.warmup)
+    solves = args.batch / (ms * 1e-3)
+    algo = riccati_trs_bytes(nxs, nus, args.horizon)
+    achieved = algo / (ms * 1e-3) / 1e9
+    log("%.3f ms/step, %.0f solves/s, %.0f GB/s algorithmic" % (ms, solves, achieved))
+
+    checks, cpu = {}, None
+    if not args.no_cpu:
+        gbuf = dev.outbuf.cpu().numpy()
+        Ltab = dev.t_dev[2].cpu().numpy().view(DMAT)
+        n = min(args.cpu_max_instances, rb.batch)
+        checks, argv, keep = trs_checks(rb, inp, rs, sin, gbuf, Ltab, ds.host(), n)
+        L = oracle.lib()
+        log("cpu baseline (%.0f s budget)" % args.cpu_seconds)
+        cpu = cpu_time(lambda: L.or_dric_trs_batch(rb.ps, n, argv.ctypes.data), n, args.cpu_seconds,
+                       "solves/s", "%d of the %d instances (same seeded data and factors)" % (n, rb.batch))
+    traffic = read_traffic("k_ric_trs", "ric_trs-%dx%d" % (args.batch, args.horizon))
+    out = {"metric": METRIC, "value": solves, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic, seeded (numpy PCG64 seed %d): the headline Riccati instances, "
+                   "b and rq ~ N(0,1) per stage" % args.seed,
+           "config": {"workload": "riccati_trs_batched", "batch": args.batch, "horizon": args.horizon,
+                      "nx": "U{8..64}", "nu": "U{2..16}", "ps": args.ps, "precision": "fp64",
+                      "parallelism": "one wavefront per instance, 1 GPU"},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                        "algorithmic_bytes": algo, "kernel": "k_ric_trs<%d>" % args.ps},
+           "cpu_baseline": cpu, "checks": checks}
+    print(json.dumps(out), flush=True)
 
 
 # ---------------------------------------------------------------------------

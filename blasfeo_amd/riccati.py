@@ -194,6 +194,48 @@ class RiccatiSolve:
         return a
 
 
+def trs_backward_error(rb, inp, rs, sin, sol, i):
+    """Normwise backward error of instance i's Riccati solve ``sol`` (a
+    RiccatiSolve buffer; ``sin`` holds its b and rq, ``inp`` the RiccatiBatch
+    inputs), in units of eps: the
+    largest over stages of the residual of the stage's KKT equations
+        H_k z_k + rq_k + BAbt_k pi_k - E_k^T pi_{k-1} = 0,   BAbt_k^T z_k + b_k - x_{k+1} = 0
+    (z_k = ux_k, x_{k+1} the state part of ux_{k+1}, E_k z_k = x_k) over
+    ||H_k|| ||z_k|| + ||rq_k|| + ||BAbt_k|| ||pi_k|| + ||pi_{k-1}|| and
+    ||BAbt_k|| ||z_k|| + ||b_k|| + ||x_{k+1}|| (infinity norms)."""
+    eps = np.finfo(np.float64).eps
+    N, nx, nu = rb.N, int(rb.nxs[i]), int(rb.nus[i])
+    C = rb.inp.get(inp, rb.i_babt[i])
+    H = np.tril(rb.inp.get(inp, rb.i_rsq[i]))
+    H = H + np.tril(H, -1).T
+    HN = np.tril(rb.inp.get(inp, rb.i_qn[i]))
+    HN = HN + np.tril(HN, -1).T
+    nrm = lambda v: float(np.abs(v).max()) if np.size(v) else 0.0
+    nC, nH, nHN = np.abs(C).sum(1).max(), np.abs(H).sum(1).max(), np.abs(HN).sum(1).max()
+    z = [rs.vec(sol, i, "ux", k) for k in range(N + 1)]
+    pi = [rs.vec(sol, i, "pi", k) for k in range(N)]
+    worst = 0.0
+    for k in range(N + 1):
+        g = rs.vec(sin, i, "rq", k)
+        Hk, nHk = (H, nH) if k < N else (HN, nHN)
+        r = Hk @ z[k] + g
+        den = nHk * nrm(z[k]) + nrm(g)
+        if k < N:
+            r = r + C @ pi[k]
+            den += nC * nrm(pi[k])
+        if k > 0:
+            r[len(r) - nx:] -= pi[k - 1]
+            den += nrm(pi[k - 1])
+        worst = max(worst, nrm(r) / (den * eps))
+        if k < N:
+            b = rs.vec(sin, i, "b", k)
+            x1 = z[k + 1][len(z[k + 1]) - nx:]
+            r = C.T @ z[k] + b - x1
+            den = np.abs(C).sum(0).max() * nrm(z[k]) + nrm(b) + nrm(x1)
+            worst = max(worst, nrm(r) / (den * eps))
+    return worst
+
+
 class DeviceRiccati:
     """A RiccatiBatch resident in GPU memory, ready to launch."""
 

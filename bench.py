@@ -153,7 +153,7 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="riccati: time replays of the batched sweep captured in a HIP graph")
     ap.add_argument("--workload", default="riccati",
-                    choices=("riccati", "ric_trs", "potrf32", "gemm_syrk", "getrf", "sgetrf", "posv256"),
+                    choices=("riccati", "ric_trs", "pack", "potrf32", "gemm_syrk", "getrf", "sgetrf", "posv256"),
                     help="riccati (default, the headline); the others measure the remaining "
                          "batched configurations")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -186,6 +186,8 @@ def main():
         return run_posv(args, torch, bfa)
     if args.workload == "ric_trs":
         return run_ric_trs(args, torch, bfa)
+    if args.workload == "pack":
+        return run_pack(args, torch, bfa)
     if args.workload != "riccati":
         return run_other(args, torch, bfa)
     return run_riccati(args, torch, bfa)
@@ -424,6 +426,87 @@ def run_ric_trs(args, torch, bfa):
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                         "algorithmic_bytes": algo, "kernel": "k_ric_trs<%d>" % args.ps},
+           "cpu_baseline": cpu, "checks": checks}
+    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# column-major <-> panel-major conversion on the device
+# ---------------------------------------------------------------------------
+def run_pack(args, torch, bfa):
+    """blasfeo_pack_dmat for 16384 column-major n x n blocks, n ~ U{16..256}
+    (the LU workload's sizes), into panel-major descriptors; each step is one
+    batched pack launch.  The batched unpack is timed the same way and
+    reported beside it."""
+    import oracle
+    from blasfeo_amd.device import to_device
+    rng = np.random.default_rng(args.seed)
+    batch = 16384
+    nn = rng.integers(16, 257, batch)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    cm_off = np.concatenate([[0], np.cumsum(nn * nn)[:-1]]).astype(np.int64)
+    cm = torch.empty(int((nn * nn).sum()), dtype=torch.float64, device="cuda")
+    cm.normal_(generator=torch.Generator(device="cuda").manual_seed(args.seed))
+    _, _, nb = bfa.bulk_layout(np.float64, args.ps, nn, nn, 0)
+    pm = torch.zeros(nb // 8, dtype=torch.float64, device="cuda")
+    d = to_device(bfa.bulk_layout(np.float64, args.ps, nn, nn, pm.data_ptr())[0])
+    rec = np.zeros(batch, dtype=bfa.DPACK_ARG)
+    rec["m"], rec["n"], rec["lda"] = nn, nn, nn
+    rec["A"] = np.uint64(cm.data_ptr()) + (8 * cm_off).astype(np.uint64)
+    rec["sA"] = np.uint64(d.data_ptr()) + np.arange(batch, dtype=np.uint64) * bfa.DMAT.itemsize
+    drec = to_device(rec)
+    out_cm = torch.empty_like(cm)
+    rec2 = rec.copy()
+    rec2["A"] = np.uint64(out_cm.data_ptr()) + (8 * cm_off).astype(np.uint64)
+    drec2 = to_device(rec2)
+    torch.cuda.synchronize()
+    hint = (0, 256, 0)
+    ms = time_steps(torch, stream, lambda: bfa.call("dpack", args.ps, batch, drec.data_ptr(), hint, sh),
+                    args.steps, args.warmup)
+    ms_un = time_steps(torch, stream, lambda: bfa.call("dunpack", args.ps, batch, drec2.data_ptr(), hint, sh),
+                       args.steps, args.warmup)
+    nf = nn.astype(np.float64)
+    algo = float((16 * nf * nf).sum())
+    achieved = algo / (ms * 1e-3) / 1e9
+    value = batch / (ms * 1e-3)
+    log("pack: %.3f ms/step, %.4g matrices/s, %.0f GB/s; unpack %.3f ms, %.0f GB/s"
+        % (ms, value, achieved, ms_un, algo / (ms_un * 1e-3) / 1e9))
+    checks = {"roundtrip_bit_exact": bool(torch.equal(out_cm, cm))}
+    cpu = None
+    if not args.no_cpu:
+        n = 1024
+        hcm = cm[: int((nn[:n] * nn[:n]).sum())].cpu().numpy()
+        hd, _, hnb = bfa.bulk_layout(np.float64, args.ps, nn[:n], nn[:n], 0)
+        hpm = np.zeros(hnb // 8)
+        hd = bfa.bulk_layout(np.float64, args.ps, nn[:n], nn[:n], hpm.ctypes.data)[0]
+        hr = rec[:n].copy()
+        hr["A"] = np.uint64(hcm.ctypes.data) + (8 * cm_off[:n]).astype(np.uint64)
+        hr["sA"] = np.uint64(hd.ctypes.data) + np.arange(n, dtype=np.uint64) * bfa.DMAT.itemsize
+        L = oracle.lib()
+        L.or_dpack_batch(args.ps, n, hr.ctypes.data)
+        gpm = pm.cpu().numpy()
+        dd = d.cpu().numpy().view(bfa.DMAT)
+        ok = True
+        for i in range(min(n, 256)):
+            g0 = (int(dd["pA"][i]) - pm.data_ptr()) // 8
+            h0 = (int(hd["pA"][i]) - hpm.ctypes.data) // 8
+            ln = int(hd["pm"][i]) * int(hd["cn"][i])
+            ok = ok and np.array_equal(gpm[g0:g0 + ln], hpm[h0:h0 + ln])
+        checks["pack_bit_exact_vs_oracle_256"] = bool(ok)
+        cpu = cpu_time(lambda: L.or_dpack_batch(args.ps, n, hr.ctypes.data), n, args.cpu_seconds,
+                       "matrices/s", "%d of the %d blocks (same data)" % (n, batch))
+    log("checks: %s" % json.dumps(checks))
+    out = {"metric": METRIC, "value": value, "unit": "matrices/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic, seeded (%d), N(0,1)" % args.seed,
+           "config": {"workload": "dpack_batched", "batch": batch, "n": "U{16..256}", "ps": args.ps,
+                      "unpack_ms_per_step": ms_un,
+                      "unpack_GBps": algo / (ms_un * 1e-3) / 1e9},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": read_traffic("k_pack", "pack-16384"),
+                        "algorithmic_bytes": algo, "kernel": "k_pack<%d,PK_PACK>" % args.ps},
            "cpu_baseline": cpu, "checks": checks}
     print(json.dumps(out), flush=True)
 

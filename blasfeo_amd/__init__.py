@@ -21,7 +21,7 @@ import numpy as np
 
 __all__ = ["lib", "lib_path", "Layout", "DMAT", "SMAT", "DGEMM_ARG", "SGEMM_ARG",
            "DTRSM_ARG", "STRSM_ARG", "DFACT_ARG", "SFACT_ARG", "DSYRK_DPOTRF_ARG",
-           "RICCATI_ARG", "RICCATI_TRS_ARG", "DVEC", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
+           "RICCATI_ARG", "RICCATI_TRS_ARG", "DVEC", "DPACK_ARG", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libblasfeo_amd.so")
@@ -56,6 +56,8 @@ DSYRK_DPOTRF_ARG = np.dtype([("m", "i4"), ("n", "i4"), ("k", "i4")] + _offs +
                             [("sA", "u8"), ("sB", "u8"), ("sC", "u8"), ("sD", "u8")], align=True)
 RICCATI_ARG = np.dtype([("N", "i4"), ("nx", "u8"), ("nu", "u8"), ("BAbt", "u8"),
                         ("RSQrq", "u8"), ("L", "u8"), ("work", "u8")], align=True)
+DPACK_ARG = np.dtype([(f, "i4") for f in ("m", "n", "lda", "ai", "aj")] +
+                     [("A", "u8"), ("sA", "u8")], align=True)
 RICCATI_TRS_ARG = np.dtype([("N", "i4")] + [(f, "u8") for f in (
     "nx", "nu", "BAbt", "b", "rq", "L", "Pb", "ux", "pi", "work")], align=True)
 DPOSV_ARG = np.dtype([("n", "i4"), ("nrhs", "i4")] +
@@ -73,7 +75,7 @@ _BATCHED = [
     "dtrsm_llnn", "dtrsm_lltn",
     "dpotrf_l", "dpotrf_l_mn", "dsyrk_dpotrf_ln", "dgetrf_rp",
     "sgemm_nt", "sgemm_nn", "ssyrk_ln", "strsm_rltn", "spotrf_l", "sgetrf_rp",
-    "d_back_ric_trf", "d_back_ric_trs",
+    "d_back_ric_trf", "d_back_ric_trs", "dpack", "dpack_tran", "dunpack",
 ]
 
 

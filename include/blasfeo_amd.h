@@ -317,6 +317,31 @@ int blasfeo_amd_spotrf_l_batched(int ps, int batch, const struct blasfeo_amd_sfa
 int blasfeo_amd_sgetrf_rp_batched(int ps, int batch, const struct blasfeo_amd_sfact_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* column-major <-> panel-major conversion on the device                    */
+/* ------------------------------------------------------------------------ */
+
+/* One blasfeo_pack_dmat(m, n, A, lda, sA, ai, aj) (auxiliary/d_aux_lib4.c:1607),
+ * blasfeo_pack_tran_dmat (:2086; the transpose of the m x n A into sA, an
+ * n x m block: sA(ai+r, aj+c) = A[c + r*lda]) or blasfeo_unpack_dmat(m, n, sA,
+ * ai, aj, A, lda) (:2238), with A a column-major device buffer (lib8 twins in
+ * auxiliary/d_aux_lib8.c).  pack and pack_tran set sA->use_dA = 0, as the
+ * reference's do. */
+struct blasfeo_amd_dpack_arg
+	{
+	int m, n, lda, ai, aj;
+	double *A;
+	struct blasfeo_dmat *sA;
+	};
+
+/* Batched conversions, one workgroup per instance (more for wide blocks:
+ * hint->max_n, the largest n, may be given), each 64 x 16 tile staged through
+ * LDS so that the column-major and the panel-major sides are both read and
+ * written in contiguous runs. */
+int blasfeo_amd_dpack_batched(int ps, int batch, const struct blasfeo_amd_dpack_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+int blasfeo_amd_dpack_tran_batched(int ps, int batch, const struct blasfeo_amd_dpack_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+int blasfeo_amd_dunpack_batched(int ps, int batch, const struct blasfeo_amd_dpack_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Riccati recursion (examples/example_d_riccati_recursion.c:115-140)        */
 /* ------------------------------------------------------------------------ */
 

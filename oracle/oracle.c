@@ -102,6 +102,27 @@ void or_dric_trf_batch(int ps, int batch, const struct blasfeo_amd_riccati_arg *
 		or_dric_trf(ps, a[i].N, a[i].nx, a[i].nu, a[i].BAbt, a[i].RSQrq, a[i].L, a[i].work);
 	}
 
+void or_dpack_batch(int ps, int batch, const struct blasfeo_amd_dpack_arg *a)
+	{
+#pragma omp parallel for schedule(dynamic, 16)
+	for(int i = 0; i < batch; i++)
+		or_dpack(ps, a[i].m, a[i].n, a[i].A, a[i].lda, a[i].sA, a[i].ai, a[i].aj);
+	}
+
+void or_dpack_tran_batch(int ps, int batch, const struct blasfeo_amd_dpack_arg *a)
+	{
+#pragma omp parallel for schedule(dynamic, 16)
+	for(int i = 0; i < batch; i++)
+		or_dpack_tran(ps, a[i].m, a[i].n, a[i].A, a[i].lda, a[i].sA, a[i].ai, a[i].aj);
+	}
+
+void or_dunpack_batch(int ps, int batch, const struct blasfeo_amd_dpack_arg *a)
+	{
+#pragma omp parallel for schedule(dynamic, 16)
+	for(int i = 0; i < batch; i++)
+		or_dunpack(ps, a[i].m, a[i].n, a[i].sA, a[i].ai, a[i].aj, a[i].A, a[i].lda);
+	}
+
 void or_dric_trs_batch(int ps, int batch, const struct blasfeo_amd_riccati_trs_arg *a)
 	{
 #pragma omp parallel for schedule(dynamic, 4)

@@ -332,6 +332,35 @@ void PFX(ric_trf)(int ps, int N, const int *nx, const int *nu, MAT *BAbt, MAT *R
 		}
 	}
 
+/* ---- column-major <-> panel-major (auxiliary/d_aux_lib4.c) ---- */
+
+/* sA(ai+i, aj+j) = A[i + j*lda]; use_dA = 0; auxiliary/d_aux_lib4.c:1607-1640 */
+void PFX(pack)(int ps, int m, int n, REAL *A, int lda, MAT *sA, int ai, int aj)
+	{
+	sA->use_dA = 0;
+	for(int j = 0; j < n; j++)
+		for(int i = 0; i < m; i++)
+			EL(sA, ai+i, aj+j) = A[i + (size_t)j*lda];
+	}
+
+/* sA(ai+i, aj+j) = A[j + i*lda], i < n, j < m (the transpose of the m x n A);
+ * use_dA = 0; auxiliary/d_aux_lib4.c:2086-2125 */
+void PFX(pack_tran)(int ps, int m, int n, REAL *A, int lda, MAT *sA, int ai, int aj)
+	{
+	sA->use_dA = 0;
+	for(int i = 0; i < n; i++)
+		for(int j = 0; j < m; j++)
+			EL(sA, ai+i, aj+j) = A[j + (size_t)i*lda];
+	}
+
+/* A[i + j*lda] = sA(ai+i, aj+j); auxiliary/d_aux_lib4.c:2238-2270 */
+void PFX(unpack)(int ps, int m, int n, MAT *sA, int ai, int aj, REAL *A, int lda)
+	{
+	for(int j = 0; j < n; j++)
+		for(int i = 0; i < m; i++)
+			A[i + (size_t)j*lda] = EL(sA, ai+i, aj+j);
+	}
+
 /* ---- level 2 (blasfeo_ref/x_blas2_ref.c), used by the Riccati solve ----
  * Rows (gemv_n, trmv, trsv) or columns (gemv_t) are taken in pairs; inside a
  * pair each partial sum adds two products at a time (a*x0 + b*x1, rounded,

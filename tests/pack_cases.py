@@ -1,0 +1,39 @@
+"""Shared batch builder for the column-major <-> panel-major conversion tests:
+random block sizes and offsets, a column-major buffer (with leading
+dimensions above the row count) and panel-major destinations larger than the
+block, both filled with a sentinel outside the moved elements."""
+import numpy as np
+
+import blasfeo_amd as bfa
+
+SENTINEL = -1.0
+
+
+def make_cases(rng, batch, ps, tran=False, maxdim=200):
+    ms = rng.integers(1, maxdim + 1, batch)
+    ns = rng.integers(1, maxdim + 1, batch)
+    ms[:3] = [1, maxdim, 7]
+    ns[:3] = [maxdim, 1, 5]
+    ais = rng.integers(0, 10, batch)
+    ajs = rng.integers(0, 10, batch)
+    ldas = ms + rng.integers(0, 5, batch)
+    R = ns if tran else ms            # panel-major block rows x cols
+    C = ms if tran else ns
+    desc, offs, nbytes = bfa.bulk_layout(np.float64, ps, ais + R + 3, ajs + C + 2, 0)
+    cm_off = np.concatenate([[0], np.cumsum(ldas * ns)[:-1]]).astype(np.int64)
+    cm_len = int((ldas * ns).sum())
+    return dict(ms=ms, ns=ns, ais=ais, ajs=ajs, ldas=ldas, R=R, C=C, pm_bytes=nbytes,
+                cm_off=cm_off, cm_len=cm_len, ps=ps, batch=batch)
+
+
+def records(c, cm_base, desc_base):
+    rec = np.zeros(c["batch"], dtype=bfa.DPACK_ARG)
+    rec["m"], rec["n"], rec["lda"] = c["ms"], c["ns"], c["ldas"]
+    rec["ai"], rec["aj"] = c["ais"], c["ajs"]
+    rec["A"] = np.uint64(int(cm_base)) + (8 * c["cm_off"]).astype(np.uint64)
+    rec["sA"] = np.uint64(int(desc_base)) + np.arange(c["batch"], dtype=np.uint64) * bfa.DMAT.itemsize
+    return rec
+
+
+def descriptors(c, pm_base):
+    return bfa.bulk_layout(np.float64, c["ps"], c["ais"] + c["R"] + 3, c["ajs"] + c["C"] + 2, pm_base)[0]

@@ -21,7 +21,7 @@ import numpy as np
 
 __all__ = ["lib", "lib_path", "Layout", "DMAT", "SMAT", "DGEMM_ARG", "SGEMM_ARG",
            "DTRSM_ARG", "STRSM_ARG", "DFACT_ARG", "SFACT_ARG", "DSYRK_DPOTRF_ARG",
-           "RICCATI_ARG", "RICCATI_TRS_ARG", "DVEC", "DPACK_ARG", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
+           "RICCATI_ARG", "RICCATI_TRS_ARG", "DVEC", "DPACK_ARG", "DROWPE_ARG", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libblasfeo_amd.so")
@@ -58,6 +58,7 @@ RICCATI_ARG = np.dtype([("N", "i4"), ("nx", "u8"), ("nu", "u8"), ("BAbt", "u8"),
                         ("RSQrq", "u8"), ("L", "u8"), ("work", "u8")], align=True)
 DPACK_ARG = np.dtype([(f, "i4") for f in ("m", "n", "lda", "ai", "aj")] +
                      [("A", "u8"), ("sA", "u8")], align=True)
+DROWPE_ARG = np.dtype([("kmax", "i4"), ("ipiv", "u8"), ("sA", "u8")], align=True)
 RICCATI_TRS_ARG = np.dtype([("N", "i4")] + [(f, "u8") for f in (
     "nx", "nu", "BAbt", "b", "rq", "L", "Pb", "ux", "pi", "work")], align=True)
 DPOSV_ARG = np.dtype([("n", "i4"), ("nrhs", "i4")] +
@@ -72,7 +73,7 @@ class HINT(ctypes.Structure):
 
 _BATCHED = [
     "dgemm_nt", "dgemm_nn", "dsyrk_ln", "dsyrk_ln_mn", "dtrsm_rltn", "dtrmm_rlnn",
-    "dtrsm_llnn", "dtrsm_lltn",
+    "dtrsm_llnn", "dtrsm_lltn", "dtrsm_llnu", "dtrsm_lunn", "drowpe",
     "dpotrf_l", "dpotrf_l_mn", "dsyrk_dpotrf_ln", "dgetrf_rp",
     "sgemm_nt", "sgemm_nn", "ssyrk_ln", "strsm_rltn", "spotrf_l", "sgetrf_rp",
     "d_back_ric_trf", "d_back_ric_trs", "dpack", "dpack_tran", "dunpack",

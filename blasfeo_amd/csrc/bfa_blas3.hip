@@ -159,8 +159,12 @@ __global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::ty
 }
 
 // ---------------------------------------------------------------------------
-// trsm_llnn / trsm_lltn: D = alpha * A^{-1} * B or alpha * A^{-T} * B, A lower
-// m x m, B m x n (blasfeo_ref/x_blas3_ref.c:445-673 and 674-900).
+// trsm_llnn / trsm_lltn / trsm_llnu / trsm_lunn: D = alpha * A^{-1} * B,
+// alpha * A^{-T} * B with A lower, alpha * A^{-1} * B with A unit lower, or
+// alpha * U^{-1} * B with U the upper triangle of A; m x m, B m x n
+// (blasfeo_ref/x_blas3_ref.c:445-673, 674-900, 572-672, 920-1100).  lunn is
+// lltn on the transposed view of A (U(i,k) = A^T(i,k)), so it shares lltn's
+// bottom-up sweep; llnu is llnn without the reciprocal diagonal.
 // A workgroup keeps a block of up to LNC columns of X = alpha*B for all m rows
 // in LDS (column-major, odd leading dimension) and sweeps the 16-row blocks of
 // A, top-down for llnn and bottom-up for lltn: wave 0 solves the 16x16
@@ -174,7 +178,19 @@ __global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::ty
 constexpr int LNC = 16;
 template <typename T> constexpr int trsml_mmax() { return sizeof(T) == 8 ? 1024 : 2048; }
 
-template <typename T, int PS, bool TRANS>
+enum TrsmLeftMode { TL_LLNN = 0, TL_LLTN = 1, TL_LLNU = 2, TL_LUNN = 3 };
+// the sweep runs bottom-up for the transposed-lower and the upper solves
+constexpr bool tl_back(int mode) { return mode == TL_LLTN || mode == TL_LUNN; }
+
+// element (r, c) of the lower triangle the sweep works on: A itself, or A^T
+// for the upper solve (its upper triangle read as a lower one)
+template <typename T, int PS, int MODE>
+__device__ __forceinline__ T tl_at(const PM<T, PS> gA, int ai, int aj, int r, int c)
+{
+	return MODE == TL_LUNN ? gA.at(ai + c, aj + r) : gA.at(ai + r, aj + c);
+}
+
+template <typename T, int PS, int MODE>
 __device__ __forceinline__ void trsml_stage_diag(const PM<T, PS> gA, int ai, int aj, int r0, int rv, T *Ad)
 {
 	for (int e = threadIdx.x; e < 256; e += NT3)
@@ -182,14 +198,16 @@ __device__ __forceinline__ void trsml_stage_diag(const PM<T, PS> gA, int ai, int
 		const int r = e & 15, c = e >> 4;
 		T v = T(0);
 		if (r < rv && c < rv && r > c)
-			v = gA.at(ai + r0 + r, aj + r0 + c);
+			v = tl_at<T, PS, MODE>(gA, ai, aj, r0 + r, r0 + c);
 		Ad[r * 17 + c] = v;
 	}
 }
 
-template <typename T, int PS, bool TRANS>
+template <typename T, int PS, int MODE>
 __global__ __launch_bounds__(NT3) void k_trsm_left(const typename TrsmArg<T>::type *__restrict__ args, int mmax)
 {
+	constexpr bool TRANS = tl_back(MODE);
+	constexpr bool UNIT = MODE == TL_LLNU;
 	extern __shared__ unsigned char smem_raw[];
 	__shared__ T Ad[2][16 * 17];
 	const int mpad = (mmax + 15) / 16 * 16;
@@ -210,9 +228,12 @@ __global__ __launch_bounds__(NT3) void k_trsm_left(const typename TrsmArg<T>::ty
 	const int w = wave_id();
 	const int mb = (m + 15) / 16;
 
-	const bool cached = (ai == 0 && aj == 0 && a.sD != a.sA && a.sA->use_dA >= m);
-	for (int j = threadIdx.x; j < m; j += NT3)
-		inv_s[j] = cached ? a.sA->dA[j] : T(1) / gA.at(ai + j, aj + j);
+	if (!UNIT)
+	{
+		const bool cached = (ai == 0 && aj == 0 && a.sD != a.sA && a.sA->use_dA >= m);
+		for (int j = threadIdx.x; j < m; j += NT3)
+			inv_s[j] = cached ? a.sA->dA[j] : T(1) / gA.at(ai + j, aj + j);
+	}
 
 	for (int j0 = blockIdx.y * LNC; j0 < n; j0 += gridDim.y * LNC)
 	{
@@ -221,7 +242,7 @@ __global__ __launch_bounds__(NT3) void k_trsm_left(const typename TrsmArg<T>::ty
 		g2s<T, PS, NT3, false>(gB, a.bi, a.bj + j0, mb * 16, LNC, m, nc, Xs, LDM);
 		{
 			const int rb0 = TRANS ? mb - 1 : 0;
-			trsml_stage_diag<T, PS, TRANS>(gA, ai, aj, rb0 * 16, min(16, m - rb0 * 16), Ad[0]);
+			trsml_stage_diag<T, PS, MODE>(gA, ai, aj, rb0 * 16, min(16, m - rb0 * 16), Ad[0]);
 		}
 		__syncthreads();
 		for (int e = threadIdx.x; e < LNC * LDM; e += NT3)
@@ -251,7 +272,7 @@ __global__ __launch_bounds__(NT3) void k_trsm_left(const typename TrsmArg<T>::ty
 #pragma unroll
 							for (int l = 0; l < i; l++)
 								c -= A0[i * 17 + l] * x[l];
-							x[i] = c * inv_s[r0 + i];
+							x[i] = UNIT ? c : c * inv_s[r0 + i];
 						}
 					}
 				}
@@ -280,7 +301,7 @@ __global__ __launch_bounds__(NT3) void k_trsm_left(const typename TrsmArg<T>::ty
 			if (s + 1 < mb)
 			{
 				const int rbn = TRANS ? rb - 1 : rb + 1;
-				trsml_stage_diag<T, PS, TRANS>(gA, ai, aj, rbn * 16, min(16, m - rbn * 16), Ad[(s + 1) & 1]);
+				trsml_stage_diag<T, PS, MODE>(gA, ai, aj, rbn * 16, min(16, m - rbn * 16), Ad[(s + 1) & 1]);
 			}
 			const int ilo = TRANS ? 0 : r0 + 16;
 			const int ihi = TRANS ? r0 : m;
@@ -292,7 +313,7 @@ __global__ __launch_bounds__(NT3) void k_trsm_left(const typename TrsmArg<T>::ty
 				{
 					T v = T(0);
 					if (l < rv)
-						v = TRANS ? gA.at(ai + r0 + l, aj + i) : gA.at(ai + i, aj + r0 + l);
+						v = TRANS ? tl_at<T, PS, MODE>(gA, ai, aj, r0 + l, i) : tl_at<T, PS, MODE>(gA, ai, aj, i, r0 + l);
 					av[l] = v;
 				}
 				for (int c = 0; c < nc; c++)
@@ -315,13 +336,20 @@ __global__ __launch_bounds__(NT3) void k_trsm_left(const typename TrsmArg<T>::ty
 // 466-480): invalidate D's cache, then refresh A's reciprocal diagonal unless
 // it was valid.  Run as its own launch so that no workgroup of the solve
 // reads a dA another one is writing.  LEFT: the triangle is m x m, else n x n.
-template <typename T, int PS, bool LEFT>
+template <typename T, int PS, bool LEFT, bool UNIT = false>
 __global__ void k_trsm_dA(const typename TrsmArg<T>::type *__restrict__ args)
 {
 	const typename TrsmArg<T>::type &a = args[blockIdx.x];
 	const int n = LEFT ? a.m : a.n;
 	if (a.m <= 0 || a.n <= 0)
 		return;
+	if (UNIT)
+	{
+		// trsm_llnu reads no reciprocal diagonal: only D's cache is invalidated
+		if (threadIdx.x == 0)
+			a.sD->use_dA = 0;
+		return;
+	}
 	const bool zero_off = (a.ai == 0 && a.aj == 0);
 	const int use0 = (a.sD == a.sA) ? 0 : a.sA->use_dA;
 	__syncthreads();
@@ -429,18 +457,18 @@ static int launch_trsm(int ps, int batch, const typename TrsmArg<T>::type *args,
 	return launch_trsm_ps<T, 16>(batch, args, grid, shm, npad, st, name);
 }
 
-template <typename T, int PS, bool TRANS>
+template <typename T, int PS, int MODE>
 static int launch_trsml_ps(int batch, const typename TrsmArg<T>::type *args, dim3 grid, size_t shm, int mmax, hipStream_t st, const char *name)
 {
-	hipLaunchKernelGGL((k_trsm_left<T, PS, TRANS>), grid, dim3(NT3), shm, st, args, mmax);
+	hipLaunchKernelGGL((k_trsm_left<T, PS, MODE>), grid, dim3(NT3), shm, st, args, mmax);
 	int rc = check_launch(name);
 	if (rc)
 		return rc;
-	hipLaunchKernelGGL((k_trsm_dA<T, PS, true>), dim3(batch), dim3(64), 0, st, args);
+	hipLaunchKernelGGL((k_trsm_dA<T, PS, true, MODE == TL_LLNU>), dim3(batch), dim3(64), 0, st, args);
 	return check_launch(name);
 }
 
-template <typename T, bool TRANS>
+template <typename T, int MODE>
 static int launch_trsml(int ps, int batch, const typename TrsmArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
 {
 	int rc = check_common(ps, batch, args, sizeof(T) == 4, name);
@@ -456,10 +484,10 @@ static int launch_trsml(int ps, int batch, const typename TrsmArg<T>::type *args
 	const int gy = hint->max_n > 0 ? min(64, (hint->max_n + LNC - 1) / LNC) : 1;
 	dim3 grid(batch, gy);
 	if (ps == 4)
-		return launch_trsml_ps<T, 4, TRANS>(batch, args, grid, shm, mmax, st, name);
+		return launch_trsml_ps<T, 4, MODE>(batch, args, grid, shm, mmax, st, name);
 	if (ps == 8)
-		return launch_trsml_ps<T, 8, TRANS>(batch, args, grid, shm, mmax, st, name);
-	return launch_trsml_ps<T, 16, TRANS>(batch, args, grid, shm, mmax, st, name);
+		return launch_trsml_ps<T, 8, MODE>(batch, args, grid, shm, mmax, st, name);
+	return launch_trsml_ps<T, 16, MODE>(batch, args, grid, shm, mmax, st, name);
 }
 
 } // namespace bfa
@@ -481,9 +509,13 @@ int blasfeo_amd_dtrmm_rlnn_batched(int ps, int batch, const blasfeo_amd_dtrsm_ar
 int blasfeo_amd_dtrsm_rltn_batched(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
 { return launch_trsm<double>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_rltn_batched"); }
 int blasfeo_amd_dtrsm_llnn_batched(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
-{ return launch_trsml<double, false>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_llnn_batched"); }
+{ return launch_trsml<double, TL_LLNN>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_llnn_batched"); }
 int blasfeo_amd_dtrsm_lltn_batched(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
-{ return launch_trsml<double, true>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_lltn_batched"); }
+{ return launch_trsml<double, TL_LLTN>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_lltn_batched"); }
+int blasfeo_amd_dtrsm_llnu_batched(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_trsml<double, TL_LLNU>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_llnu_batched"); }
+int blasfeo_amd_dtrsm_lunn_batched(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_trsml<double, TL_LUNN>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dtrsm_lunn_batched"); }
 
 int blasfeo_amd_sgemm_nt_batched(int ps, int batch, const blasfeo_amd_sgemm_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
 { return launch_gemm<float, OP_NT>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_sgemm_nt_batched"); }

@@ -298,6 +298,26 @@ int blasfeo_amd_dtrsm_llnn_batched(int ps, int batch, const struct blasfeo_amd_d
 int blasfeo_amd_dtrsm_lltn_batched(int ps, int batch, const struct blasfeo_amd_dtrsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 /* replaces batch x blasfeo_dtrmm_rlnn (blasfeo_hp_pm/d_blas3_lib4.c:4893) */
 int blasfeo_amd_dtrmm_rlnn_batched(int ps, int batch, const struct blasfeo_amd_dtrsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* LU solve pieces (examples/example_d_lu_factorization.c:163-171: drowpe, then
+ * trsm_llnu with the unit-lower L, then trsm_lunn with the upper U of one
+ * getrf_rp result).
+ * replaces batch x blasfeo_dtrsm_llnu (blasfeo_ref/x_blas3_ref.c:572): D = alpha * L^{-1} * B,
+ * L unit lower m x m (its diagonal is not read); sD->use_dA = 0, sA untouched */
+int blasfeo_amd_dtrsm_llnu_batched(int ps, int batch, const struct blasfeo_amd_dtrsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dtrsm_lunn (blasfeo_ref/x_blas3_ref.c:920): D = alpha * U^{-1} * B,
+ * U the upper triangle of sA (m x m, non-unit); the dA / use_dA rule of trsm_lltn */
+int blasfeo_amd_dtrsm_lunn_batched(int ps, int batch, const struct blasfeo_amd_dtrsm_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* One blasfeo_drowpe(kmax, ipiv, sA) (auxiliary/d_aux_lib4.c:2767): rows ii and
+ * ipiv[ii] swapped over all sA->n columns for ii = 0 .. kmax-1 in order;
+ * sA->use_dA = 0.  ipiv is a device array; kmax is capped at sA->m and entries
+ * outside 0 .. sA->m-1 are skipped (the reference would write out of bounds). */
+struct blasfeo_amd_drowpe_arg
+	{
+	int kmax;
+	int *ipiv;
+	struct blasfeo_dmat *sA;
+	};
+int blasfeo_amd_drowpe_batched(int ps, int batch, const struct blasfeo_amd_drowpe_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 /* replaces batch x blasfeo_dpotrf_l (blasfeo_hp_pm/d_lapack_lib4.c:4269,918) */
 int blasfeo_amd_dpotrf_l_batched(int ps, int batch, const struct blasfeo_amd_dfact_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 /* replaces batch x blasfeo_dpotrf_l_mn (blasfeo_hp_pm/d_lapack_lib4.c:1155) */

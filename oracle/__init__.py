@@ -23,6 +23,7 @@ _SIGS = {
     "syrk_ln": "iiRpiipiiRpiipii", "syrk_ln_mn": "iiiRpiipiiRpiipii",
     "trsm_rltn": "iiRpiipiipii", "trmm_rlnn": "iiRpiipiipii",
     "trsm_llnn": "iiRpiipiipii", "trsm_lltn": "iiRpiipiipii",
+    "trsm_llnu": "iiRpiipiipii", "trsm_lunn": "iiRpiipiipii", "rowpe": "ipp",
     "potrf_l": "ipiipii", "potrf_l_mn": "iipiipii",
     "syrk_potrf_ln": "iipiipiipiipii", "syrk_potrf_ln_mn": "iiipiipiipiipii",
     "getrf_rp": "iipiipiip", "ric_trf": "ippppp",

@@ -173,6 +173,60 @@ void PFX(trsm_lltn)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, M
 			}
 	}
 
+/* D <= alpha * A^{-1} * B, A unit lower m x m (diagonal not read), sD->use_dA = 0 and
+ * sA untouched; blasfeo_ref/x_blas3_ref.c:572-672 (k ascending; the in-pair term of a
+ * row pair is the last, which is also ascending) */
+void PFX(trsm_llnu)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	sD->use_dA = 0;
+	for(int j = 0; j < n; j++)
+		for(int i = 0; i < m; i++)
+			{
+			REAL c = alpha * EL(sB, bi+i, bj+j);
+			for(int l = 0; l < i; l++)
+				c -= EL(sA, ai+i, aj+l) * EL(sD, di+l, dj+j);
+			EL(sD, di+i, dj+j) = c;
+			}
+	}
+
+/* D <= alpha * U^{-1} * B, U the upper triangle of A (m x m, non-unit);
+ * blasfeo_ref/x_blas3_ref.c:920-1100: the dA rule and the pair order of
+ * trsm_lltn with A(k, i) read as A(i, k) */
+void PFX(trsm_lunn)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sD, int di, int dj)
+	{
+	if(m <= 0 || n <= 0)
+		return;
+	REAL *dA = PFX(trsm_left_diag)(ps, m, sA, ai, aj, sD);
+	for(int j = 0; j < n; j++)
+		for(int i = m - 1; i >= 0; i--)
+			{
+			REAL c = alpha * EL(sB, bi+i, bj+j);
+			const int pair_top = ((m - i) % 2 == 0);
+			for(int l = pair_top ? i + 2 : i + 1; l < m; l++)
+				c -= EL(sA, ai+i, aj+l) * EL(sD, di+l, dj+j);
+			if(pair_top)
+				c -= EL(sA, ai+i, aj+i+1) * EL(sD, di+i+1, dj+j);
+			EL(sD, di+i, dj+j) = c * dA[i];
+			}
+	}
+
+/* blasfeo_drowpe: rows ii and ipiv[ii] swapped over all columns, ii ascending;
+ * use_dA = 0; auxiliary/d_aux_lib4.c:2767-2780 */
+void PFX(rowpe)(int ps, int kmax, int *ipiv, MAT *sA)
+	{
+	sA->use_dA = 0;
+	for(int ii = 0; ii < kmax; ii++)
+		if(ipiv[ii] != ii)
+			for(int j = 0; j < sA->n; j++)
+				{
+				REAL t = EL(sA, ii, j);
+				EL(sA, ii, j) = EL(sA, ipiv[ii], j);
+				EL(sA, ipiv[ii], j) = t;
+				}
+	}
+
 /* D <= alpha * B * A, A lower n x n ; blasfeo_ref/x_blas3_ref.c:3218-3330 */
 void PFX(trmm_rlnn)(int ps, int m, int n, REAL alpha, MAT *sA, int ai, int aj, MAT *sB, int bi, int bj, MAT *sD, int di, int dj)
 	{

@@ -153,7 +153,7 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="riccati: time replays of the batched sweep captured in a HIP graph")
     ap.add_argument("--workload", default="riccati",
-                    choices=("riccati", "ric_trs", "pack", "potrf32", "gemm_syrk", "getrf", "sgetrf", "posv256"),
+                    choices=("riccati", "ric_trs", "pack", "getrs", "potrf32", "gemm_syrk", "getrf", "sgetrf", "posv256"),
                     help="riccati (default, the headline); the others measure the remaining "
                          "batched configurations")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -188,6 +188,8 @@ def main():
         return run_ric_trs(args, torch, bfa)
     if args.workload == "pack":
         return run_pack(args, torch, bfa)
+    if args.workload == "getrs":
+        return run_getrs(args, torch, bfa)
     if args.workload != "riccati":
         return run_other(args, torch, bfa)
     return run_riccati(args, torch, bfa)
@@ -507,6 +509,144 @@ def run_pack(args, torch, bfa):
                         "frac": achieved / HBM_PEAK_GBS,
                         "traffic": read_traffic("k_pack", "pack-16384"),
                         "algorithmic_bytes": algo, "kernel": "k_pack<%d,PK_PACK>" % args.ps},
+           "cpu_baseline": cpu, "checks": checks}
+    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# LU solve on getrf_rp factors
+# ---------------------------------------------------------------------------
+def run_getrs(args, torch, bfa):
+    """The LU solve of examples/example_d_lu_factorization.c:163-171 for the LU
+    workload's sizes: 16384 n x n factors, n ~ U{16..256}, 4 right-hand sides.
+    getrf_rp runs once, untimed; each step is drowpe + trsm_llnu + trsm_lunn on
+    right-hand sides restored from a copy (the copy is timed alone and
+    subtracted)."""
+    import oracle
+    from blasfeo_amd.device import to_device
+    rng = np.random.default_rng(args.seed)
+    batch, nrhs = 16384, 4
+    nn = rng.integers(16, 257, batch)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    gen = torch.Generator(device="cuda").manual_seed(args.seed)
+    _, _, nb = bfa.bulk_layout(np.float64, args.ps, nn, nn, 0)
+    Abuf = torch.empty(nb // 8, dtype=torch.float64, device="cuda").uniform_(-1, 1, generator=gen)
+    dA = to_device(bfa.bulk_layout(np.float64, args.ps, nn, nn, Abuf.data_ptr())[0])
+    rr = np.full(batch, nrhs)
+    _, _, nbb = bfa.bulk_layout(np.float64, args.ps, nn, rr, 0)
+    Bbuf = torch.empty(nbb // 8, dtype=torch.float64, device="cuda").normal_(generator=gen)
+    B0 = Bbuf.clone()
+    dB = to_device(bfa.bulk_layout(np.float64, args.ps, nn, rr, Bbuf.data_ptr())[0])
+    idx = np.arange(batch, dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(nn)[:-1]]).astype(np.uint64)
+    piv = torch.zeros(int(nn.sum()), dtype=torch.int32, device="cuda")
+    f = np.zeros(batch, dtype=bfa.DFACT_ARG)
+    f["m"], f["n"] = nn, nn
+    f["sC"] = np.uint64(dA.data_ptr()) + idx * bfa.DMAT.itemsize
+    f["sD"] = f["sC"]
+    f["ipiv"] = np.uint64(piv.data_ptr()) + 4 * offs
+    p = np.zeros(batch, dtype=bfa.DROWPE_ARG)
+    p["kmax"], p["ipiv"] = nn, f["ipiv"]
+    p["sA"] = np.uint64(dB.data_ptr()) + idx * bfa.DMAT.itemsize
+    t = np.zeros(batch, dtype=bfa.DTRSM_ARG)
+    t["m"], t["n"], t["alpha"] = nn, nrhs, 1.0
+    t["sA"], t["sB"], t["sD"] = f["sC"], p["sA"], p["sA"]
+    df, dp, dt = to_device(f), to_device(p), to_device(t)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        bfa.call("dgetrf_rp", args.ps, batch, df.data_ptr(), (256, 256, 0), sh)
+    torch.cuda.synchronize()
+    hint = (256, nrhs, 0)
+
+    def step():
+        Bbuf.copy_(B0)
+        bfa.call("drowpe", args.ps, batch, dp.data_ptr(), None, sh)
+        bfa.call("dtrsm_llnu", args.ps, batch, dt.data_ptr(), hint, sh)
+        bfa.call("dtrsm_lunn", args.ps, batch, dt.data_ptr(), hint, sh)
+    with torch.cuda.stream(stream):
+        ms_tot = time_steps(torch, stream, step, args.steps, args.warmup)
+        ms_cp = time_steps(torch, stream, lambda: Bbuf.copy_(B0), args.steps, 1)
+        step()
+    torch.cuda.synchronize()
+    ms = ms_tot - ms_cp
+    nf = nn.astype(np.float64)
+    # the factor read once (strict lower by llnu, upper by lunn) + the pivots,
+    # the right-hand sides read and written by each of the three launches
+    algo = float((8 * nf * nf + 4 * nf + 3 * 2 * 8 * nf * nrhs).sum())
+    flops = float((2 * nf * nf * nrhs).sum())
+    achieved = algo / (ms * 1e-3) / 1e9
+    value = batch / (ms * 1e-3)
+    log("getrs: %.3f ms/step (copy %.3f ms subtracted), %.4g solves/s, %.0f GB/s"
+        % (ms, ms_cp, value, achieved))
+    # checks: normwise backward error of the first 256 solves in FP64
+    nchk = 256
+    Ah = Abuf.cpu().numpy()
+    dAh = bfa.bulk_layout(np.float64, args.ps, nn, nn, 0)[0]
+    dBh = bfa.bulk_layout(np.float64, args.ps, nn, rr, 0)[0]
+    # the factored A is gone; rebuild the original from the same generator
+    gen2 = torch.Generator(device="cuda").manual_seed(args.seed)
+    A0 = torch.empty(nb // 8, dtype=torch.float64, device="cuda").uniform_(-1, 1, generator=gen2).cpu().numpy()
+    X = Bbuf.cpu().numpy()
+    Bh = B0.cpu().numpy()
+    worst = 0.0
+    for i in range(nchk):
+        n = int(nn[i])
+        cnA, cnB = int(dAh["cn"][i]), int(dBh["cn"][i])
+        oA, oB = int(dAh["pA"][i]) // 8, int(dBh["pA"][i]) // 8
+        A = bfa.pm_unpack(A0[oA:], args.ps, cnA, n, n)
+        x = bfa.pm_unpack(X[oB:], args.ps, cnB, n, nrhs)
+        bb = bfa.pm_unpack(Bh[oB:], args.ps, cnB, n, nrhs)
+        e = np.abs(A @ x - bb).max() / ((np.abs(A).sum(1).max() * np.abs(x).max() + np.abs(bb).max())
+                                        * n * np.finfo(float).eps)
+        worst = max(worst, float(e))
+    checks = {"instances": nchk, "max_backward_error_over_n_eps": worst, "pass": worst <= 20}
+    log("checks: %s" % json.dumps(checks))
+    cpu = None
+    if not args.no_cpu:
+        n = 512
+        hA = bfa.bulk_layout(np.float64, args.ps, nn[:n], nn[:n], 0)
+        hAbuf = np.zeros(hA[2] // 8)
+        hAd = bfa.bulk_layout(np.float64, args.ps, nn[:n], nn[:n], hAbuf.ctypes.data)[0]
+        hB = bfa.bulk_layout(np.float64, args.ps, nn[:n], rr[:n], 0)
+        hBbuf = np.zeros(hB[2] // 8)
+        hBd = bfa.bulk_layout(np.float64, args.ps, nn[:n], rr[:n], hBbuf.ctypes.data)[0]
+        for i in range(n):
+            oA, oB = int(dAh["pA"][i]) // 8, int(dBh["pA"][i]) // 8
+            ln, lb = int(hAd["pm"][i]) * int(hAd["cn"][i]), int(hBd["pm"][i]) * int(hBd["cn"][i])
+            hAbuf[(int(hAd["pA"][i]) - hAbuf.ctypes.data) // 8:][:ln] = Ah[oA:oA + ln]
+            hBbuf[(int(hBd["pA"][i]) - hBbuf.ctypes.data) // 8:][:lb] = Bh[oB:oB + lb]
+        hpiv = piv.cpu().numpy()
+        fo = oracle.fn("d", "rowpe")
+        lo, uo = oracle.fn("d", "trsm_llnu"), oracle.fn("d", "trsm_lunn")
+        sa = [hAd.ctypes.data + i * bfa.DMAT.itemsize for i in range(n)]
+        sb = [hBd.ctypes.data + i * bfa.DMAT.itemsize for i in range(n)]
+        pv = [hpiv[int(offs[i]):int(offs[i]) + int(nn[i])].copy() for i in range(n)]
+
+        hB0 = hBbuf.copy()
+
+        def cpu_run():
+            hBbuf[:] = hB0      # right-hand sides restored, as on the GPU
+            for i in range(n):
+                m = int(nn[i])
+                fo(args.ps, m, pv[i].ctypes.data, sb[i])
+                lo(args.ps, m, nrhs, 1.0, sa[i], 0, 0, sb[i], 0, 0, sb[i], 0, 0)
+                uo(args.ps, m, nrhs, 1.0, sa[i], 0, 0, sb[i], 0, 0, sb[i], 0, 0)
+        cpu = cpu_time(cpu_run, n, args.cpu_seconds, "solves/s",
+                       "%d of the %d factors (same data), one host thread" % (n, batch))
+        cpu["cores"] = 1
+    out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic, seeded (%d): A ~ U[-1,1], B ~ N(0,1)" % args.seed,
+           "config": {"workload": "dgetrs_batched (drowpe + dtrsm_llnu + dtrsm_lunn)", "batch": batch,
+                      "n": "U{16..256}", "nrhs": nrhs, "ps": args.ps,
+                      "timing": "step time minus a same-size device copy that restores B"},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": algo,
+                        "flops_per_launch": flops,
+                        "kernel": "k_rowpe + k_trsm_left<double,%d,TL_LLNU> + k_trsm_left<double,%d,TL_LUNN>"
+                                  % (args.ps, args.ps)},
            "cpu_baseline": cpu, "checks": checks}
     print(json.dumps(out), flush=True)
 

@@ -2,43 +2,93 @@
 // for ii = 0 .. kmax-1, rows ii and ipiv[ii] of sA are swapped over all of its
 // columns, in that order (the LAPACK-style pivot vector getrf_rp returns).
 //
-// The swaps are sequential in ii but independent across columns, so each
-// thread of an instance's workgroup replays the whole swap sequence on its own
-// columns: no synchronisation, and every swap of a column stays in the thread
-// that owns it.
+// The swap sequence is composed first into one permutation in LDS (perm[i] =
+// the original row that ends in row i; one thread replays the swaps on the
+// index array, with ipiv staged in LDS by all threads), then every column
+// block of the matrix is gathered through it: all threads read the source
+// rows of a block into LDS, then write the rows back in place.  Matrices
+// taller than RP_MAXM rows replay the swaps per column instead (each thread
+// owns columns, so no synchronisation is needed there).
 #include "bfa_common.hpp"
 #include "bfa_internal.hpp"
 
 namespace bfa {
 
-constexpr int RP_NT = 64;
+constexpr int RP_NT = 256;
+constexpr int RP_MAXM = 512;     // rows of the LDS form
+constexpr int RP_CB = 4;         // columns gathered per pass
 
 template <int PS>
 __global__ __launch_bounds__(RP_NT) void k_rowpe(const blasfeo_amd_drowpe_arg *__restrict__ args)
 {
+	__shared__ int perm[RP_MAXM];
+	__shared__ double blk[RP_MAXM * RP_CB];
 	const blasfeo_amd_drowpe_arg a = args[blockIdx.x];
 	blasfeo_dmat *sA = a.sA;
-	if (threadIdx.x == 0)
+	const int t = threadIdx.x;
+	if (t == 0)
 		sA->use_dA = 0;
 	// entries outside the matrix (a malformed ipiv) are skipped, not followed
 	const int m = sA->m, kmax = min(a.kmax, m), n = sA->n, cn = sA->cn;
 	double *pA = sA->pA;
 	const int *ipiv = a.ipiv;
-	for (int c = threadIdx.x; c < n; c += RP_NT)
+	auto at = [&](int i, int c) -> double & {
+		return pA[(size_t)(i & ~(PS - 1)) * cn + (size_t)c * PS + (i & (PS - 1))];
+	};
+	if (m > RP_MAXM)
 	{
-		double *col = pA + (size_t)c * PS;
+		for (int c = t; c < n; c += RP_NT)
+			for (int ii = 0; ii < kmax; ii++)
+			{
+				const int p = ipiv[ii];
+				if (p != ii && p >= 0 && p < m)
+				{
+					const double x = at(ii, c);
+					at(ii, c) = at(p, c);
+					at(p, c) = x;
+				}
+			}
+		return;
+	}
+	// ipiv staged in the gather buffer (as ints), then the swaps composed
+	int *piv_s = reinterpret_cast<int *>(blk);
+	for (int i = t; i < m; i += RP_NT)
+	{
+		perm[i] = i;
+		piv_s[i] = i < kmax ? ipiv[i] : i;
+	}
+	__syncthreads();
+	if (t == 0)
+	{
 		for (int ii = 0; ii < kmax; ii++)
 		{
-			const int p = ipiv[ii];
+			const int p = piv_s[ii];
 			if (p != ii && p >= 0 && p < m)
 			{
-				double *x = col + (size_t)(ii & ~(PS - 1)) * cn + (ii & (PS - 1));
-				double *y = col + (size_t)(p & ~(PS - 1)) * cn + (p & (PS - 1));
-				const double t = *x;
-				*x = *y;
-				*y = t;
+				const int x = perm[ii];
+				perm[ii] = perm[p];
+				perm[p] = x;
 			}
 		}
+	}
+	__syncthreads();
+	for (int c0 = 0; c0 < n; c0 += RP_CB)
+	{
+		const int nc = min(RP_CB, n - c0);
+		for (int e = t; e < m * RP_CB; e += RP_NT)
+		{
+			const int i = e % m, c = e / m;
+			if (c < nc)
+				blk[c * RP_MAXM + i] = at(perm[i], c0 + c);
+		}
+		__syncthreads();
+		for (int e = t; e < m * RP_CB; e += RP_NT)
+		{
+			const int i = e % m, c = e / m;
+			if (c < nc && perm[i] != i)
+				at(i, c0 + c) = blk[c * RP_MAXM + i];
+		}
+		__syncthreads();
 	}
 }
 

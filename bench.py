@@ -633,8 +633,9 @@ def run_getrs(args, torch, bfa):
                 lo(args.ps, m, nrhs, 1.0, sa[i], 0, 0, sb[i], 0, 0, sb[i], 0, 0)
                 uo(args.ps, m, nrhs, 1.0, sa[i], 0, 0, sb[i], 0, 0, sb[i], 0, 0)
         cpu = cpu_time(cpu_run, n, args.cpu_seconds, "solves/s",
-                       "%d of the %d factors (same data), one host thread" % (n, batch))
+                       "%d of the %d factors (same data)" % (n, batch))
         cpu["cores"] = 1
+        cpu["sample"] = cpu["sample"].replace("OpenMP over instances", "one host thread, instance by instance")
     out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64",

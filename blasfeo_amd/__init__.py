@@ -74,7 +74,7 @@ class HINT(ctypes.Structure):
 _BATCHED = [
     "dgemm_nt", "dgemm_nn", "dsyrk_ln", "dsyrk_ln_mn", "dtrsm_rltn", "dtrmm_rlnn",
     "dtrsm_llnn", "dtrsm_lltn", "dtrsm_llnu", "dtrsm_lunn", "drowpe",
-    "dpotrf_l", "dpotrf_l_mn", "dsyrk_dpotrf_ln", "dgetrf_rp",
+    "dpotrf_l", "dpotrf_l_mn", "dsyrk_dpotrf_ln", "dgetrf_rp", "dgetrf_np",
     "sgemm_nt", "sgemm_nn", "ssyrk_ln", "strsm_rltn", "spotrf_l", "sgetrf_rp",
     "d_back_ric_trf", "d_back_ric_trs", "dpack", "dpack_tran", "dunpack",
 ]

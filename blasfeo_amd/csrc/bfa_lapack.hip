@@ -448,9 +448,12 @@ __global__ __launch_bounds__(NTL) void k_syrk_potrf(const typename SPArg<T>::typ
 }
 
 // ---------------------------------------------------------------------------
-// LU with partial row pivoting (blasfeo_ref/x_lapack_ref.c:1156-1536 semantics)
+// LU with partial row pivoting (blasfeo_ref/x_lapack_ref.c:1156-1536 semantics);
+// PIV = false: LU without pivoting, getrf_np (x_lapack_ref.c:880-1150): no
+// pivot search, no swaps, ipiv not touched, and dA = 1/U(j,j) with no guard
+// for a zero pivot (the reference divides unconditionally)
 // ---------------------------------------------------------------------------
-template <typename T, int PS>
+template <typename T, int PS, bool PIV = true>
 __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k_getrf(const typename FactArg<T>::type *__restrict__ args, int mmax)
 {
 	typedef typename Vec4<T>::type V4;
@@ -609,6 +612,9 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 			for (int j = 0; j < cf; j++)
 			{
 				const int jr = c0 + j;
+				int ip = jr;
+				if (PIV)
+				{
 				T best = T(0);
 				int bi = jr;
 				for (int i = jr + threadIdx.x; i < m; i += NTL)
@@ -629,7 +635,7 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 				}
 				__syncthreads();
 				T b = wbest[0];
-				int ip = widx[0];
+				ip = widx[0];
 				for (int ww = 1; ww < NTL / WAVE; ww++)
 					if (wbest[ww] > b || (wbest[ww] == b && widx[ww] < ip))
 					{
@@ -638,7 +644,8 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 					}
 				if (threadIdx.x == 0)
 					pivs[j] = ip;
-				if (ip != jr)
+				}
+				if (PIV && ip != jr)
 				{
 					if (threadIdx.x < PW)
 					{
@@ -655,7 +662,7 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 				}
 				__syncthreads();
 				const T piv = P[j * LDP + jr];
-				const T inv = piv != T(0) ? T(1) / piv : T(0);
+				const T inv = PIV ? (piv != T(0) ? T(1) / piv : T(0)) : T(1) / piv;
 				if (threadIdx.x == 0)
 					invs[j] = inv;
 				for (int i = jr + 1 + threadIdx.x; i < m; i += NTL)
@@ -673,6 +680,9 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 			for (int j = 0; j < cf; j++)
 			{
 				const int jr = c0 + j;
+				int ip = jr;
+				if (PIV)
+				{
 				// pivot: first row of largest |x| among jr..m-1, from (0, jr)
 				T best = T(0);
 				int bi = jr;
@@ -686,10 +696,11 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 						bi = i;
 					}
 				}
-				const int ip = wave_argmax_abs<T>(best, bi);
+				ip = wave_argmax_abs<T>(best, bi);
 				if (lane == 0)
 					pivs[j] = ip;
-				if (ip != jr)
+				}
+				if (PIV && ip != jr)
 				{
 					if (lane < PW)
 					{
@@ -706,7 +717,7 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 				}
 				wave_sync();
 				const T piv = P[j * LDP + jr];
-				const T inv = piv != T(0) ? T(1) / piv : T(0);
+				const T inv = PIV ? (piv != T(0) ? T(1) / piv : T(0)) : T(1) / piv;
 				if (lane == 0)
 					invs[j] = inv;
 				for (int i = jr + 1 + lane; i < m; i += WAVE)
@@ -724,12 +735,13 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 		s2g<T, PS, NTL, false>(gD, di, dj + c0, m, cw, P, LDP, 0);
 		for (int j = threadIdx.x; j < cf; j += NTL)
 		{
-			ipiv[c0 + j] = pivs[j];
+			if (PIV)
+				ipiv[c0 + j] = pivs[j];
 			dD[c0 + j] = invs[j];
 		}
 		__syncthreads();
 		// this panel's swaps on the finished columns 0..c0-1 (one thread per column)
-		for (int c = threadIdx.x; c < c0; c += NTL)
+		for (int c = threadIdx.x; PIV && c < c0; c += NTL)
 		{
 			for (int j = 0; j < cf; j++)
 			{
@@ -858,7 +870,7 @@ static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args
 	return check_launch(name);
 }
 
-template <typename T>
+template <typename T, bool PIV = true>
 static int launch_getrf(int ps, int batch, const typename FactArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
 {
 	int rc = fact_check(ps, batch, args, sizeof(T) == 4, name);
@@ -869,11 +881,11 @@ static int launch_getrf(int ps, int batch, const typename FactArg<T>::type *args
 		return rc;
 	const size_t shm = sizeof(T) * PW * ((mmax + 15) / 16 * 16 + 1) + sizeof(int) * mmax;
 	if (ps == 4)
-		hipLaunchKernelGGL((k_getrf<T, 4>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+		hipLaunchKernelGGL((k_getrf<T, 4, PIV>), dim3(batch), dim3(NTL), shm, st, args, mmax);
 	else if (ps == 8)
-		hipLaunchKernelGGL((k_getrf<T, 8>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+		hipLaunchKernelGGL((k_getrf<T, 8, PIV>), dim3(batch), dim3(NTL), shm, st, args, mmax);
 	else
-		hipLaunchKernelGGL((k_getrf<T, 16>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+		hipLaunchKernelGGL((k_getrf<T, 16, PIV>), dim3(batch), dim3(NTL), shm, st, args, mmax);
 	return check_launch(name);
 }
 
@@ -924,6 +936,8 @@ int blasfeo_amd_dsyrk_dpotrf_ln_batched(int ps, int batch, const blasfeo_amd_dsy
 { return launch_syrk_potrf(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dsyrk_dpotrf_ln_batched"); }
 int blasfeo_amd_dgetrf_rp_batched(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
 { return launch_getrf<double>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dgetrf_rp_batched"); }
+int blasfeo_amd_dgetrf_np_batched(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
+{ return launch_getrf<double, false>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dgetrf_np_batched"); }
 int blasfeo_amd_spotrf_l_batched(int ps, int batch, const blasfeo_amd_sfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
 { return launch_potrf<float, 0>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_spotrf_l_batched"); }
 int blasfeo_amd_sgetrf_rp_batched(int ps, int batch, const blasfeo_amd_sfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)

@@ -324,6 +324,11 @@ int blasfeo_amd_dpotrf_l_batched(int ps, int batch, const struct blasfeo_amd_dfa
 int blasfeo_amd_dpotrf_l_mn_batched(int ps, int batch, const struct blasfeo_amd_dfact_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 /* replaces batch x blasfeo_dsyrk_dpotrf_ln(_mn) (blasfeo_hp_pm/d_lapack_lib4.c:1848) */
 int blasfeo_amd_dsyrk_dpotrf_ln_batched(int ps, int batch, const struct blasfeo_amd_dsyrk_dpotrf_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+/* replaces batch x blasfeo_dgetrf_np (blasfeo_hp_pm/d_lapack_lib4.c:2078; REF
+ * blasfeo_ref/x_lapack_ref.c:880): LU without pivoting, unit-lower L and U in D,
+ * dA = 1/U(j,j) (a zero pivot gives inf, as the reference divides unconditionally),
+ * use_dA = 1 at offset (0,0); the record's ipiv is ignored */
+int blasfeo_amd_dgetrf_np_batched(int ps, int batch, const struct blasfeo_amd_dfact_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 /* replaces batch x blasfeo_dgetrf_rp (blasfeo_hp_pm/d_lapack_lib4.c:4311,2396) */
 int blasfeo_amd_dgetrf_rp_batched(int ps, int batch, const struct blasfeo_amd_dfact_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 

@@ -26,7 +26,7 @@ _SIGS = {
     "trsm_llnu": "iiRpiipiipii", "trsm_lunn": "iiRpiipiipii", "rowpe": "ipp",
     "potrf_l": "ipiipii", "potrf_l_mn": "iipiipii",
     "syrk_potrf_ln": "iipiipiipiipii", "syrk_potrf_ln_mn": "iiipiipiipiipii",
-    "getrf_rp": "iipiipiip", "ric_trf": "ippppp",
+    "getrf_rp": "iipiipiip", "getrf_np": "iipiipii", "ric_trf": "ippppp",
     "gemv_n": "iiRpiipiRpipi", "gemv_t": "iiRpiipiRpipi",
     "trmv_lnn": "ipiipipi", "trmv_ltn": "ipiipipi",
     "trsv_lnn_mn": "iipiipipi", "trsv_ltn_mn": "iipiipipi",

@@ -372,6 +372,40 @@ void PFX(getrf_rp)(int ps, int m, int n, MAT *sC, int ci, int cj, MAT *sD, int d
 		}
 	}
 
+/* LU without pivoting, blasfeo_ref/x_lapack_ref.c:880-1150: column by column,
+ * U(i,j) = C(i,j) - sum_{k<i} L(i,k) U(k,j) for i <= j, then
+ * L(i,j) = (C(i,j) - sum_{k<j} L(i,k) U(k,j)) * (1/U(j,j)) for i > j, every sum in
+ * ascending k (the reference's 2x2 blocks subtract the in-block term last, which
+ * is also ascending); dA = 1/U(j,j) with no zero guard; use_dA = 1 at offset (0,0) */
+void PFX(getrf_np)(int ps, int m, int n, MAT *sC, int ci, int cj, MAT *sD, int di, int dj)
+	{
+	sD->use_dA = (di == 0 && dj == 0) ? 1 : 0;
+	if(m <= 0 || n <= 0)
+		return;
+	REAL *dD = sD->dA;
+	for(int j = 0; j < n; j++)
+		{
+		for(int i = 0; i <= j && i < m; i++)
+			{
+			REAL c = EL(sC, ci+i, cj+j);
+			for(int k = 0; k < i; k++)
+				c -= EL(sD, di+i, dj+k) * EL(sD, di+k, dj+j);
+			EL(sD, di+i, dj+j) = c;
+			}
+		if(j >= m)
+			continue;
+		REAL inv = (REAL)1.0 / EL(sD, di+j, dj+j);
+		dD[j] = inv;
+		for(int i = j+1; i < m; i++)
+			{
+			REAL c = EL(sC, ci+i, cj+j);
+			for(int k = 0; k < j; k++)
+				c -= EL(sD, di+i, dj+k) * EL(sD, di+k, dj+j);
+			EL(sD, di+i, dj+j) = c * inv;
+			}
+		}
+	}
+
 /* backward Riccati factorisation d_back_ric_trf_libstr,
  * examples/example_d_riccati_recursion.c:115-140 (fused syrk_dpotrf branch) */
 void PFX(ric_trf)(int ps, int N, const int *nx, const int *nu, MAT *BAbt, MAT *RSQrq, MAT *L, MAT *work)

@@ -148,3 +148,38 @@ def test_lu_solve_pipeline(gpu, ps):
         berr = np.abs(A @ X - B).max() / (np.abs(A).sum(1).max() * np.abs(X).max() + np.abs(B).max())
         assert berr <= 20 * n * eps, "instance %d n=%d berr %.3g" % (i, n, berr)
         np.testing.assert_allclose(X, Xr, rtol=1e-8, atol=1e-8 * np.abs(Xr).max(), err_msg="X %d" % i)
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_getrf_np(gpu, ps):
+    """LU without pivoting on diagonally dominant blocks (single-wave and
+    multi-wave panel paths, rectangular blocks, offsets), ipiv NULL: L, U,
+    dA and use_dA against the oracle."""
+    rng = np.random.default_rng(980 + ps)
+    b = Batch("d", ps, seed=ps + 9)
+    recs = []
+    shapes = [(int(rng.integers(1, 300)),) * 2 for _ in range(16)] + [(40, 70), (150, 90), (256, 256), (1, 1), (64, 64), (65, 65)]
+    for r, (m, n) in enumerate(shapes):
+        ci, di = (int(x) for x in rng.integers(0, 2 * ps, 2))
+        cj, dj = (int(x) for x in rng.integers(0, 3, 2))
+        if r % 3 == 0:
+            di = dj = 0
+        A = rng.uniform(-1, 1, (ci + m, cj + n))
+        for t in range(min(m, n)):
+            A[ci + t, cj + t] += (n if t % 2 else -n)
+        C = b.mat(ci + m, cj + n, "data", A)
+        if r % 4 == 3:
+            D, di, dj = C, ci, cj
+        else:
+            D = b.mat(di + m, dj + n, "const", -1.0)
+        recs.append(dict(m=m, n=n, ci=ci, cj=cj, di=di, dj=dj, sC=C, sD=D, ipiv=0))
+    buf = b.host()
+    f = oracle.fn("d", "getrf_np")
+
+    def ocall(h, r):
+        f(ps, r["m"], r["n"], h.desc_addr(r["sC"]), r["ci"], r["cj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    print("getrf_np ps=%d batch=%d" % (ps, len(recs)), flush=True)
+    g, gd, rr, rd = run_both(b, buf, bfa.DFACT_ARG, recs, "dgetrf_np", ocall, (300, 300, 0))
+    compare_buffers(b, g, rr, "dgetrf_np ps=%d" % ps)
+    assert (gd["use_dA"] == rd["use_dA"]).all()

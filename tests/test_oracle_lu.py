@@ -85,3 +85,37 @@ def test_lu_solve_composition(ps):
         X = lay.get(hb.buf, 2)
         r = np.abs(A @ X - B).max() / (np.abs(A).sum(1).max() * np.abs(X).max() + np.abs(B).max())
         assert r < 20 * n * np.finfo(float).eps, (n, r)
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+@pytest.mark.parametrize("m,n,ci,di", [(6, 6, 0, 0), (9, 5, 3, 0), (5, 9, 0, 2), (1, 1, 0, 0), (13, 13, 5, 6)])
+def test_getrf_np_exact(ps, m, n, ci, di):
+    """C = L U with integer unit-lower L and integer upper U with dyadic
+    diagonal: every step is exact, so getrf_np returns L and U bit for bit,
+    dA = 1/diag(U), and use_dA is 1 only at offset (0, 0)."""
+    rng = np.random.default_rng(m * 7 + n + ps + ci)
+    p = min(m, n)
+    L = np.tril(rng.integers(-2, 3, (m, p)).astype(float), -1) + np.eye(m, p)
+    U = np.triu(rng.integers(-3, 4, (p, n)).astype(float), 1)
+    dg = rng.choice([1.0, 2.0, -4.0, 0.5], p)
+    U[np.arange(p), np.arange(p)] = dg
+    C = L @ U
+    Cb = np.full((ci + m, n + 1), 7.0)
+    Cb[ci:ci + m, :n] = C
+    Db = np.full((di + m, n + 2), -1.0)
+    lay, hb = _hb([Cb, Db], ps)
+    oracle.fn("d", "getrf_np")(ps, m, n, hb.desc_addr(0), ci, 0, hb.desc_addr(1), di, 0)
+    D = lay.get(hb.buf, 1)
+    got = D[di:di + m, :n]
+    np.testing.assert_array_equal(np.triu(got)[:p], np.triu(U)[:p])
+    np.testing.assert_array_equal(np.tril(got, -1)[:, :p], np.tril(L, -1)[:, :p])
+    np.testing.assert_array_equal(D[di:di + m, n:], -1.0)      # outside the block untouched
+    assert hb.desc[1]["use_dA"] == (1 if di == 0 else 0)
+    np.testing.assert_array_equal(lay.dview(hb.buf, 1)[:p], 1.0 / dg)
+
+
+def test_getrf_np_zero_pivot_gives_inf():
+    lay, hb = _hb([np.array([[0.0, 1.0], [1.0, 1.0]]), np.zeros((2, 2))], 4)
+    with np.errstate(all="ignore"):
+        oracle.fn("d", "getrf_np")(4, 2, 2, hb.desc_addr(0), 0, 0, hb.desc_addr(1), 0, 0)
+    assert np.isinf(lay.dview(hb.buf, 1)[0])

@@ -153,7 +153,8 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="riccati: time replays of the batched sweep captured in a HIP graph")
     ap.add_argument("--workload", default="riccati",
-                    choices=("riccati", "ric_trs", "pack", "getrs", "potrf32", "gemm_syrk", "getrf", "sgetrf", "posv256"),
+                    choices=("riccati", "ric_trs", "pack", "getrs", "potrf32", "gemm_syrk", "getrf", "sgetrf", "getrf_np",
+                             "posv256"),
                     help="riccati (default, the headline); the others measure the remaining "
                          "batched configurations")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -807,7 +808,18 @@ def run_other(args, torch, bfa):
             v.uniform_(-1.0, 1.0, generator=gen)
         else:
             _fill(torch, buf, "uniform", gen)
-        d = to_device(bfa.bulk_layout(dt, ps, nn, nn, buf.data_ptr())[0])
+        hd = bfa.bulk_layout(dt, ps, nn, nn, buf.data_ptr())[0]
+        d = to_device(hd)
+        if args.workload == "getrf_np":
+            # no pivoting: make every block diagonally dominant (+-n on the diagonal)
+            v = buf.view(torch.float64)
+            base = buf.data_ptr()
+            di = []
+            for i in range(batch):
+                t = np.arange(int(nn[i]))
+                di.append((int(hd["pA"][i]) - base) // 8 + (t - t % ps) * int(hd["cn"][i]) + t * ps + t % ps)
+            di = torch.from_numpy(np.concatenate(di)).to("cuda")
+            v[di] += torch.from_numpy(np.concatenate([np.where(np.arange(k) % 2, k, -k) for k in nn]).astype(np.float64)).to("cuda")
         piv_off = np.concatenate([[0], np.cumsum(nn)[:-1]]).astype(np.uint64)
         piv = torch.zeros(int(nn.sum()), dtype=torch.int32, device="cuda")
         rec = _records(bfa.DFACT_ARG, batch)
@@ -818,7 +830,7 @@ def run_other(args, torch, bfa):
         rec["ipiv"] = piv.data_ptr() + 4 * piv_off
         drec = to_device(rec)
         orig = buf.clone()
-        name = prec + "getrf_rp"
+        name = "dgetrf_np" if args.workload == "getrf_np" else prec + "getrf_rp"
 
         def fn():
             buf.copy_(orig)
@@ -836,7 +848,35 @@ def run_other(args, torch, bfa):
         bound, punit = "mfma", "TFLOP/s"
         peak = FP64_PEAK_TFLOPS if prec == "d" else 157.3
         achieved = flops / (ms * 1e-3) / 1e12
-        kernel = "k_getrf<%s,%d>" % ("double" if prec == "d" else "float", ps)
+        kernel = "k_getrf<%s,%d%s>" % ("double" if prec == "d" else "float", ps,
+                                       ",PIV=false" if args.workload == "getrf_np" else "")
+        if args.workload == "getrf_np":
+            # the first 64 factors against the oracle's getrf_np on the same blocks
+            # (the copy timing above left the input in buf: factor it once more)
+            with torch.cuda.stream(stream):
+                fn()
+            torch.cuda.synchronize()
+            gout, gin = buf.cpu().numpy().view(np.float64), orig.cpu().numpy().view(np.float64)
+            worst = 0.0
+            for i in range(64):
+                n = int(nn[i])
+                o = (int(hd["pA"][i]) - buf.data_ptr()) // 8
+                ln = int(hd["pm"][i]) * int(hd["cn"][i])
+                lay = bfa.Layout(np.float64, ps)
+                lay.add(n, n)
+                lay.add(n, n)
+                hb = lay.host_buffer()
+                lay.view(hb, 0)[:] = gin[o:o + ln]
+                desc = lay.build(hb.ctypes.data)
+                oracle.fn("d", "getrf_np")(ps, n, n, desc.ctypes.data, 0, 0,
+                                           desc.ctypes.data + bfa.DMAT.itemsize, 0, 0)
+                r = lay.get(hb, 1)                                  # the n x n block only
+                g = bfa.pm_unpack(gout[o:o + ln], ps, int(hd["cn"][i]), n, n)
+                dd = np.abs(g - r)
+                rel = np.where(dd > 1e-11, dd / np.maximum(np.abs(g) + np.abs(r), 1e-300), 0.0)
+                worst = max(worst, float(rel.max()))
+            checks = {"instances": 64, "max_rel_diff_vs_oracle": worst, "pass": worst <= 1e-11}
+            log("checks: %s" % json.dumps(checks))
         wl = {"workload": "%s_batched" % name, "batch": batch, "n": "U{16..256}", "ps": ps,
               "precision": "fp64" if prec == "d" else "fp32",
               "timing": "step time minus a same-size device copy that restores the input"}

@@ -21,7 +21,7 @@ import numpy as np
 
 __all__ = ["lib", "lib_path", "Layout", "DMAT", "SMAT", "DGEMM_ARG", "SGEMM_ARG",
            "DTRSM_ARG", "STRSM_ARG", "DFACT_ARG", "SFACT_ARG", "DSYRK_DPOTRF_ARG",
-           "RICCATI_ARG", "RICCATI_TRS_ARG", "DVEC", "DPACK_ARG", "DROWPE_ARG", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
+           "RICCATI_ARG", "RICCATI_TRS_ARG", "DVEC", "DPACK_ARG", "DROWPE_ARG", "PM_DMAT", "PM_DGECP_ARG", "pm_dgecp", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libblasfeo_amd.so")
@@ -56,6 +56,10 @@ DSYRK_DPOTRF_ARG = np.dtype([("m", "i4"), ("n", "i4"), ("k", "i4")] + _offs +
                             [("sA", "u8"), ("sB", "u8"), ("sC", "u8"), ("sD", "u8")], align=True)
 RICCATI_ARG = np.dtype([("N", "i4"), ("nx", "u8"), ("nu", "u8"), ("BAbt", "u8"),
                         ("RSQrq", "u8"), ("L", "u8"), ("work", "u8")], align=True)
+PM_DMAT = np.dtype([("mem", "u8"), ("pA", "u8"), ("dA", "u8")] +
+                   [(f, "i4") for f in ("m", "n", "pm", "cn", "use_dA", "ps", "memsize")], align=True)
+PM_DGECP_ARG = np.dtype([(f, "i4") for f in ("m", "n", "ai", "aj", "bi", "bj")] +
+                        [("sA", "u8"), ("sB", "u8")], align=True)
 DPACK_ARG = np.dtype([(f, "i4") for f in ("m", "n", "lda", "ai", "aj")] +
                      [("A", "u8"), ("sA", "u8")], align=True)
 DROWPE_ARG = np.dtype([("kmax", "i4"), ("ipiv", "u8"), ("sA", "u8")], align=True)
@@ -136,6 +140,16 @@ def call(name, ps, batch, args_ptr, hint=None, stream=0):
     if hint is not None:
         h = ctypes.byref(HINT(*[int(x) for x in hint]))
     check(f(ps, batch, ctypes.c_void_p(int(args_ptr)), h, ctypes.c_void_p(int(stream))), name)
+
+
+def pm_dgecp(batch, args_ptr, hint=None, stream=0):
+    """blasfeo_amd_pm_dgecp_batched (copy between panel sizes); raises on error."""
+    L = lib()
+    f = L.blasfeo_amd_pm_dgecp_batched
+    f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(HINT), ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    h = ctypes.byref(HINT(*[int(x) for x in hint])) if hint is not None else None
+    check(f(batch, ctypes.c_void_p(int(args_ptr)), h, ctypes.c_void_p(int(stream))), "pm_dgecp")
 
 
 def posv_mixed(ps, batch, args_ptr, tol, max_iter, hint, stream=0):

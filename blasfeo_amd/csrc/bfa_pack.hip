@@ -122,6 +122,78 @@ __global__ __launch_bounds__(PK_NT) void k_pack(const blasfeo_amd_dpack_arg *__r
 	}
 }
 
+// ---------------------------------------------------------------------------
+// blasfeo_dgecp (auxiliary/d_aux_lib4.c:3006) between panel-major matrices of
+// any two panel sizes: B(bi+i, bj+j) = A(ai+i, aj+j), sB->use_dA = 0.  The
+// panel sizes come from the blasfeo_pm_dmat descriptors; the tiles follow
+// B's panel grid (writes in 16 * psB contiguous runs), A is read one column
+// of a 64-row tile at a time (runs of psA contiguous doubles).
+// ---------------------------------------------------------------------------
+template <int PA, int PB>
+__device__ __forceinline__ void pm_gecp_tiles(const blasfeo_amd_pm_dgecp_arg &a, double *T)
+{
+	const int m = a.m, n = a.n;
+	const double *pa = a.sA->pA;
+	double *pb = a.sB->pA;
+	const int cna = a.sA->cn, cnb = a.sB->cn;
+	const int base = a.bi - (a.bi & (PB - 1));
+	const int span = (a.bi & (PB - 1)) + m;
+	const int t = threadIdx.x;
+	for (int c0 = blockIdx.y * PK_C; c0 < n; c0 += gridDim.y * PK_C)
+	{
+		const int nc = min(PK_C, n - c0);
+		for (int r0 = 0; r0 < span; r0 += PK_R)
+		{
+			const int b0 = base + r0 - a.bi;
+#pragma unroll
+			for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
+			{
+				const int rr = t & (PK_R - 1), cc = (t >> 6) + 4 * q;
+				const int br = b0 + rr;
+				if (cc < nc && br >= 0 && br < m)
+				{
+					const int i = a.ai + br;
+					T[cc * PK_LD + rr] = pa[(size_t)(i & ~(PA - 1)) * cna + (size_t)(a.aj + c0 + cc) * PA + (i & (PA - 1))];
+				}
+			}
+			__syncthreads();
+#pragma unroll
+			for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
+			{
+				const int e = t + q * PK_NT;
+				const int r = e & (PB - 1);
+				const int cc = (e / PB) & (PK_C - 1);
+				const int p = e / (PB * PK_C);
+				const int rr = p * PB + r;
+				const int br = b0 + rr;
+				if (cc < nc && br >= 0 && br < m)
+					pb[(size_t)(base + r0 + p * PB) * cnb + (size_t)(a.bj + c0 + cc) * PB + r] = T[cc * PK_LD + rr];
+			}
+			__syncthreads();
+		}
+	}
+}
+
+__global__ __launch_bounds__(PK_NT) void k_pm_gecp(const blasfeo_amd_pm_dgecp_arg *__restrict__ args)
+{
+	__shared__ double T[PK_C * PK_LD];
+	const blasfeo_amd_pm_dgecp_arg a = args[blockIdx.x];
+	if (blockIdx.y == 0 && threadIdx.x == 0)
+		a.sB->use_dA = 0;
+	if (a.m <= 0 || a.n <= 0)
+		return;
+	const int pa = a.sA->ps, pb = a.sB->ps;
+	// panel sizes other than 4 and 8 are left alone (the instance is skipped)
+	if (pa == 4 && pb == 4)
+		pm_gecp_tiles<4, 4>(a, T);
+	else if (pa == 4 && pb == 8)
+		pm_gecp_tiles<4, 8>(a, T);
+	else if (pa == 8 && pb == 4)
+		pm_gecp_tiles<8, 4>(a, T);
+	else if (pa == 8 && pb == 8)
+		pm_gecp_tiles<8, 8>(a, T);
+}
+
 template <int DIR>
 static int launch_pack(int ps, int batch, const blasfeo_amd_dpack_arg *args, const blasfeo_amd_hint *hint,
 	hipStream_t st, const char *name)
@@ -164,6 +236,21 @@ extern "C" int blasfeo_amd_dpack_batched(int ps, int batch, const blasfeo_amd_dp
 extern "C" int blasfeo_amd_dpack_tran_batched(int ps, int batch, const blasfeo_amd_dpack_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream)
 {
 	return pack_entry<PK_TRAN>(ps, batch, args, hint, stream, "blasfeo_amd_dpack_tran_batched");
+}
+
+extern "C" int blasfeo_amd_pm_dgecp_batched(int batch, const blasfeo_amd_pm_dgecp_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream)
+{
+	const char *name = "blasfeo_amd_pm_dgecp_batched";
+	if (batch < 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: batch=%d < 0", name, batch);
+	if (batch == 0)
+		return BLASFEO_AMD_OK;
+	if (!args)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: args is NULL", name);
+	const int nmax = hint && hint->max_n > 0 ? hint->max_n : PK_C;
+	const int gy = max(1, min(16, (nmax + PK_C - 1) / PK_C));
+	hipLaunchKernelGGL(k_pm_gecp, dim3(batch, gy), dim3(PK_NT), 0, (hipStream_t)stream, args);
+	return check_launch(name);
 }
 
 extern "C" int blasfeo_amd_dunpack_batched(int ps, int batch, const blasfeo_amd_dpack_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream)

@@ -366,6 +366,21 @@ int blasfeo_amd_dpack_batched(int ps, int batch, const struct blasfeo_amd_dpack_
 int blasfeo_amd_dpack_tran_batched(int ps, int batch, const struct blasfeo_amd_dpack_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 int blasfeo_amd_dunpack_batched(int ps, int batch, const struct blasfeo_amd_dpack_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 
+/* Panel size conversion on the device.  No counterpart in the reference, whose
+ * blasfeo_pm_dmat API (include/blasfeo_d_aux.h:222-226) only sizes, creates
+ * and prints: one blasfeo_dgecp(m, n, sA, ai, aj, sB, bi, bj)
+ * (auxiliary/d_aux_lib4.c:3006; B(bi+i, bj+j) = A(ai+i, aj+j), sB->use_dA = 0)
+ * between blasfeo_pm_dmat descriptors whose panel sizes (4 or 8, from their ps
+ * fields) may differ, so that a batch can move between ps = 4 and ps = 8
+ * without a host round trip.  Instances with another panel size are skipped. */
+struct blasfeo_amd_pm_dgecp_arg
+	{
+	int m, n, ai, aj, bi, bj;
+	struct blasfeo_pm_dmat *sA;
+	struct blasfeo_pm_dmat *sB;
+	};
+int blasfeo_amd_pm_dgecp_batched(int batch, const struct blasfeo_amd_pm_dgecp_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Riccati recursion (examples/example_d_riccati_recursion.c:115-140)        */
 /* ------------------------------------------------------------------------ */

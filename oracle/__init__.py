@@ -57,6 +57,10 @@ def lib():
             f = getattr(L, "or_%s_batch" % name)
             f.argtypes = [_I, _I, _V]
             f.restype = None
+        L.or_pm_dgecp.argtypes = [_I, _I, _V, _I, _I, _V, _I, _I]
+        L.or_pm_dgecp.restype = None
+        L.or_pm_dgecp_batch.argtypes = [_I, _V]
+        L.or_pm_dgecp_batch.restype = None
         L.or_num_threads.restype = _I
         _lib = L
     return _lib

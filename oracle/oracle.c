@@ -123,6 +123,25 @@ void or_dunpack_batch(int ps, int batch, const struct blasfeo_amd_dpack_arg *a)
 		or_dunpack(ps, a[i].m, a[i].n, a[i].sA, a[i].ai, a[i].aj, a[i].A, a[i].lda);
 	}
 
+/* blasfeo_dgecp between blasfeo_pm_dmat descriptors of any panel sizes:
+ * B(bi+i, bj+j) = A(ai+i, aj+j), sB->use_dA = 0 (auxiliary/d_aux_lib4.c:3006) */
+#define PMEL(s, i, j) ((s)->pA[((i) - (i) % (s)->ps) * (s)->cn + (j) * (s)->ps + (i) % (s)->ps])
+void or_pm_dgecp(int m, int n, struct blasfeo_pm_dmat *sA, int ai, int aj, struct blasfeo_pm_dmat *sB, int bi, int bj)
+	{
+	sB->use_dA = 0;
+	for(int j = 0; j < n; j++)
+		for(int i = 0; i < m; i++)
+			PMEL(sB, bi+i, bj+j) = PMEL(sA, ai+i, aj+j);
+	}
+#undef PMEL
+
+void or_pm_dgecp_batch(int batch, const struct blasfeo_amd_pm_dgecp_arg *a)
+	{
+#pragma omp parallel for schedule(dynamic, 16)
+	for(int i = 0; i < batch; i++)
+		or_pm_dgecp(a[i].m, a[i].n, a[i].sA, a[i].ai, a[i].aj, a[i].sB, a[i].bi, a[i].bj);
+	}
+
 void or_dric_trs_batch(int ps, int batch, const struct blasfeo_amd_riccati_trs_arg *a)
 	{
 #pragma omp parallel for schedule(dynamic, 4)

@@ -37,3 +37,24 @@ def records(c, cm_base, desc_base):
 
 def descriptors(c, pm_base):
     return bfa.bulk_layout(np.float64, c["ps"], c["ais"] + c["R"] + 3, c["ajs"] + c["C"] + 2, pm_base)[0]
+
+
+def pm_layout(pss, ms, ns, base):
+    """blasfeo_pm_dmat descriptors (runtime panel size, panel leading
+    dimension 4, as blasfeo_pm_create_dmat lays them out) for matrices of
+    panel sizes pss and sizes ms x ns packed from address base, each on a
+    256-byte boundary.  Returns (descriptors, byte offsets, total bytes)."""
+    pss, ms, ns = (np.asarray(x, dtype=np.int64) for x in (pss, ms, ns))
+    pm = -(-ms // pss) * pss
+    cn = -(-ns // 4) * 4
+    al = pss * 4
+    tmp = -(-np.minimum(ms, ns) // al) * al
+    size = (pm * cn + tmp) * 8
+    padded = -(-size // 256) * 256
+    offs = np.concatenate([[0], np.cumsum(padded)[:-1]]).astype(np.int64)
+    d = np.zeros(len(ms), dtype=bfa.PM_DMAT)
+    addr = np.uint64(int(base)) + offs.astype(np.uint64)
+    d["mem"] = d["pA"] = addr
+    d["dA"] = addr + (pm * cn * 8).astype(np.uint64)
+    d["m"], d["n"], d["pm"], d["cn"], d["ps"], d["memsize"] = ms, ns, pm, cn, pss, size
+    return d, offs, int(offs[-1] + padded[-1])

@@ -80,7 +80,8 @@ def test_rowpe(gpu, ps):
     rec["ipiv"] = np.uint64(dpiv.data_ptr()) + (4 * offs).astype(np.uint64)
     rec["sA"] = [dev.desc_addr(i) for i in range(len(shapes))]
     print("rowpe ps=%d batch=%d" % (ps, len(shapes)), flush=True)
-    bfa.call("drowpe", ps, len(shapes), to_device(rec).data_ptr())
+    drec = to_device(rec)
+    bfa.call("drowpe", ps, len(shapes), drec.data_ptr())
     torch.cuda.synchronize()
     for i in range(len(shapes)):
         oracle.fn("d", "rowpe")(ps, kmax[i], piv[i].ctypes.data, host.desc_addr(i))
@@ -124,8 +125,9 @@ def test_lu_solve_pipeline(gpu, ps):
     t["sB"] = p["sA"]
     t["sD"] = p["sA"]
     print("lu pipeline ps=%d batch=%d" % (ps, N), flush=True)
-    bfa.call("dgetrf_rp", ps, N, to_device(f).data_ptr(), (256, 256, 0))
-    bfa.call("drowpe", ps, N, to_device(p).data_ptr())
+    df, dp = to_device(f), to_device(p)      # records kept alive until the launches ran
+    bfa.call("dgetrf_rp", ps, N, df.data_ptr(), (256, 256, 0))
+    bfa.call("drowpe", ps, N, dp.data_ptr())
     dt = to_device(t)
     bfa.call("dtrsm_llnu", ps, N, dt.data_ptr(), (256, nrhs, 0))
     bfa.call("dtrsm_lunn", ps, N, dt.data_ptr(), (256, nrhs, 0))

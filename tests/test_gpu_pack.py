@@ -79,11 +79,58 @@ def test_pack_unpack_roundtrip_large(gpu):
     rec["m"], rec["n"], rec["lda"], rec["ai"], rec["aj"] = m, n, lda, ai, aj
     rec["A"], rec["sA"] = A.data_ptr(), dd.data_ptr()
     print("roundtrip 1024x700", flush=True)
-    bfa.call("dpack", 4, 1, to_device(rec).data_ptr(), (0, n, 0))
+    r1 = to_device(rec)
+    bfa.call("dpack", 4, 1, r1.data_ptr(), (0, n, 0))
     rec["A"] = B.data_ptr()
-    bfa.call("dunpack", 4, 1, to_device(rec).data_ptr(), (0, n, 0))
+    r2 = to_device(rec)
+    bfa.call("dunpack", 4, 1, r2.data_ptr(), (0, n, 0))
     torch.cuda.synchronize()
     a = A.cpu().numpy().reshape(n, lda).T
     b = B.cpu().numpy().reshape(n, lda).T
     np.testing.assert_array_equal(b[:m], a[:m])
     assert np.all(b[m:] == SENTINEL)
+
+
+def test_pm_dgecp(gpu):
+    """blasfeo_amd_pm_dgecp_batched: one batch mixing all four (psA, psB)
+    pairs, random sizes up to 200, offsets 0..9; every byte of the
+    destination buffer and the use_dA flags against the oracle."""
+    import torch
+    from pack_cases import pm_layout
+    rng = np.random.default_rng(850)
+    batch = 48
+    ms = rng.integers(1, 201, batch)
+    ns = rng.integers(1, 201, batch)
+    ais, ajs, bis, bjs = (rng.integers(0, 10, batch) for _ in range(4))
+    pa = rng.choice([4, 8], batch)
+    pb = rng.choice([4, 8], batch)
+    pss = np.concatenate([pa, pb])
+    rows = np.concatenate([ais + ms + 1, bis + ms + 2])
+    cols = np.concatenate([ajs + ns, bjs + ns + 1])
+    _, offs, nb = pm_layout(pss, rows, cols, 0)
+    init = rng.standard_normal(nb // 8)
+    g = torch.from_numpy(init.copy()).to("cuda")
+    dg = pm_layout(pss, rows, cols, g.data_ptr())[0]
+    dg["use_dA"] = 2
+    ddev = to_device(dg)
+    sz = bfa.PM_DMAT.itemsize
+
+    def recs(dbase):
+        r = np.zeros(batch, dtype=bfa.PM_DGECP_ARG)
+        r["m"], r["n"], r["ai"], r["aj"], r["bi"], r["bj"] = ms, ns, ais, ajs, bis, bjs
+        idx = np.arange(batch, dtype=np.uint64)
+        r["sA"] = np.uint64(dbase) + idx * sz
+        r["sB"] = np.uint64(dbase) + (idx + batch) * sz
+        return r
+    print("pm_dgecp batch=%d" % batch, flush=True)
+    drec = to_device(recs(ddev.data_ptr()))        # kept alive until the launch has run
+    bfa.pm_dgecp(batch, drec.data_ptr(), (0, int(ns.max()), 0))
+    torch.cuda.synchronize()
+    h = init.copy()
+    dh = pm_layout(pss, rows, cols, h.ctypes.data)[0]
+    dh["use_dA"] = 2
+    hrec = recs(dh.ctypes.data)
+    oracle.lib().or_pm_dgecp_batch(batch, hrec.ctypes.data)
+    np.testing.assert_array_equal(g.cpu().numpy(), h)
+    gd = ddev.cpu().numpy().view(bfa.PM_DMAT)
+    np.testing.assert_array_equal(gd["use_dA"], dh["use_dA"])

@@ -97,3 +97,39 @@ def test_pack_matches_host_library():
         oracle.fn("d", "pack")(ps, m, n, Af.ctypes.data, lda, d2.ctypes.data, ai, aj)
         np.testing.assert_array_equal(b1, b2)
     lib.blasfeo_amd_set_ps(4)
+
+
+def test_pm_dgecp_between_panel_sizes():
+    """or_pm_dgecp: every (psA, psB) pair against numpy layouts of the same
+    dense blocks, sentinels outside the block kept, use_dA cleared, and a
+    4 -> 8 -> 4 round trip returning the original bytes."""
+    from pack_cases import pm_layout
+    rng = np.random.default_rng(61)
+    for pa, pb in ((4, 4), (4, 8), (8, 4), (8, 8)):
+        m, n, ai, aj, bi, bj = 13, 9, 3, 2, 5, 1
+        A = rng.standard_normal((ai + m + 2, aj + n))
+        d, offs, nb = pm_layout([pa, pb], [ai + m + 2, bi + m + 1], [aj + n, bj + n + 3], 0)
+        buf = np.full(nb // 8, SENTINEL)
+        d, offs, nb = pm_layout([pa, pb], [ai + m + 2, bi + m + 1], [aj + n, bj + n + 3], buf.ctypes.data)
+        buf[offs[0] // 8: offs[0] // 8 + int(d["pm"][0] * d["cn"][0])] = bfa.pm_pack(
+            np.pad(A, ((0, int(d["pm"][0]) - A.shape[0]), (0, int(d["cn"][0]) - A.shape[1]))), pa, int(d["cn"][0]))
+        d["use_dA"][1] = 3
+        oracle.lib().or_pm_dgecp(m, n, d.ctypes.data, ai, aj, d.ctypes.data + bfa.PM_DMAT.itemsize, bi, bj)
+        assert d["use_dA"][1] == 0
+        E = np.full((int(d["pm"][1]), int(d["cn"][1])), SENTINEL)
+        E[bi:bi + m, bj:bj + n] = A[ai:ai + m, aj:aj + n]
+        got = buf[offs[1] // 8: offs[1] // 8 + E.size]
+        np.testing.assert_array_equal(got, bfa.pm_pack(E, pb, int(d["cn"][1])), err_msg="%d->%d" % (pa, pb))
+    # 4 -> 8 -> 4 round trip of a whole matrix
+    m, n = 21, 10
+    d, offs, nb = pm_layout([4, 8, 4], [m] * 3, [n] * 3, 0)
+    buf = np.zeros(nb // 8)
+    d, offs, nb = pm_layout([4, 8, 4], [m] * 3, [n] * 3, buf.ctypes.data)
+    ln = int(d["pm"][0] * d["cn"][0])
+    buf[:ln] = rng.standard_normal(ln)
+    sz = bfa.PM_DMAT.itemsize
+    oracle.lib().or_pm_dgecp(m, n, d.ctypes.data, 0, 0, d.ctypes.data + sz, 0, 0)
+    oracle.lib().or_pm_dgecp(m, n, d.ctypes.data + sz, 0, 0, d.ctypes.data + 2 * sz, 0, 0)
+    a0 = bfa.pm_unpack(buf[offs[0] // 8:], 4, int(d["cn"][0]), m, n)
+    a2 = bfa.pm_unpack(buf[offs[2] // 8:], 4, int(d["cn"][2]), m, n)
+    np.testing.assert_array_equal(a0, a2)

@@ -469,13 +469,44 @@ def run_pack(args, torch, bfa):
                     args.steps, args.warmup)
     ms_un = time_steps(torch, stream, lambda: bfa.call("dunpack", args.ps, batch, drec2.data_ptr(), hint, sh),
                        args.steps, args.warmup)
+    # the same blocks moved to the other panel size (blasfeo_pm_dmat descriptors)
+    ps2 = 8 if args.ps == 4 else 4
+    pmd = np.zeros(2 * batch, dtype=bfa.PM_DMAT)
+    hd0 = bfa.bulk_layout(np.float64, args.ps, nn, nn, pm.data_ptr())[0]
+    for f_ in ("mem", "pA", "dA", "m", "n", "pm", "cn", "use_dA", "memsize"):
+        pmd[f_][:batch] = hd0[f_]
+    pmd["ps"][:batch] = args.ps
+    _, _, nb2 = bfa.bulk_layout(np.float64, ps2, nn, nn, 0)
+    pm2 = torch.zeros(nb2 // 8, dtype=torch.float64, device="cuda")
+    hd2 = bfa.bulk_layout(np.float64, ps2, nn, nn, pm2.data_ptr())[0]
+    for f_ in ("mem", "pA", "dA", "m", "n", "pm", "cn", "use_dA", "memsize"):
+        pmd[f_][batch:] = hd2[f_]
+    pmd["ps"][batch:] = ps2
+    dpmd = to_device(pmd)
+    crec = np.zeros(batch, dtype=bfa.PM_DGECP_ARG)
+    crec["m"], crec["n"] = nn, nn
+    crec["sA"] = np.uint64(dpmd.data_ptr()) + np.arange(batch, dtype=np.uint64) * bfa.PM_DMAT.itemsize
+    crec["sB"] = crec["sA"] + np.uint64(batch * bfa.PM_DMAT.itemsize)
+    dcrec = to_device(crec)
+    ms_cv = time_steps(torch, stream, lambda: bfa.pm_dgecp(batch, dcrec.data_ptr(), hint, sh),
+                       args.steps, args.warmup)
     nf = nn.astype(np.float64)
     algo = float((16 * nf * nf).sum())
     achieved = algo / (ms * 1e-3) / 1e9
     value = batch / (ms * 1e-3)
-    log("pack: %.3f ms/step, %.4g matrices/s, %.0f GB/s; unpack %.3f ms, %.0f GB/s"
-        % (ms, value, achieved, ms_un, algo / (ms_un * 1e-3) / 1e9))
+    log("pack: %.3f ms/step, %.4g matrices/s, %.0f GB/s; unpack %.3f ms, %.0f GB/s; ps %d -> %d %.3f ms, %.0f GB/s"
+        % (ms, value, achieved, ms_un, algo / (ms_un * 1e-3) / 1e9, args.ps, ps2, ms_cv,
+           algo / (ms_cv * 1e-3) / 1e9))
     checks = {"roundtrip_bit_exact": bool(torch.equal(out_cm, cm))}
+    # the converted blocks hold the same values (first 64)
+    ok = True
+    g1, g2 = pm.cpu().numpy(), pm2.cpu().numpy()
+    for i in range(64):
+        n_ = int(nn[i])
+        a1 = bfa.pm_unpack(g1[(int(hd0["pA"][i]) - pm.data_ptr()) // 8:], args.ps, int(hd0["cn"][i]), n_, n_)
+        a2 = bfa.pm_unpack(g2[(int(hd2["pA"][i]) - pm2.data_ptr()) // 8:], ps2, int(hd2["cn"][i]), n_, n_)
+        ok = ok and np.array_equal(a1, a2)
+    checks["ps_convert_bit_exact_64"] = bool(ok)
     cpu = None
     if not args.no_cpu:
         n = 1024
@@ -505,7 +536,10 @@ def run_pack(args, torch, bfa):
            "vs_baseline": None, "dtype": "f64", "data": "synthetic, seeded (%d), N(0,1)" % args.seed,
            "config": {"workload": "dpack_batched", "batch": batch, "n": "U{16..256}", "ps": args.ps,
                       "unpack_ms_per_step": ms_un,
-                      "unpack_GBps": algo / (ms_un * 1e-3) / 1e9},
+                      "unpack_GBps": algo / (ms_un * 1e-3) / 1e9,
+                      "ps_convert": "%d -> %d (pm_dgecp)" % (args.ps, ps2),
+                      "ps_convert_ms_per_step": ms_cv,
+                      "ps_convert_GBps": algo / (ms_cv * 1e-3) / 1e9},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS,
                         "traffic": read_traffic("k_pack", "pack-16384"),

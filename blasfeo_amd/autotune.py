@@ -61,6 +61,7 @@ def reset():
         set_knob("ric_threads", c, 0)
     set_knob("gemm_grid_y", 0, 0)
     set_knob("potrf_bw", 0, 0)
+    set_knob("trsm_thin", 0, 0)
 
 
 def apply(tuning):

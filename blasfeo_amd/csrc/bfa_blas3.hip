@@ -6,6 +6,7 @@
 // so a batch of small instances uses one workgroup each and a batch of large
 // ones spreads its 64x64 tiles over several (tile engine: bfa_gemm_core.hpp).
 #include "bfa_gemm_core.hpp"
+#include "bfa_chol_core.hpp"
 #include "bfa_internal.hpp"
 
 namespace bfa {
@@ -331,6 +332,119 @@ __global__ __launch_bounds__(NT3) void k_trsm_left(const typename TrsmArg<T>::ty
 	}
 }
 
+// ---------------------------------------------------------------------------
+// The same four left-side solves for few right-hand sides (chosen when the hint
+// bounds n by 4) and m <= 256: one wavefront per instance, row i of X in lane
+// i % 64 (slot i / 64), four columns of X at a time in registers.  Column j of the solve is taken in order (top-down for llnn
+// and llnu, bottom-up for lltn and lunn): row j's values are scaled by the
+// reciprocal diagonal and broadcast with v_readlane, and every row still to
+// be solved drops its multiple of them.  No barriers; the elements of A for
+// the next column are loaded while the current one is applied.
+// ---------------------------------------------------------------------------
+constexpr int THIN_RHS = 4;
+constexpr int THIN_Q = 4;
+
+template <typename T, int PS, int MODE>
+__device__ __forceinline__ void thin_coef(const PM<T, PS> gA, int ai, int aj, int m, int j, T (&e)[THIN_Q])
+{
+	constexpr bool BACK = tl_back(MODE);
+	const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+	for (int q = 0; q < THIN_Q; q++)
+	{
+		const int i = lane + q * WAVE;
+		T v = T(0);
+		if (BACK ? (i < j) : (i > j && i < m))
+			v = MODE == TL_LLTN ? gA.at(ai + j, aj + i) : gA.at(ai + i, aj + j);
+		e[q] = v;
+	}
+}
+
+template <typename T, int PS, int MODE>
+__global__ __launch_bounds__(WAVE) void k_trsm_left_thin(const typename TrsmArg<T>::type *__restrict__ args)
+{
+	constexpr bool BACK = tl_back(MODE);
+	constexpr bool UNIT = MODE == TL_LLNU;
+	const typename TrsmArg<T>::type &a = args[blockIdx.x];
+	const int m = a.m;
+	if (m <= 0 || a.n <= 0 || m > THIN_Q * WAVE)
+		return;
+	const PM<T, PS> gA = view<T, PS>(a.sA);
+	const PM<T, PS> gB = view<T, PS>(a.sB);
+	PM<T, PS> gD = view<T, PS>(a.sD);
+	const int ai = a.ai, aj = a.aj;
+	const int lane = threadIdx.x;
+	const bool cached = !UNIT && ai == 0 && aj == 0 && a.sD != a.sA && a.sA->use_dA >= m;
+	const T *dA = a.sA->dA;
+	// right-hand sides in groups of THIN_RHS (one group when n <= 4)
+	for (int c0 = 0; c0 < a.n; c0 += THIN_RHS)
+	{
+	const int n = min(THIN_RHS, a.n - c0);
+	T x[THIN_Q][THIN_RHS];
+#pragma unroll
+	for (int q = 0; q < THIN_Q; q++)
+#pragma unroll
+		for (int c = 0; c < THIN_RHS; c++)
+		{
+			const int i = lane + q * WAVE;
+			x[q][c] = (i < m && c < n) ? a.alpha * gB.at(a.bi + i, a.bj + c0 + c) : T(0);
+		}
+	T e[THIN_Q];
+	thin_coef<T, PS, MODE>(gA, ai, aj, m, BACK ? m - 1 : 0, e);
+	// the solve's columns j in order, grouped by the register slot qb = j / 64
+	// that holds row j (unrolled, so every register index is a constant)
+#pragma unroll
+	for (int qq = 0; qq < THIN_Q; qq++)
+	{
+		const int qb = BACK ? THIN_Q - 1 - qq : qq;
+		for (int ll = 0; ll < WAVE; ll++)
+		{
+			const int lj = BACK ? WAVE - 1 - ll : ll;
+			const int j = qb * WAVE + lj;
+			if (j >= m)
+			{
+				if (BACK)
+					continue;
+				break;
+			}
+			const int jn = BACK ? j - 1 : j + 1;          // the next column
+			const bool more = BACK ? jn >= 0 : jn < m;
+			T en[THIN_Q];
+			if (more)
+				thin_coef<T, PS, MODE>(gA, ai, aj, m, jn, en);
+			const T d = UNIT ? T(1) : (cached ? dA[j] : T(1) / gA.at(ai + j, aj + j));
+			T xj[THIN_RHS];
+#pragma unroll
+			for (int c = 0; c < THIN_RHS; c++)
+				xj[c] = readlane(x[qb][c], lj) * d;
+#pragma unroll
+			for (int q = 0; q < THIN_Q; q++)
+			{
+				const int i = lane + q * WAVE;
+#pragma unroll
+				for (int c = 0; c < THIN_RHS; c++)
+					x[q][c] = (i == j) ? xj[c] : x[q][c] - e[q] * xj[c];
+			}
+			if (more)
+			{
+#pragma unroll
+				for (int q = 0; q < THIN_Q; q++)
+					e[q] = en[q];
+			}
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < THIN_Q; q++)
+#pragma unroll
+		for (int c = 0; c < THIN_RHS; c++)
+		{
+			const int i = lane + q * WAVE;
+			if (i < m && c < n)
+				gD.at(a.di + i, a.dj + c0 + c) = x[q][c];
+		}
+	}
+}
+
 // Cache update after trsm_rltn / trsm_llnn / trsm_lltn, in the reference's
 // order (blasfeo_hp_pm/d_blas3_lib4.c:4157-4178; blasfeo_ref/x_blas3_ref.c:
 // 466-480): invalidate D's cache, then refresh A's reciprocal diagonal unless
@@ -458,9 +572,12 @@ static int launch_trsm(int ps, int batch, const typename TrsmArg<T>::type *args,
 }
 
 template <typename T, int PS, int MODE>
-static int launch_trsml_ps(int batch, const typename TrsmArg<T>::type *args, dim3 grid, size_t shm, int mmax, hipStream_t st, const char *name)
+static int launch_trsml_ps(int batch, const typename TrsmArg<T>::type *args, dim3 grid, size_t shm, int mmax, bool thin, hipStream_t st, const char *name)
 {
-	hipLaunchKernelGGL((k_trsm_left<T, PS, MODE>), grid, dim3(NT3), shm, st, args, mmax);
+	if (thin)
+		hipLaunchKernelGGL((k_trsm_left_thin<T, PS, MODE>), dim3(batch), dim3(WAVE), 0, st, args);
+	else
+		hipLaunchKernelGGL((k_trsm_left<T, PS, MODE>), grid, dim3(NT3), shm, st, args, mmax);
 	int rc = check_launch(name);
 	if (rc)
 		return rc;
@@ -483,11 +600,21 @@ static int launch_trsml(int ps, int batch, const typename TrsmArg<T>::type *args
 	const size_t shm = sizeof(T) * ((size_t)LNC * (mpad + 1) + mpad);
 	const int gy = hint->max_n > 0 ? min(64, (hint->max_n + LNC - 1) / LNC) : 1;
 	dim3 grid(batch, gy);
+	// few right-hand sides (hint->max_n <= 4), m <= 256 and a batch that fills
+	// the GPU by itself (16 instances per CU): one wavefront per instance with X
+	// in registers.  Smaller batches keep the four-wave block sweep, whose
+	// per-instance chain is shorter (2048 solves of n = 256 with 4 right-hand
+	// sides: 1.34 ms against 1.48 ms one wavefront each).
+	// The "trsm_thin" knob forces it on (1) or off (2) where it applies.
+	const int cus = num_cus() > 0 ? num_cus() : 256;
+	const int knob = tune_value(TUNE_TRSM_THIN, 0);
+	const bool thin = sizeof(T) == 8 && hint->max_n > 0 && hint->max_n <= THIN_RHS && mmax <= THIN_Q * WAVE
+	                  && (knob == 1 || (knob == 0 && batch >= 16 * cus));
 	if (ps == 4)
-		return launch_trsml_ps<T, 4, MODE>(batch, args, grid, shm, mmax, st, name);
+		return launch_trsml_ps<T, 4, MODE>(batch, args, grid, shm, mmax, thin, st, name);
 	if (ps == 8)
-		return launch_trsml_ps<T, 8, MODE>(batch, args, grid, shm, mmax, st, name);
-	return launch_trsml_ps<T, 16, MODE>(batch, args, grid, shm, mmax, st, name);
+		return launch_trsml_ps<T, 8, MODE>(batch, args, grid, shm, mmax, thin, st, name);
+	return launch_trsml_ps<T, 16, MODE>(batch, args, grid, shm, mmax, thin, st, name);
 }
 
 } // namespace bfa

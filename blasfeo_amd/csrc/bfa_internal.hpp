@@ -23,7 +23,7 @@ int lds_check(const void *fn, size_t dyn, const char *what);
 void *workspace(size_t bytes, hipStream_t stream);
 
 // Launch-shape knobs (blasfeo_amd_tune_set); 0 means the built-in choice.
-enum TuneKnob { TUNE_RIC_PER_CU = 0, TUNE_RIC_THREADS = 1, TUNE_GEMM_GRID_Y = 2, TUNE_POTRF_BW = 3 };
+enum TuneKnob { TUNE_RIC_PER_CU = 0, TUNE_RIC_THREADS = 1, TUNE_GEMM_GRID_Y = 2, TUNE_POTRF_BW = 3, TUNE_TRSM_THIN = 4 };
 int tune_value(TuneKnob knob, int index);
 
 } // namespace bfa

@@ -14,6 +14,7 @@ std::atomic<int> g_ric_per_cu[NCLASS];
 std::atomic<int> g_ric_threads[NCLASS];
 std::atomic<int> g_gemm_grid_y{0};
 std::atomic<int> g_potrf_bw{0};
+std::atomic<int> g_trsm_thin{0};
 
 std::atomic<int> *slot(const char *name, int index)
 {
@@ -27,6 +28,8 @@ std::atomic<int> *slot(const char *name, int index)
 		return index == 0 ? &g_gemm_grid_y : nullptr;
 	if (!std::strcmp(name, "potrf_bw"))
 		return index == 0 ? &g_potrf_bw : nullptr;
+	if (!std::strcmp(name, "trsm_thin"))
+		return index == 0 ? &g_trsm_thin : nullptr;
 	return nullptr;
 }
 
@@ -39,6 +42,9 @@ bool valid(const char *name, int index, int value)
 	// potrf_bw: tile columns per block panel of potrf_l for 128 < n <= 256
 	if (!std::strcmp(name, "potrf_bw"))
 		return value == 1 || value == 2 || value == 4;
+	// trsm_thin: 1 = always the one-wavefront left solve when it applies, 2 = never
+	if (!std::strcmp(name, "trsm_thin"))
+		return value == 1 || value == 2;
 	// ric_threads: one wave only for the classes whose stages have nu+nx <= 64
 	return value == 128 || value == 256 || (value == 64 && index <= 4);
 }
@@ -56,6 +62,8 @@ int tune_value(TuneKnob knob, int index)
 		return g_gemm_grid_y.load();
 	case TUNE_POTRF_BW:
 		return g_potrf_bw.load();
+	case TUNE_TRSM_THIN:
+		return g_trsm_thin.load();
 	}
 	return 0;
 }

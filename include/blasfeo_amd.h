@@ -130,7 +130,11 @@ const char *blasfeo_amd_last_error(void);
  *   "gemm_grid_y", index 0: most workgroups per gemm/syrk/trmm instance (1..64);
  *   "potrf_bw",    index 0: tile columns per block panel of potrf_l for
  *                  128 < n <= 256 (1, 2 or 4: 16-, 32- or 64-column panels,
- *                  four, two or one instance per CU; built-in 1).
+ *                  four, two or one instance per CU; built-in 1);
+ *   "trsm_thin",   index 0: left-side solves (llnn, lltn, llnu, lunn) with
+ *                  hint->max_n <= 4 and m <= 256 on one wavefront per instance:
+ *                  1 always, 2 never; built-in: when the batch has at least
+ *                  16 instances per CU.
  * blasfeo_amd_tune_set returns BLASFEO_AMD_OK or BLASFEO_AMD_EINVAL;
  * blasfeo_amd_tune_get returns the value (0 = built-in) or BLASFEO_AMD_EINVAL. */
 int blasfeo_amd_tune_set(const char *name, int index, int value);

@@ -177,8 +177,12 @@ def test_tuning_knobs_without_gpu():
     assert at.get_knob("potrf_bw", 0) == 2
     with pytest.raises(RuntimeError):
         at.set_knob("potrf_bw", 0, 3)          # 1, 2 or 4 tile columns per block panel
+    at.set_knob("trsm_thin", 0, 2)
+    assert at.get_knob("trsm_thin", 0) == 2
+    with pytest.raises(RuntimeError):
+        at.set_knob("trsm_thin", 0, 3)         # 0 built-in, 1 always, 2 never
     at.reset()
-    assert at.get_knob("potrf_bw", 0) == 0
+    assert at.get_knob("potrf_bw", 0) == 0 and at.get_knob("trsm_thin", 0) == 0
     assert at.get_knob("ric_per_cu", 3) == 0 and at.get_knob("ric_threads", 2) == 0
     # footprints of the resident form: largest bench stage (nu+nx = 80, nx = 64)
     # fits two per CU, nu+nx = 64, nx = 48 three per CU

@@ -136,18 +136,21 @@ def test_trxm(gpu, prec, ps, op):
     assert (gd["use_dA"] == rd["use_dA"]).all()
 
 
+@pytest.mark.parametrize("thin", [False, True])
 @pytest.mark.parametrize("ps", [4, 8])
 @pytest.mark.parametrize("op", ["trsm_llnn", "trsm_lltn"])
-def test_trsm_left(gpu, ps, op):
+def test_trsm_left(gpu, ps, op, thin):
     """D = alpha A^{-1} B and D = alpha A^{-T} B (A lower m x m), with offsets,
     the reciprocal-diagonal cache of A (valid, stale and offset cases), the
-    right-hand sides spread over several column blocks, and in-place D = B."""
-    rng = np.random.default_rng(400 + ps)
+    right-hand sides spread over several column blocks, and in-place D = B.
+    thin: at most 4 right-hand sides and m <= 256 (hint (256, 4)), the
+    one-wavefront kernel."""
+    rng = np.random.default_rng(400 + ps + 7 * thin)
     b = Batch("d", ps, seed=ps + 11)
     recs = []
     for r in range(18):
-        m = int(rng.integers(1, 260)) if r else 256
-        n = int(rng.integers(1, 40)) if r else 4
+        m = int(rng.integers(1, 257 if thin else 260)) if r else 256
+        n = (int(rng.integers(1, 5)) if thin else int(rng.integers(1, 40))) if r else 4
         ai, bi, di = (int(x) for x in rng.integers(0, 2 * ps, 3))
         aj, bj, dj = (int(x) for x in rng.integers(0, 3, 3))
         if r % 3 == 0:
@@ -170,6 +173,12 @@ def test_trsm_left(gpu, ps, op):
         f(ps, r["m"], r["n"], r["alpha"], h.desc_addr(r["sA"]), r["ai"], r["aj"],
           h.desc_addr(r["sB"]), r["bi"], r["bj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
 
-    g, gd, rr, rd = run_both(b, buf, bfa.DTRSM_ARG, recs, "d" + op, ocall, (260, 40, 0))
+    # thin: the one-wavefront kernel is forced (the built-in choice takes it only
+    # for batches of at least 16 instances per CU)
+    assert bfa.lib().blasfeo_amd_tune_set(b"trsm_thin", 0, 1 if thin else 0) == 0
+    try:
+        g, gd, rr, rd = run_both(b, buf, bfa.DTRSM_ARG, recs, "d" + op, ocall, (256, 4, 0) if thin else (260, 40, 0))
+    finally:
+        bfa.lib().blasfeo_amd_tune_set(b"trsm_thin", 0, 0)
     compare_buffers(b, g, rr, "d%s ps=%d" % (op, ps))
     assert (gd["use_dA"] == rd["use_dA"]).all()

@@ -15,15 +15,17 @@ from helpers import Batch, run_both, compare_buffers
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("thin", [False, True])
 @pytest.mark.parametrize("ps", [4, 8])
 @pytest.mark.parametrize("op", ["trsm_llnu", "trsm_lunn"])
-def test_trsm_unit_and_upper(gpu, ps, op):
-    rng = np.random.default_rng(900 + ps + (op == "trsm_lunn"))
+def test_trsm_unit_and_upper(gpu, ps, op, thin):
+    """thin: at most 4 right-hand sides, m <= 256 (the one-wavefront kernel)."""
+    rng = np.random.default_rng(900 + ps + (op == "trsm_lunn") + 5 * thin)
     b = Batch("d", ps, seed=ps + 21)
     recs = []
     for r in range(18):
-        m = int(rng.integers(1, 260)) if r else 256
-        n = int(rng.integers(1, 40)) if r else 4
+        m = int(rng.integers(1, 257 if thin else 260)) if r else 256
+        n = (int(rng.integers(1, 5)) if thin else int(rng.integers(1, 40))) if r else 4
         ai, bi, di = (int(x) for x in rng.integers(0, 2 * ps, 3))
         aj, bj, dj = (int(x) for x in rng.integers(0, 3, 3))
         if r % 3 == 0:
@@ -51,7 +53,13 @@ def test_trsm_unit_and_upper(gpu, ps, op):
           h.desc_addr(r["sB"]), r["bi"], r["bj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
 
     print("%s ps=%d batch=%d" % (op, ps, len(recs)), flush=True)
-    g, gd, rr, rd = run_both(b, buf, bfa.DTRSM_ARG, recs, "d" + op, ocall, (260, 40, 0))
+    # thin: the one-wavefront kernel is forced (the built-in choice takes it only
+    # for batches of at least 16 instances per CU)
+    assert bfa.lib().blasfeo_amd_tune_set(b"trsm_thin", 0, 1 if thin else 0) == 0
+    try:
+        g, gd, rr, rd = run_both(b, buf, bfa.DTRSM_ARG, recs, "d" + op, ocall, (256, 4, 0) if thin else (260, 40, 0))
+    finally:
+        bfa.lib().blasfeo_amd_tune_set(b"trsm_thin", 0, 0)
     compare_buffers(b, g, rr, "d%s ps=%d" % (op, ps))
     assert (gd["use_dA"] == rd["use_dA"]).all()
 

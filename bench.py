@@ -150,8 +150,10 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="riccati: pick launch shapes and the panel size with blasfeo_amd.autotune on "
                          "this workload before timing")
-    ap.add_argument("--graph", action="store_true",
-                    help="riccati: time replays of the batched sweep captured in a HIP graph")
+    ap.add_argument("--eager", action="store_true",
+                    help="riccati: time eager calls instead of replays of the batched sweep captured "
+                         "in a HIP graph (the default; every replay runs the whole call: plan, "
+                         "queue reset and class launches)")
     ap.add_argument("--workload", default="riccati",
                     choices=("riccati", "ric_trs", "pack", "getrs", "potrf32", "gemm_syrk", "getrf", "sgetrf", "getrf_np",
                              "posv256"),
@@ -223,6 +225,7 @@ def run_riccati(args, torch, bfa):
     torch.cuda.synchronize()
 
     step = lambda: dev.run(sh)
+    args.graph = not args.eager
     if args.graph:
         graph = dev.capture(stream)
         torch.cuda.synchronize()

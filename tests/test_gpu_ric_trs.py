@@ -47,12 +47,14 @@ def _sizes(rng, n):
     return nxs, nus
 
 
+@pytest.mark.parametrize("N", [5, 1])
 @pytest.mark.parametrize("ps", [4, 8])
-def test_riccati_trs_parity(gpu, ps):
+def test_riccati_trs_parity(gpu, ps, N):
+    """N = 1: the stage-0 forms only (full trsv, all of ux[0] negated)."""
     import torch
-    rng = np.random.default_rng(700 + ps)
+    rng = np.random.default_rng(700 + ps + 10 * N)
     nxs, nus = _sizes(rng, 20)
-    rb = RiccatiBatch(nxs, nus, N=5, ps=ps)
+    rb = RiccatiBatch(nxs, nus, N=N, ps=ps)
     inp = rb.fill_inputs(rng)
     dev = DeviceRiccati(rb, inp)
     dev.run()
@@ -129,7 +131,8 @@ def test_riccati_trs_forms_reciprocals(gpu):
             assert np.all(np.isfinite(gd))
 
 
-def test_riccati_trs_full_horizon_accuracy(gpu):
+@pytest.mark.parametrize("ps", [4, 8])
+def test_riccati_trs_full_horizon_accuracy(gpu, ps):
     """N = 20 on the headline size distribution: the recursion amplifies
     rounding over the horizon (the oracle's own KKT backward error reaches
     1e5 eps on weakly controllable instances), so every instance's KKT
@@ -137,10 +140,10 @@ def test_riccati_trs_full_horizon_accuracy(gpu):
     (+100 eps), and the median at most a few hundred eps."""
     import torch
     from blasfeo_amd.riccati import trs_backward_error
-    rng = np.random.default_rng(740)
+    rng = np.random.default_rng(740 + ps)
     nxs = rng.integers(8, 65, 48)
     nus = rng.integers(2, 17, 48)
-    rb = RiccatiBatch(nxs, nus, N=20, ps=4)
+    rb = RiccatiBatch(nxs, nus, N=20, ps=ps)
     inp = rb.fill_inputs(rng)
     dev = DeviceRiccati(rb, inp)
     dev.run()

@@ -682,9 +682,9 @@ def run_getrs(args, torch, bfa):
                       "n": "U{16..256}", "nrhs": nrhs, "ps": args.ps,
                       "timing": "step time minus a same-size device copy that restores B"},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": algo,
-                        "flops_per_launch": flops,
-                        "kernel": "k_rowpe + k_trsm_left<double,%d,TL_LLNU> + k_trsm_left<double,%d,TL_LUNN>"
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("getrs", "getrs-16384"),
+                        "algorithmic_bytes": algo, "flops_per_launch": flops,
+                        "kernel": "k_rowpe + k_trsm_left_thin<double,%d,TL_LLNU> + k_trsm_left_thin<double,%d,TL_LUNN>"
                                   % (args.ps, args.ps)},
            "cpu_baseline": cpu, "checks": checks}
     print(json.dumps(out), flush=True)

@@ -420,6 +420,12 @@ __global__ __launch_bounds__(WAVE) void k_trsm_left_thin(const typename TrsmArg<
 #pragma unroll
 			for (int q = 0; q < THIN_Q; q++)
 			{
+				// slots whose rows are all solved already (before qb on the way
+				// down, after it on the way up) or all past m are left alone
+				if (BACK ? q > qb : q < qb)
+					continue;
+				if (q * WAVE >= m)
+					continue;
 				const int i = lane + q * WAVE;
 #pragma unroll
 				for (int c = 0; c < THIN_RHS; c++)

@@ -4,9 +4,9 @@
 //
 // The swap sequence is composed first into one permutation in LDS (perm[i] =
 // the original row that ends in row i; one thread replays the swaps on the
-// index array, with ipiv staged in LDS by all threads), then every column
-// block of the matrix is gathered through it: all threads read the source
-// rows of a block into LDS, then write the rows back in place.  Matrices
+// index array, with ipiv staged in LDS by all threads), then every column of
+// the matrix is gathered through it: all threads read the column's source
+// rows into LDS, then write the rows back in place.  Matrices
 // taller than RP_MAXM rows replay the swaps per column instead (each thread
 // owns columns, so no synchronisation is needed there).
 #include "bfa_common.hpp"
@@ -14,9 +14,12 @@
 
 namespace bfa {
 
-constexpr int RP_NT = 256;
+// one wavefront per instance and a one-column gather buffer (6 KB of LDS), so
+// that many instances share a CU: the composition of the swaps is one thread's
+// serial chain, overlapped across instances
+constexpr int RP_NT = 64;
 constexpr int RP_MAXM = 512;     // rows of the LDS form
-constexpr int RP_CB = 4;         // columns gathered per pass
+constexpr int RP_CB = 1;         // columns gathered per pass
 
 template <int PS>
 __global__ __launch_bounds__(RP_NT) void k_rowpe(const blasfeo_amd_drowpe_arg *__restrict__ args)

@@ -391,6 +391,11 @@ __global__ __launch_bounds__(WAVE) void k_trsm_left_thin(const typename TrsmArg<
 		}
 	T e[THIN_Q];
 	thin_coef<T, PS, MODE>(gA, ai, aj, m, BACK ? m - 1 : 0, e);
+	// the reciprocal diagonal of column j, also loaded one column ahead
+	auto recip = [&](int jj) -> T {
+		return UNIT ? T(1) : (cached ? dA[jj] : T(1) / gA.at(ai + jj, aj + jj));
+	};
+	T d = recip(BACK ? m - 1 : 0);
 	// the solve's columns j in order, grouped by the register slot qb = j / 64
 	// that holds row j (unrolled, so every register index is a constant)
 #pragma unroll
@@ -410,9 +415,12 @@ __global__ __launch_bounds__(WAVE) void k_trsm_left_thin(const typename TrsmArg<
 			const int jn = BACK ? j - 1 : j + 1;          // the next column
 			const bool more = BACK ? jn >= 0 : jn < m;
 			T en[THIN_Q];
+			T dn = T(1);
 			if (more)
+			{
 				thin_coef<T, PS, MODE>(gA, ai, aj, m, jn, en);
-			const T d = UNIT ? T(1) : (cached ? dA[j] : T(1) / gA.at(ai + j, aj + j));
+				dn = recip(jn);
+			}
 			T xj[THIN_RHS];
 #pragma unroll
 			for (int c = 0; c < THIN_RHS; c++)
@@ -436,6 +444,7 @@ __global__ __launch_bounds__(WAVE) void k_trsm_left_thin(const typename TrsmArg<
 #pragma unroll
 				for (int q = 0; q < THIN_Q; q++)
 					e[q] = en[q];
+				d = dn;
 			}
 		}
 	}

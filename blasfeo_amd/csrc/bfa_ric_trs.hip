@@ -69,6 +69,26 @@ __device__ __forceinline__ double trs_trmv_n(const PmRead<PS> A, int r0, const d
 	return acc;
 }
 
+// L(i, j) for the lane's rows i = lane, lane + 64 with j < i < n0, else 0
+template <int PS>
+__device__ __forceinline__ void trs_coef(const PmRead<PS> L, int j, int n0, double (&e)[2])
+{
+	const int lane = threadIdx.x;
+#pragma unroll
+	for (int q = 0; q < 2; q++)
+	{
+		const int i = lane + q * WAVE;
+		e[q] = (i > j && i < n0) ? L(i, j) : 0.0;
+	}
+}
+
+// 1 / L(j, j): the cached reciprocal, or formed as the reference's trsv does
+template <int PS>
+__device__ __forceinline__ double trs_recip(const PmRead<PS> L, const double *dA, bool cached, int j)
+{
+	return cached ? dA[j] : 1.0 / L(j, j);
+}
+
 template <int PS>
 __global__ __launch_bounds__(WAVE) void k_ric_trs(const blasfeo_amd_riccati_trs_arg *__restrict__ args, int batch)
 {
@@ -152,9 +172,23 @@ __global__ __launch_bounds__(WAVE) void k_ric_trs(const blasfeo_amd_riccati_trs_
 		const PmRead<PS> L0 = pm_read<PS>(*Ld);
 		double *dA = Ld->dA;
 		const bool cached = Ld->use_dA == 1;
+		// the column's coefficients and reciprocal are loaded one column ahead,
+		// off the serial chain of v_readlane broadcasts
+		double e[2] = {0.0, 0.0}, d = 0.0;
+		if (ns > 0)
+		{
+			trs_coef<PS>(L0, 0, n0, e);
+			d = trs_recip<PS>(L0, dA, cached, 0);
+		}
 		for (int j = 0; j < ns; j++)
 		{
-			const double d = cached ? dA[j] : 1.0 / L0(j, j);
+			double en[2] = {0.0, 0.0}, dn = 0.0;
+			const bool more = j + 1 < ns;
+			if (more)
+			{
+				trs_coef<PS>(L0, j + 1, n0, en);
+				dn = trs_recip<PS>(L0, dA, cached, j + 1);
+			}
 			const double zj = readlane(j < WAVE ? z[0] : z[1], j & (WAVE - 1)) * d;
 			if (!cached && lane == 0)
 				dA[j] = d;
@@ -165,7 +199,13 @@ __global__ __launch_bounds__(WAVE) void k_ric_trs(const blasfeo_amd_riccati_trs_
 				if (i == j)
 					z[q] = zj;
 				else if (i > j && i < n0)
-					z[q] -= L0(i, j) * zj;
+					z[q] -= e[q] * zj;
+			}
+			if (more)
+			{
+				e[0] = en[0];
+				e[1] = en[1];
+				d = dn;
 			}
 		}
 		if (!cached && ns > 0 && n0 > 0 && lane == 0)
@@ -213,24 +253,36 @@ __global__ __launch_bounds__(WAVE) void k_ric_trs(const blasfeo_amd_riccati_trs_
 		const PmRead<PS> L0 = pm_read<PS>(*Ld);
 		double *dA = Ld->dA;
 		const bool cached = Ld->use_dA == 1;
+		double e[2] = {0.0, 0.0}, d = 0.0;
+		if (ns > 0)
+		{
+			trs_coef<PS>(L0, ns - 1, n0, e);
+			d = trs_recip<PS>(L0, dA, cached, ns - 1);
+		}
 		for (int j = ns - 1; j >= 0; j--)
 		{
-			double part = 0.0;
-#pragma unroll
-			for (int q = 0; q < 2; q++)
+			double en[2] = {0.0, 0.0}, dn = 0.0;
+			const bool more = j > 0;
+			if (more)
 			{
-				const int i = lane + q * WAVE;
-				if (i > j && i < n0)
-					part += L0(i, j) * z[q];
+				trs_coef<PS>(L0, j - 1, n0, en);
+				dn = trs_recip<PS>(L0, dA, cached, j - 1);
 			}
+			// e is zero outside rows j+1 .. n0-1
+			const double part = e[0] * z[0] + e[1] * z[1];
 			const double s = readlane(wave_sum_d(part), WAVE - 1);
-			const double d = cached ? dA[j] : 1.0 / L0(j, j);
 			if (!cached && lane == 0)
 				dA[j] = d;
 #pragma unroll
 			for (int q = 0; q < 2; q++)
 				if (lane + q * WAVE == j)
 					z[q] = (z[q] - s) * d;
+			if (more)
+			{
+				e[0] = en[0];
+				e[1] = en[1];
+				d = dn;
+			}
 		}
 		if (!cached && n0 > 0 && lane == 0)
 			a.L[k].use_dA = 1;

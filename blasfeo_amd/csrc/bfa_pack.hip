@@ -122,6 +122,75 @@ __global__ __launch_bounds__(PK_NT) void k_pack(const blasfeo_amd_dpack_arg *__r
 	}
 }
 
+// pack with the next tile's column-major loads issued before the current
+// tile's panel-major stores (the tiles of a workgroup in one flat sequence:
+// column tiles blockIdx.y, blockIdx.y + gridDim.y, ..., every row tile of each)
+template <int PS>
+__global__ __launch_bounds__(PK_NT) void k_pack_pipe(const blasfeo_amd_dpack_arg *__restrict__ args)
+{
+	__shared__ double T[PK_C * PK_LD];
+	const blasfeo_amd_dpack_arg a = args[blockIdx.x];
+	const int R = a.m, C = a.n;
+	blasfeo_dmat *sA = a.sA;
+	const int t = threadIdx.x;
+	if (blockIdx.y == 0 && t == 0)
+		sA->use_dA = 0;
+	if (R <= 0 || C <= 0)
+		return;
+	double *pA = sA->pA;
+	const int cn = sA->cn;
+	const int lda = a.lda;
+	const double *A = a.A;
+	const int base = a.ai - (a.ai & (PS - 1));
+	const int span = (a.ai & (PS - 1)) + R;
+	const int nrt = (span + PK_R - 1) / PK_R;
+	const int nct = (C + PK_C - 1) / PK_C;
+	const int my_ct = nct > (int)blockIdx.y ? (nct - (int)blockIdx.y + (int)gridDim.y - 1) / (int)gridDim.y : 0;
+	const int ntile = my_ct * nrt;
+	double v[PK_R * PK_C / PK_NT];
+	auto load = [&](int k) {
+		const int c0 = ((int)blockIdx.y + (k / nrt) * (int)gridDim.y) * PK_C;
+		const int r0 = (k % nrt) * PK_R;
+		const int nc = min(PK_C, C - c0);
+		const int b0 = base + r0 - a.ai;
+#pragma unroll
+		for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
+		{
+			const int rr = t & (PK_R - 1), cc = (t >> 6) + 4 * q;
+			const int br = b0 + rr;
+			v[q] = (cc < nc && br >= 0 && br < R) ? A[br + (size_t)(c0 + cc) * lda] : 0.0;
+		}
+	};
+	if (ntile > 0)
+		load(0);
+	for (int k = 0; k < ntile; k++)
+	{
+#pragma unroll
+		for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
+			T[((t >> 6) + 4 * q) * PK_LD + (t & (PK_R - 1))] = v[q];
+		__syncthreads();
+		if (k + 1 < ntile)
+			load(k + 1);
+		const int c0 = ((int)blockIdx.y + (k / nrt) * (int)gridDim.y) * PK_C;
+		const int r0 = (k % nrt) * PK_R;
+		const int nc = min(PK_C, C - c0);
+		const int b0 = base + r0 - a.ai;
+#pragma unroll
+		for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
+		{
+			const int e = t + q * PK_NT;
+			const int r = e & (PS - 1);
+			const int cc = (e / PS) & (PK_C - 1);
+			const int p = e / (PS * PK_C);
+			const int rr = p * PS + r;
+			const int br = b0 + rr;
+			if (cc < nc && br >= 0 && br < R)
+				pA[(size_t)(base + r0 + p * PS) * cn + (size_t)(a.aj + c0 + cc) * PS + r] = T[cc * PK_LD + rr];
+		}
+		__syncthreads();
+	}
+}
+
 // ---------------------------------------------------------------------------
 // blasfeo_dgecp (auxiliary/d_aux_lib4.c:3006) between panel-major matrices of
 // any two panel sizes: B(bi+i, bj+j) = A(ai+i, aj+j), sB->use_dA = 0.  The
@@ -202,7 +271,14 @@ static int launch_pack(int ps, int batch, const blasfeo_amd_dpack_arg *args, con
 	const int nmax = hint && hint->max_n > 0 ? hint->max_n : PK_C;
 	const int gy = max(1, min(16, (nmax + PK_C - 1) / PK_C));
 	const dim3 grid(batch, gy);
-	if (ps == 4)
+	if (DIR == PK_PACK)
+	{
+		if (ps == 4)
+			hipLaunchKernelGGL((k_pack_pipe<4>), grid, dim3(PK_NT), 0, st, args);
+		else
+			hipLaunchKernelGGL((k_pack_pipe<8>), grid, dim3(PK_NT), 0, st, args);
+	}
+	else if (ps == 4)
 		hipLaunchKernelGGL((k_pack<4, DIR>), grid, dim3(PK_NT), 0, st, args);
 	else
 		hipLaunchKernelGGL((k_pack<8, DIR>), grid, dim3(PK_NT), 0, st, args);

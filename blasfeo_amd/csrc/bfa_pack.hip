@@ -191,6 +191,78 @@ __global__ __launch_bounds__(PK_NT) void k_pack_pipe(const blasfeo_amd_dpack_arg
 	}
 }
 
+// unpack with the same flat tile sequence: the next tile's panel-major loads
+// are issued before the current tile's column-major stores
+template <int PS>
+__global__ __launch_bounds__(PK_NT) void k_unpack_pipe(const blasfeo_amd_dpack_arg *__restrict__ args)
+{
+	__shared__ double T[PK_C * PK_LD];
+	const blasfeo_amd_dpack_arg a = args[blockIdx.x];
+	const int R = a.m, C = a.n;
+	if (R <= 0 || C <= 0)
+		return;
+	const blasfeo_dmat *sA = a.sA;
+	const double *pA = sA->pA;
+	const int cn = sA->cn;
+	const int lda = a.lda;
+	double *A = a.A;
+	const int t = threadIdx.x;
+	const int base = a.ai - (a.ai & (PS - 1));
+	const int span = (a.ai & (PS - 1)) + R;
+	const int nrt = (span + PK_R - 1) / PK_R;
+	const int nct = (C + PK_C - 1) / PK_C;
+	const int my_ct = nct > (int)blockIdx.y ? (nct - (int)blockIdx.y + (int)gridDim.y - 1) / (int)gridDim.y : 0;
+	const int ntile = my_ct * nrt;
+	double v[PK_R * PK_C / PK_NT];
+	auto load = [&](int k) {
+		const int c0 = ((int)blockIdx.y + (k / nrt) * (int)gridDim.y) * PK_C;
+		const int r0 = (k % nrt) * PK_R;
+		const int nc = min(PK_C, C - c0);
+		const int b0 = base + r0 - a.ai;
+#pragma unroll
+		for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
+		{
+			const int e = t + q * PK_NT;
+			const int r = e & (PS - 1);
+			const int cc = (e / PS) & (PK_C - 1);
+			const int p = e / (PS * PK_C);
+			const int br = b0 + p * PS + r;
+			v[q] = (cc < nc && br >= 0 && br < R)
+			           ? pA[(size_t)(base + r0 + p * PS) * cn + (size_t)(a.aj + c0 + cc) * PS + r] : 0.0;
+		}
+	};
+	if (ntile > 0)
+		load(0);
+	for (int k = 0; k < ntile; k++)
+	{
+#pragma unroll
+		for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
+		{
+			const int e = t + q * PK_NT;
+			const int r = e & (PS - 1);
+			const int cc = (e / PS) & (PK_C - 1);
+			const int p = e / (PS * PK_C);
+			T[cc * PK_LD + p * PS + r] = v[q];
+		}
+		__syncthreads();
+		if (k + 1 < ntile)
+			load(k + 1);
+		const int c0 = ((int)blockIdx.y + (k / nrt) * (int)gridDim.y) * PK_C;
+		const int r0 = (k % nrt) * PK_R;
+		const int nc = min(PK_C, C - c0);
+		const int b0 = base + r0 - a.ai;
+#pragma unroll
+		for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
+		{
+			const int rr = t & (PK_R - 1), cc = (t >> 6) + 4 * q;
+			const int br = b0 + rr;
+			if (cc < nc && br >= 0 && br < R)
+				A[br + (size_t)(c0 + cc) * lda] = T[cc * PK_LD + rr];
+		}
+		__syncthreads();
+	}
+}
+
 // ---------------------------------------------------------------------------
 // blasfeo_dgecp (auxiliary/d_aux_lib4.c:3006) between panel-major matrices of
 // any two panel sizes: B(bi+i, bj+j) = A(ai+i, aj+j), sB->use_dA = 0.  The
@@ -277,6 +349,13 @@ static int launch_pack(int ps, int batch, const blasfeo_amd_dpack_arg *args, con
 			hipLaunchKernelGGL((k_pack_pipe<4>), grid, dim3(PK_NT), 0, st, args);
 		else
 			hipLaunchKernelGGL((k_pack_pipe<8>), grid, dim3(PK_NT), 0, st, args);
+	}
+	else if (DIR == PK_UNPACK)
+	{
+		if (ps == 4)
+			hipLaunchKernelGGL((k_unpack_pipe<4>), grid, dim3(PK_NT), 0, st, args);
+		else
+			hipLaunchKernelGGL((k_unpack_pipe<8>), grid, dim3(PK_NT), 0, st, args);
 	}
 	else if (ps == 4)
 		hipLaunchKernelGGL((k_pack<4, DIR>), grid, dim3(PK_NT), 0, st, args);

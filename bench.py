@@ -546,7 +546,7 @@ def run_pack(args, torch, bfa):
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS,
                         "traffic": read_traffic("k_pack", "pack-16384"),
-                        "algorithmic_bytes": algo, "kernel": "k_pack<%d,PK_PACK>" % args.ps},
+                        "algorithmic_bytes": algo, "kernel": "k_pack_pipe<%d>" % args.ps},
            "cpu_baseline": cpu, "checks": checks}
     print(json.dumps(out), flush=True)
 

@@ -268,11 +268,14 @@ __global__ __launch_bounds__(PK_NT) void k_unpack_pipe(const blasfeo_amd_dpack_a
 // any two panel sizes: B(bi+i, bj+j) = A(ai+i, aj+j), sB->use_dA = 0.  The
 // panel sizes come from the blasfeo_pm_dmat descriptors; the tiles follow
 // B's panel grid (writes in 16 * psB contiguous runs), A is read one column
-// of a 64-row tile at a time (runs of psA contiguous doubles).
+// of a 64-row tile at a time (runs of psA contiguous doubles), the next
+// tile's loads issued before the current tile's stores.
 // ---------------------------------------------------------------------------
 template <int PA, int PB>
 __device__ __forceinline__ void pm_gecp_tiles(const blasfeo_amd_pm_dgecp_arg &a, double *T)
 {
+	// the flat tile sequence of k_pack_pipe on B's panel grid, the next tile's
+	// loads from A issued before the current tile's stores to B
 	const int m = a.m, n = a.n;
 	const double *pa = a.sA->pA;
 	double *pb = a.sB->pA;
@@ -280,38 +283,52 @@ __device__ __forceinline__ void pm_gecp_tiles(const blasfeo_amd_pm_dgecp_arg &a,
 	const int base = a.bi - (a.bi & (PB - 1));
 	const int span = (a.bi & (PB - 1)) + m;
 	const int t = threadIdx.x;
-	for (int c0 = blockIdx.y * PK_C; c0 < n; c0 += gridDim.y * PK_C)
-	{
+	const int nrt = (span + PK_R - 1) / PK_R;
+	const int nct = (n + PK_C - 1) / PK_C;
+	const int my_ct = nct > (int)blockIdx.y ? (nct - (int)blockIdx.y + (int)gridDim.y - 1) / (int)gridDim.y : 0;
+	const int ntile = my_ct * nrt;
+	double v[PK_R * PK_C / PK_NT];
+	auto load = [&](int k) {
+		const int c0 = ((int)blockIdx.y + (k / nrt) * (int)gridDim.y) * PK_C;
+		const int r0 = (k % nrt) * PK_R;
 		const int nc = min(PK_C, n - c0);
-		for (int r0 = 0; r0 < span; r0 += PK_R)
+		const int b0 = base + r0 - a.bi;
+#pragma unroll
+		for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
 		{
-			const int b0 = base + r0 - a.bi;
-#pragma unroll
-			for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
-			{
-				const int rr = t & (PK_R - 1), cc = (t >> 6) + 4 * q;
-				const int br = b0 + rr;
-				if (cc < nc && br >= 0 && br < m)
-				{
-					const int i = a.ai + br;
-					T[cc * PK_LD + rr] = pa[(size_t)(i & ~(PA - 1)) * cna + (size_t)(a.aj + c0 + cc) * PA + (i & (PA - 1))];
-				}
-			}
-			__syncthreads();
-#pragma unroll
-			for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
-			{
-				const int e = t + q * PK_NT;
-				const int r = e & (PB - 1);
-				const int cc = (e / PB) & (PK_C - 1);
-				const int p = e / (PB * PK_C);
-				const int rr = p * PB + r;
-				const int br = b0 + rr;
-				if (cc < nc && br >= 0 && br < m)
-					pb[(size_t)(base + r0 + p * PB) * cnb + (size_t)(a.bj + c0 + cc) * PB + r] = T[cc * PK_LD + rr];
-			}
-			__syncthreads();
+			const int rr = t & (PK_R - 1), cc = (t >> 6) + 4 * q;
+			const int br = b0 + rr;
+			const int i = a.ai + br;
+			v[q] = (cc < nc && br >= 0 && br < m)
+			           ? pa[(size_t)(i & ~(PA - 1)) * cna + (size_t)(a.aj + c0 + cc) * PA + (i & (PA - 1))] : 0.0;
 		}
+	};
+	if (ntile > 0)
+		load(0);
+	for (int k = 0; k < ntile; k++)
+	{
+#pragma unroll
+		for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
+			T[((t >> 6) + 4 * q) * PK_LD + (t & (PK_R - 1))] = v[q];
+		__syncthreads();
+		if (k + 1 < ntile)
+			load(k + 1);
+		const int c0 = ((int)blockIdx.y + (k / nrt) * (int)gridDim.y) * PK_C;
+		const int r0 = (k % nrt) * PK_R;
+		const int nc = min(PK_C, n - c0);
+		const int b0 = base + r0 - a.bi;
+#pragma unroll
+		for (int q = 0; q < PK_R * PK_C / PK_NT; q++)
+		{
+			const int e = t + q * PK_NT;
+			const int r = e & (PB - 1);
+			const int cc = (e / PB) & (PK_C - 1);
+			const int p = e / (PB * PK_C);
+			const int br = b0 + p * PB + r;
+			if (cc < nc && br >= 0 && br < m)
+				pb[(size_t)(base + r0 + p * PB) * cnb + (size_t)(a.bj + c0 + cc) * PB + r] = T[cc * PK_LD + p * PB + r];
+		}
+		__syncthreads();
 	}
 }
 

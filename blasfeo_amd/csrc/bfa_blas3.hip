@@ -55,7 +55,10 @@ __global__ __launch_bounds__(NT3) void k_trmm_rlnn(const typename TrsmArg<T>::ty
 constexpr int TRSM_NMAX = 256;
 constexpr int LDX = TM + 1;
 
-template <typename T, int PS>
+// DA_GIVEN: the reciprocal diagonal is read from sA->dA whatever use_dA says
+// (the trapezoid Cholesky, whose diagonal block was just factored into the
+// same descriptor with its reciprocals stored there).
+template <typename T, int PS, bool DA_GIVEN = false>
 __global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::type *__restrict__ args, int nmax)
 {
 	typedef typename Vec4<T>::type V4;
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::ty
 
 	// reciprocal diagonal: cached copy when valid at launch, else 1/A_jj
 	// (an output aliasing A invalidates the cache first, as in the reference)
-	const bool cached = (a.ai == 0 && a.aj == 0 && a.sD != a.sA && a.sA->use_dA >= n);
+	const bool cached = DA_GIVEN || (a.ai == 0 && a.aj == 0 && a.sD != a.sA && a.sA->use_dA >= n);
 	for (int j = threadIdx.x; j < n; j += NT3)
 		inv_s[j] = cached ? a.sA->dA[j] : T(1) / gA.at(a.ai + j, a.aj + j);
 
@@ -435,9 +438,13 @@ __global__ __launch_bounds__(WAVE) void k_trsm_left_thin(const typename TrsmArg<
 				if (q * WAVE >= m)
 					continue;
 				const int i = lane + q * WAVE;
+				// only the rows still to be solved take the update (a solved
+				// row is never touched again, so an Inf or NaN in xj cannot
+				// reach it through a zero coefficient)
+				const bool open = BACK ? i < j : i > j;
 #pragma unroll
 				for (int c = 0; c < THIN_RHS; c++)
-					x[q][c] = (i == j) ? xj[c] : x[q][c] - e[q] * xj[c];
+					x[q][c] = (i == j) ? xj[c] : (open ? x[q][c] - e[q] * xj[c] : x[q][c]);
 			}
 			if (more)
 			{
@@ -565,8 +572,17 @@ static int launch_trsm_ps(int batch, const typename TrsmArg<T>::type *args, dim3
 	return check_launch(name);
 }
 
+// the solve of the rows below the diagonal block in the trapezoid Cholesky:
+// reciprocals from sA->dA, no cache update (the caller sets use_dA)
+template <int PS>
+static int launch_trsm_given(int batch, const blasfeo_amd_dtrsm_arg *args, dim3 grid, size_t shm, int nmax, hipStream_t st, const char *name)
+{
+	hipLaunchKernelGGL((k_trsm_rltn<double, PS, true>), grid, dim3(NT3), shm, st, args, nmax);
+	return check_launch(name);
+}
+
 template <typename T>
-static int launch_trsm(int ps, int batch, const typename TrsmArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
+static int launch_trsm(int ps, int batch, const typename TrsmArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name, bool given = false)
 {
 	int rc = check_common(ps, batch, args, sizeof(T) == 4, name);
 	if (rc || batch == 0)
@@ -579,6 +595,14 @@ static int launch_trsm(int ps, int batch, const typename TrsmArg<T>::type *args,
 	const size_t shm = (size_t)npad * LDX * sizeof(T);
 	const int gy = hint->max_m > 0 ? min(64, (hint->max_m + TM - 1) / TM) : 1;
 	dim3 grid(batch, gy);
+	if (given)
+	{
+		if (sizeof(T) != 8 || (ps != 4 && ps != 8))
+			return set_error(BLASFEO_AMD_EINVAL, "%s: unsupported panel size %d", name, ps);
+		const blasfeo_amd_dtrsm_arg *d = reinterpret_cast<const blasfeo_amd_dtrsm_arg *>(args);
+		return ps == 4 ? launch_trsm_given<4>(batch, d, grid, shm, npad, st, name)
+		               : launch_trsm_given<8>(batch, d, grid, shm, npad, st, name);
+	}
 	if (ps == 4)
 		return launch_trsm_ps<T, 4>(batch, args, grid, shm, npad, st, name);
 	if (ps == 8)
@@ -630,6 +654,11 @@ static int launch_trsml(int ps, int batch, const typename TrsmArg<T>::type *args
 	if (ps == 8)
 		return launch_trsml_ps<T, 8, MODE>(batch, args, grid, shm, mmax, thin, st, name);
 	return launch_trsml_ps<T, 16, MODE>(batch, args, grid, shm, mmax, thin, st, name);
+}
+
+int trsm_rltn_given_dA(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
+{
+	return launch_trsm<double>(ps, batch, args, hint, st, name, true);
 }
 
 } // namespace bfa

@@ -92,6 +92,10 @@ struct WsBlock
 };
 static std::mutex g_ws_mutex;
 static std::map<std::pair<int, hipStream_t>, WsBlock> g_ws;
+// Blocks that a larger request replaced.  They are never freed: a HIP graph
+// captured while one was current keeps its address in its kernel arguments,
+// and replaying that graph must still find the memory there.
+static std::vector<void *> g_ws_retired;
 
 void *workspace(size_t bytes, hipStream_t stream)
 {
@@ -102,21 +106,13 @@ void *workspace(size_t bytes, hipStream_t stream)
 	WsBlock &b = g_ws[std::make_pair(dev, stream)];
 	if (b.bytes >= bytes)
 		return b.ptr;
-	if (b.ptr)
-	{
-		// the previous block may still be in use by work queued on the stream
-		if (hipStreamSynchronize(stream) != hipSuccess)
-			return nullptr;
-		(void)hipFree(b.ptr);
-		b.ptr = nullptr;
-		b.bytes = 0;
-	}
 	size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
-	if (hipMalloc(&b.ptr, want) != hipSuccess)
-	{
-		b.ptr = nullptr;
+	void *p = nullptr;
+	if (hipMalloc(&p, want) != hipSuccess)
 		return nullptr;
-	}
+	if (b.ptr)
+		g_ws_retired.push_back(b.ptr);
+	b.ptr = p;
 	b.bytes = want;
 	return b.ptr;
 }
@@ -331,6 +327,10 @@ void blasfeo_pm_create_dmat(int ps, int m, int n, struct blasfeo_pm_dmat *sA, vo
 // device matrices are converted through a host copy of that range.
 static void pm_transfer(int m, int n, double *A, int lda, struct blasfeo_dmat *sA, int ai, int aj, int dir, bool tran)
 {
+	// pack_tran clears the cache even for an empty block, pack and unpack
+	// return first (auxiliary/d_aux_lib4.c:2090, :1609, :2238)
+	if (dir == 0 && tran)
+		sA->use_dA = 0;
 	if (m <= 0 || n <= 0)
 		return;
 	const int ps = g_ps;

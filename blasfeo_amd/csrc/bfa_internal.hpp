@@ -17,10 +17,17 @@ int num_cus();
 int lds_check(const void *fn, size_t dyn, const char *what);
 
 // Device scratch owned by the library for one (device, stream) pair, at least
-// `bytes` long.  Allocated on first use and grown when a larger size is asked
-// for (the only allocation a batched entry may make); launches on one stream
-// are ordered, so reusing it across calls on that stream is safe.
+// `bytes` long.  Allocated on first use and replaced by a larger block when a
+// larger size is asked for (the only allocation a batched entry may make);
+// replaced blocks stay allocated until the process ends, so a HIP graph that
+// captured an earlier block still finds it.  Launches on one stream are
+// ordered, so reusing a block across calls on that stream is safe.
 void *workspace(size_t bytes, hipStream_t stream);
+
+// D = B * A^{-T} (trsm_rltn, alpha from the records) with the reciprocal
+// diagonal of A read from sA->dA whatever its use_dA, and no cache update:
+// the rows below the diagonal block of a trapezoid Cholesky (hint->max_n <= 256).
+int trsm_rltn_given_dA(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name);
 
 // Launch-shape knobs (blasfeo_amd_tune_set); 0 means the built-in choice.
 enum TuneKnob { TUNE_RIC_PER_CU = 0, TUNE_RIC_THREADS = 1, TUNE_GEMM_GRID_Y = 2, TUNE_POTRF_BW = 3, TUNE_TRSM_THIN = 4 };

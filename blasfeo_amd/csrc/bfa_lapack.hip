@@ -28,8 +28,9 @@ template <> struct FactArg<float> { typedef blasfeo_amd_sfact_arg type; };
 template <typename T> struct SPArg;
 template <> struct SPArg<double> { typedef blasfeo_amd_dsyrk_dpotrf_arg type; };
 
-// mode 0: potrf_l (use_dA = m), 1: potrf_l_mn (use_dA = 1)
-template <typename T, int PS, int MODE>
+// potrf_l for n > 256: left-looking over 16-column LDS panels at full height,
+// the earlier panels read back from D (chol_instance)
+template <typename T, int PS>
 __global__ __launch_bounds__(NTL) void k_potrf(const typename FactArg<T>::type *__restrict__ args, int mmax)
 {
 	extern __shared__ unsigned char smem_raw[];
@@ -38,17 +39,14 @@ __global__ __launch_bounds__(NTL) void k_potrf(const typename FactArg<T>::type *
 	T *P = reinterpret_cast<T *>(smem_raw);
 	const typename FactArg<T>::type &a = args[blockIdx.x];
 	const int m = a.m;
-	int n = MODE == 0 ? a.m : a.n;
-	if (m <= 0 || n <= 0 || m > mmax)
+	if (m <= 0 || m > mmax)
 		return;
-	if (n > m)
-		n = m;
 	const int LDP = (mmax + 15) / 16 * 16 + 1;
 	PM<T, PS> gD = view<T, PS>(a.sD);
-	chol_instance<T, PS>(P, LDP, L11, invs, m, n, 0, gD, 0, 0, gD, 0, 0,
+	chol_instance<T, PS>(P, LDP, L11, invs, m, m, 0, gD, 0, 0, gD, 0, 0,
 		view<T, PS>(a.sC), a.ci, a.cj, gD, a.di, a.dj, a.sD->dA);
 	if (threadIdx.x == 0)
-		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? (MODE == 0 ? m : 1) : 0;
+		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? m : 0;
 }
 
 // potrf_l (MN = false) and potrf_l_mn (MN = true) with the whole m x n lower
@@ -425,28 +423,6 @@ __global__ __launch_bounds__(NTH) void k_potrf_tiles(const typename FactArg<T>::
 }
 
 
-template <typename T, int PS>
-__global__ __launch_bounds__(NTL) void k_syrk_potrf(const typename SPArg<T>::type *__restrict__ args, int mmax)
-{
-	extern __shared__ unsigned char smem_raw[];
-	__shared__ T L11[PW * (PW + 1)];
-	__shared__ T invs[PW];
-	T *P = reinterpret_cast<T *>(smem_raw);
-	const typename SPArg<T>::type &a = args[blockIdx.x];
-	const int m = a.m;
-	int n = a.n;
-	if (m <= 0 || n <= 0 || m > mmax)
-		return;
-	if (n > m)
-		n = m;
-	const int LDP = (mmax + 15) / 16 * 16 + 1;
-	chol_instance<T, PS>(P, LDP, L11, invs, m, n, a.k, view<T, PS>(a.sA), a.ai, a.aj,
-		view<T, PS>(a.sB), a.bi, a.bj, view<T, PS>(a.sC), a.ci, a.cj,
-		view<T, PS>(a.sD), a.di, a.dj, a.sD->dA);
-	if (threadIdx.x == 0)
-		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? 1 : 0;
-}
-
 // ---------------------------------------------------------------------------
 // LU with partial row pivoting (blasfeo_ref/x_lapack_ref.c:1156-1536 semantics);
 // PIV = false: LU without pivoting, getrf_np (x_lapack_ref.c:880-1150): no
@@ -784,6 +760,114 @@ static int fact_check(int ps, int batch, const void *args, bool is_float, const 
 	return BLASFEO_AMD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// potrf_l_mn and syrk_dpotrf_ln(_mn) for m > 128: the m x n lower trapezoid
+// as the n x n Cholesky of its top block (potrf_l, whose kernels store the
+// reciprocal diagonal in D's dA) followed by the rows below,
+// L21 = C21 L11^{-T}, solved with those reciprocals (trsm_rltn reading dA);
+// syrk_dpotrf first forms C + A B^T in D (syrk_ln_mn) and then factors D in
+// place.  The per-instance records of the steps are built on the device into
+// the stream's workspace, so the call stays asynchronous.  This is the
+// reference's own decomposition of the trapezoid
+// (blasfeo_ref/x_lapack_ref.c:185-326: column j of the rows below is the
+// pivot column's update times 1/L_jj).
+// ---------------------------------------------------------------------------
+constexpr int MN_NMAX = 256;
+
+__global__ void k_mn_split(const blasfeo_amd_dfact_arg *__restrict__ args, int batch,
+	blasfeo_amd_dfact_arg *top, blasfeo_amd_dtrsm_arg *rows)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= batch)
+		return;
+	const blasfeo_amd_dfact_arg a = args[i];
+	const int m = a.m;
+	const int n = a.n < m ? a.n : m;
+	blasfeo_amd_dfact_arg t = a;
+	blasfeo_amd_dtrsm_arg r;
+	const bool live = m > 0 && n > 0;
+	t.m = t.n = live ? n : 0;
+	t.ipiv = nullptr;
+	r.m = live ? m - n : 0;
+	r.n = n;
+	r.ai = a.di; r.aj = a.dj;
+	r.bi = a.ci + n; r.bj = a.cj;
+	r.di = a.di + n; r.dj = a.dj;
+	r.alpha = 1.0;
+	r.sA = a.sD; r.sB = a.sC; r.sD = a.sD;
+	top[i] = t;
+	rows[i] = r;
+}
+
+// use_dA of the trapezoid factorisations (blasfeo_hp_pm/d_lapack_lib4.c:1185,1545,1882)
+__global__ void k_mn_finish(const blasfeo_amd_dfact_arg *__restrict__ args, int batch)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= batch)
+		return;
+	const blasfeo_amd_dfact_arg &a = args[i];
+	if (a.m > 0 && a.n > 0)
+		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? 1 : 0;
+}
+
+// syrk_dpotrf records -> syrk_ln_mn records (D = C + A B^T on the trapezoid)
+// and trapezoid Cholesky records (in place on D)
+__global__ void k_sp_split(const blasfeo_amd_dsyrk_dpotrf_arg *__restrict__ args, int batch,
+	blasfeo_amd_dgemm_arg *upd, blasfeo_amd_dfact_arg *fact)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= batch)
+		return;
+	const blasfeo_amd_dsyrk_dpotrf_arg a = args[i];
+	const int m = a.m;
+	const int n = a.n < m ? a.n : m;
+	const bool live = m > 0 && n > 0 && a.k >= 0;
+	blasfeo_amd_dgemm_arg g;
+	g.m = live ? m : 0; g.n = n; g.k = a.k;
+	g.ai = a.ai; g.aj = a.aj; g.bi = a.bi; g.bj = a.bj;
+	g.ci = a.ci; g.cj = a.cj; g.di = a.di; g.dj = a.dj;
+	g.alpha = 1.0; g.beta = 1.0;
+	g.sA = a.sA; g.sB = a.sB; g.sC = a.sC; g.sD = a.sD;
+	blasfeo_amd_dfact_arg f;
+	f.m = live ? m : 0; f.n = n;
+	f.ci = a.di; f.cj = a.dj; f.di = a.di; f.dj = a.dj;
+	f.sC = a.sD; f.sD = a.sD;
+	f.ipiv = nullptr;
+	upd[i] = g;
+	fact[i] = f;
+}
+
+static int mn_nmax(const blasfeo_amd_hint *hint, int mmax)
+{
+	return (hint && hint->max_n > 0 && hint->max_n < mmax) ? hint->max_n : mmax;
+}
+
+static int launch_potrf_mn_split(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, int mmax, hipStream_t st, const char *name)
+{
+	const int nmax = mn_nmax(hint, mmax);
+	if (nmax > MN_NMAX)
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: n up to %d supported when m > 128, got a bound of %d", name, MN_NMAX, nmax);
+	const size_t tb = ((size_t)batch * sizeof(blasfeo_amd_dfact_arg) + 255) / 256 * 256;
+	unsigned char *ws = (unsigned char *)workspace(tb + (size_t)batch * sizeof(blasfeo_amd_dtrsm_arg), st);
+	if (!ws)
+		return set_error(BLASFEO_AMD_ENOMEM, "%s: workspace allocation failed", name);
+	blasfeo_amd_dfact_arg *top = (blasfeo_amd_dfact_arg *)ws;
+	blasfeo_amd_dtrsm_arg *rows = (blasfeo_amd_dtrsm_arg *)(ws + tb);
+	const int nb = (batch + 255) / 256;
+	hipLaunchKernelGGL(k_mn_split, dim3(nb), dim3(256), 0, st, args, batch, top, rows);
+	int rc = check_launch(name);
+	if (rc)
+		return rc;
+	blasfeo_amd_hint ht = {nmax, nmax, 0};
+	if ((rc = blasfeo_amd_dpotrf_l_batched(ps, batch, top, &ht, st)))
+		return rc;
+	blasfeo_amd_hint hr = {mmax, nmax, 0};
+	if ((rc = trsm_rltn_given_dA(ps, batch, rows, &hr, st, name)))
+		return rc;
+	hipLaunchKernelGGL(k_mn_finish, dim3(nb), dim3(256), 0, st, args, batch);
+	return check_launch(name);
+}
+
 template <typename T, int MODE>
 static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
 {
@@ -818,7 +902,7 @@ static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args
 		return check_launch(name);
 	}
 	if (MODE == 1)
-		return set_error(BLASFEO_AMD_ETOOBIG, "%s: m up to 128 supported (the LDS-resident form); got a bound of %d", name, mmax);
+		return launch_potrf_mn_split(ps, batch, (const blasfeo_amd_dfact_arg *)args, hint, mmax, st, name);
 	if (MODE == 0 && mmax <= PT_NMAX)
 	{
 		// block panels of 16 columns (four 4-wave instances per CU, whose
@@ -862,11 +946,11 @@ static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args
 	}
 	const size_t shm = sizeof(T) * PW * ((mmax + 15) / 16 * 16 + 1);
 	if (ps == 4)
-		hipLaunchKernelGGL((k_potrf<T, 4, MODE>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+		hipLaunchKernelGGL((k_potrf<T, 4>), dim3(batch), dim3(NTL), shm, st, args, mmax);
 	else if (ps == 8)
-		hipLaunchKernelGGL((k_potrf<T, 8, MODE>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+		hipLaunchKernelGGL((k_potrf<T, 8>), dim3(batch), dim3(NTL), shm, st, args, mmax);
 	else
-		hipLaunchKernelGGL((k_potrf<T, 16, MODE>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+		hipLaunchKernelGGL((k_potrf<T, 16>), dim3(batch), dim3(NTL), shm, st, args, mmax);
 	return check_launch(name);
 }
 
@@ -898,7 +982,40 @@ static int launch_syrk_potrf(int ps, int batch, const blasfeo_amd_dsyrk_dpotrf_a
 	if ((rc = fact_mmax_arg<double>(hint, name, &mmax)))
 		return rc;
 	if (mmax > 128)
-		return set_error(BLASFEO_AMD_ETOOBIG, "%s: m up to 128 supported (the LDS-resident form); got a bound of %d", name, mmax);
+	{
+		const int nmax = mn_nmax(hint, mmax);
+		if (nmax > MN_NMAX)
+			return set_error(BLASFEO_AMD_ETOOBIG, "%s: n up to %d supported when m > 128, got a bound of %d", name, MN_NMAX, nmax);
+		const size_t gb = ((size_t)batch * sizeof(blasfeo_amd_dgemm_arg) + 255) / 256 * 256;
+		const size_t fb = ((size_t)batch * sizeof(blasfeo_amd_dfact_arg) + 255) / 256 * 256;
+		const size_t tb = fb;
+		// layout: [gemm records][fact records][potrf_l_mn split: top records][rows]
+		unsigned char *ws = (unsigned char *)workspace(gb + fb + tb + (size_t)batch * sizeof(blasfeo_amd_dtrsm_arg), st);
+		if (!ws)
+			return set_error(BLASFEO_AMD_ENOMEM, "%s: workspace allocation failed", name);
+		blasfeo_amd_dgemm_arg *upd = (blasfeo_amd_dgemm_arg *)ws;
+		blasfeo_amd_dfact_arg *fact = (blasfeo_amd_dfact_arg *)(ws + gb);
+		blasfeo_amd_dfact_arg *top = (blasfeo_amd_dfact_arg *)(ws + gb + fb);
+		blasfeo_amd_dtrsm_arg *rows = (blasfeo_amd_dtrsm_arg *)(ws + gb + fb + tb);
+		const int nb = (batch + 255) / 256;
+		hipLaunchKernelGGL(k_sp_split, dim3(nb), dim3(256), 0, st, args, batch, upd, fact);
+		if ((rc = check_launch(name)))
+			return rc;
+		blasfeo_amd_hint hg = {mmax, nmax, hint ? hint->max_k : 0};
+		if ((rc = blasfeo_amd_dsyrk_ln_mn_batched(ps, batch, upd, &hg, st)))
+			return rc;
+		hipLaunchKernelGGL(k_mn_split, dim3(nb), dim3(256), 0, st, fact, batch, top, rows);
+		if ((rc = check_launch(name)))
+			return rc;
+		blasfeo_amd_hint ht = {nmax, nmax, 0};
+		if ((rc = blasfeo_amd_dpotrf_l_batched(ps, batch, top, &ht, st)))
+			return rc;
+		blasfeo_amd_hint hr = {mmax, nmax, 0};
+		if ((rc = trsm_rltn_given_dA(ps, batch, rows, &hr, st, name)))
+			return rc;
+		hipLaunchKernelGGL(k_mn_finish, dim3(nb), dim3(256), 0, st, fact, batch);
+		return check_launch(name);
+	}
 	const int np = (mmax + 15) / 16 * 16;
 	const size_t shm = sizeof(double) * ((size_t)np * (np + 1) + np);
 	if (np <= 32)
@@ -926,10 +1043,8 @@ extern "C" {
 
 int blasfeo_amd_dpotrf_l_batched(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
 { return launch_potrf<double, 0>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dpotrf_l_batched"); }
-// The LDS-resident forms (m <= 128).  The global-memory forms of these two
-// routines (k_potrf<T,PS,1> and k_syrk_potrf, 16-column panels through global
-// memory) each ended a GPU run in a memory fault whose cause was not found, so
-// they are not launched: larger bounds return BLASFEO_AMD_ETOOBIG.
+// m <= 128: the LDS-resident forms; larger m: the trapezoid as potrf_l of
+// the top block plus the solve of the rows below (see launch_potrf_mn_split).
 int blasfeo_amd_dpotrf_l_mn_batched(int ps, int batch, const blasfeo_amd_dfact_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)
 { return launch_potrf<double, 1>(ps, batch, args, hint, (hipStream_t)s, "blasfeo_amd_dpotrf_l_mn_batched"); }
 int blasfeo_amd_dsyrk_dpotrf_ln_batched(int ps, int batch, const blasfeo_amd_dsyrk_dpotrf_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t s)

@@ -30,9 +30,15 @@ __global__ __launch_bounds__(PK_NT) void k_pack(const blasfeo_amd_dpack_arg *__r
 	// the panel-major block: R rows x C columns at (ai, aj)
 	const int R = DIR == PK_TRAN ? a.n : a.m;
 	const int C = DIR == PK_TRAN ? a.m : a.n;
-	if (R <= 0 || C <= 0)
-		return;
 	blasfeo_dmat *sA = a.sA;
+	// pack_tran clears the cache even for an empty block, pack returns first
+	// (auxiliary/d_aux_lib4.c:2090, :1609)
+	if (R <= 0 || C <= 0)
+	{
+		if (DIR == PK_TRAN && blockIdx.y == 0 && threadIdx.x == 0)
+			sA->use_dA = 0;
+		return;
+	}
 	double *pA = sA->pA;
 	const int cn = sA->cn;
 	const int lda = a.lda;
@@ -133,10 +139,11 @@ __global__ __launch_bounds__(PK_NT) void k_pack_pipe(const blasfeo_amd_dpack_arg
 	const int R = a.m, C = a.n;
 	blasfeo_dmat *sA = a.sA;
 	const int t = threadIdx.x;
-	if (blockIdx.y == 0 && t == 0)
-		sA->use_dA = 0;
+	// an empty block leaves the cache alone (auxiliary/d_aux_lib4.c:1609)
 	if (R <= 0 || C <= 0)
 		return;
+	if (blockIdx.y == 0 && t == 0)
+		sA->use_dA = 0;
 	double *pA = sA->pA;
 	const int cn = sA->cn;
 	const int lda = a.lda;

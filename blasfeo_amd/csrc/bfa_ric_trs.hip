@@ -268,8 +268,10 @@ __global__ __launch_bounds__(WAVE) void k_ric_trs(const blasfeo_amd_riccati_trs_
 				trs_coef<PS>(L0, j - 1, n0, en);
 				dn = trs_recip<PS>(L0, dA, cached, j - 1);
 			}
-			// e is zero outside rows j+1 .. n0-1
-			const double part = e[0] * z[0] + e[1] * z[1];
+			// only rows j+1 .. n0-1 contribute (e is zero elsewhere, but a zero
+			// coefficient times an Inf or NaN in an unsolved row is NaN)
+			const int i0 = lane, i1 = lane + WAVE;
+			const double part = (i0 > j && i0 < n0 ? e[0] * z[0] : 0.0) + (i1 > j && i1 < n0 ? e[1] * z[1] : 0.0);
 			const double s = readlane(wave_sum_d(part), WAVE - 1);
 			if (!cached && lane == 0)
 				dA[j] = d;

@@ -311,3 +311,32 @@ class HostRiccati:
 
     def L(self, i, k):
         return self.rb.out.get(self.outbuf, self.rb.i_L[i][k])
+
+
+def stage_backward_errors(rb, inp, gbuf, instances):
+    """Per-stage backward error of computed Riccati factors, in units of
+    n_k eps: ||M_k - L_k L_k^T||_F / (||M_k||_F n_k eps) with
+    M_N = Q_N and M_k = RSQ + W W^T, W = BAbt L_{k+1}(nu:, nu:), rebuilt from
+    the computed L_{k+1} itself, so that each stage is judged on its own
+    inputs (the recursion amplifies forward differences over the horizon).
+    gbuf: the output buffer (host bytes) of a RiccatiBatch, inp its input
+    buffer.  Returns an array (len(instances), N + 1); column k is stage k."""
+    eps = np.finfo(np.float64).eps
+    out = []
+    for i in instances:
+        nx, nu = int(rb.nxs[i]), int(rb.nus[i])
+        babt = rb.inp.get(inp, rb.i_babt[i])
+        rsq = np.tril(rb.inp.get(inp, rb.i_rsq[i]))
+        rsq = rsq + np.tril(rsq, -1).T
+        qn = np.tril(rb.inp.get(inp, rb.i_qn[i]))
+        qn = qn + np.tril(qn, -1).T
+        L = [np.tril(rb.out.get(gbuf, rb.i_L[i][k])) for k in range(rb.N + 1)]
+        row = np.empty(rb.N + 1)
+        row[rb.N] = np.linalg.norm(qn - L[rb.N] @ L[rb.N].T) / (np.linalg.norm(qn) * max(nx, 1) * eps)
+        for k in range(rb.N):
+            nu1 = nu if k + 1 < rb.N else 0
+            W = babt @ L[k + 1][nu1:, nu1:]
+            M = rsq + W @ W.T
+            row[k] = np.linalg.norm(M - L[k] @ L[k].T) / (np.linalg.norm(M) * (nx + nu) * eps)
+        out.append(row)
+    return np.array(out)

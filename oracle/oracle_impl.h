@@ -425,6 +425,8 @@ void PFX(ric_trf)(int ps, int N, const int *nx, const int *nu, MAT *BAbt, MAT *R
 /* sA(ai+i, aj+j) = A[i + j*lda]; use_dA = 0; auxiliary/d_aux_lib4.c:1607-1640 */
 void PFX(pack)(int ps, int m, int n, REAL *A, int lda, MAT *sA, int ai, int aj)
 	{
+	if(m <= 0 || n <= 0)
+		return;
 	sA->use_dA = 0;
 	for(int j = 0; j < n; j++)
 		for(int i = 0; i < m; i++)

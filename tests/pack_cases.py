@@ -9,11 +9,18 @@ import blasfeo_amd as bfa
 SENTINEL = -1.0
 
 
-def make_cases(rng, batch, ps, tran=False, maxdim=200):
+def make_cases(rng, batch, ps, tran=False, maxdim=200, empties=()):
+    """empties: instance numbers given an empty block (m = 0 for an even
+    entry of the list, n = 0 for an odd one)."""
     ms = rng.integers(1, maxdim + 1, batch)
     ns = rng.integers(1, maxdim + 1, batch)
     ms[:3] = [1, maxdim, 7]
     ns[:3] = [maxdim, 1, 5]
+    for j, i in enumerate(empties):
+        if j % 2 == 0:
+            ms[i] = 0
+        else:
+            ns[i] = 0
     ais = rng.integers(0, 10, batch)
     ajs = rng.integers(0, 10, batch)
     ldas = ms + rng.integers(0, 5, batch)

@@ -193,10 +193,13 @@ def test_tuning_knobs_without_gpu():
 
 
 
-def test_lds_only_entries_reject_large_bounds():
-    """Trapezoid potrf and fused syrk+potrf run only in their LDS-resident
-    form (m <= 128); larger bounds are refused before any launch."""
+def test_trapezoid_entries_reject_large_bounds():
+    """Trapezoid potrf and fused syrk+potrf: m up to 1024 (the LDS-resident
+    form up to 128, above it potrf_l of the top block plus the solve of the
+    rows below), n up to 256 when m > 128; larger bounds are refused before
+    any launch with BLASFEO_AMD_ETOOBIG."""
     L = bfa.lib()
-    h = bfa.HINT(200, 200, 0)
-    assert L.blasfeo_amd_dpotrf_l_mn_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
-    assert L.blasfeo_amd_dsyrk_dpotrf_ln_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
+    for mx in ((300, 0, 0), (300, 280, 0), (1100, 100, 0)):
+        h = bfa.HINT(*mx)
+        assert L.blasfeo_amd_dpotrf_l_mn_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
+        assert L.blasfeo_amd_dsyrk_dpotrf_ln_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4

@@ -1,0 +1,68 @@
+"""Kernel sources run on the CPU under AddressSanitizer (tests/emu/): the
+library's .hip files compiled for the host against an emulation of the HIP
+device model, each matrix in its own exactly-sized allocation, results
+checked against the oracle with the reference's rule (tol 1e-11).  A kernel
+access outside a matrix, descriptor, record array or the launch's LDS fails
+the case with the source line.  No GPU is used."""
+import glob
+import os
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU = os.path.join(HERE, "emu")
+
+
+def _asan_runtime():
+    c = sorted(glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so"))
+    return c[-1] if c else None
+
+
+@pytest.fixture(scope="module")
+def emu_build():
+    if _asan_runtime() is None or not os.path.exists("/opt/rocm/llvm/bin/clang++"):
+        pytest.skip("clang++ with the AddressSanitizer runtime is not available")
+    r = subprocess.run(["make", "-s", "-j8", "-C", EMU], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return True
+
+
+def run_case(script, *args):
+    env = dict(os.environ, LD_PRELOAD=_asan_runtime(), ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([sys.executable, os.path.join(EMU, script)] + [str(a) for a in args],
+                       capture_output=True, text=True, env=env, timeout=600, cwd=EMU)
+    assert r.returncode == 0 and r.stdout.rstrip().endswith("ok"), (r.stdout + r.stderr)[-4000:]
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_potrf_l_mn_split(emu_build, ps):
+    """dpotrf_l_mn beyond the LDS-resident form (m up to 275, n up to 256):
+    potrf_l of the top block, then the rows below solved with its
+    reciprocals; the records of the GPU test test_potrf[potrf_l_mn]."""
+    run_case("case_chol.py", "potrf_l_mn", ps)
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_syrk_dpotrf_ln(emu_build, ps):
+    """dsyrk_dpotrf_ln(_mn) on test_syrk_dpotrf_ln's records: m up to 299,
+    n <= 256, offsets, k up to 69 (LDS form up to 128, split form above)."""
+    run_case("case_chol.py", "syrk_potrf", ps)
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_potrf_global_kernel(emu_build, ps):
+    """k_potrf (potrf_l for n > 256; 16-column panels at full height, the
+    earlier panels read back from D) on test_potrf_large's records: n = 257
+    .. 400, offsets, in place."""
+    run_case("case_chol.py", "potrf_large", ps)
+
+
+@pytest.mark.parametrize("thin", [1, 0])
+@pytest.mark.parametrize("op", ["trsm_llnn", "trsm_lltn", "trsm_llnu", "trsm_lunn"])
+def test_trsm_left_nonfinite(emu_build, op, thin):
+    """Left solves with a zero diagonal entry or a NaN in B (records of the GPU
+    test test_trsm_left_nonfinite): Inf and NaN only where the reference has
+    them, on the one-wavefront (thin) and the four-wave kernel."""
+    run_case("case_trsm.py", "nonfinite", op, thin)

@@ -9,6 +9,7 @@ import pytest
 import blasfeo_amd as bfa
 import oracle
 from helpers import Batch, run_both, compare_buffers, assert_close
+import chol_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -54,24 +55,8 @@ def test_potrf_lds(gpu, prec, ps, nmax):
 def test_potrf(gpu, prec, ps, op):
     if prec == "s" and op == "potrf_l_mn":
         pytest.skip("no FP32 potrf_l_mn entry")
-    if op == "potrf_l_mn":
-        pytest.skip("sizes beyond 128 need the global-memory form of dpotrf_l_mn, which is not "
-                    "launched (its first GPU run faulted, cause not found); test_potrf_mn_lds "
-                    "covers the LDS-resident form")
-    rng = np.random.default_rng(300 + ps)
     b = Batch(prec, ps, seed=1)
-    recs = []
-    sizes = [1, 2, 3, 15, 16, 17, 31, 32, 33, 63, 64, 65, 100, 128, 129, 200, 256]
-    for r, n in enumerate(sizes):
-        m = n if op == "potrf_l" else n + int(rng.integers(0, 20))
-        ci, cj = (0, 0) if r % 2 == 0 else (int(rng.integers(0, 2 * ps)), int(rng.integers(0, 3)))
-        di, dj = (0, 0) if r % 3 == 0 else (int(rng.integers(0, 2 * ps)), int(rng.integers(0, 3)))
-        S = np.zeros((ci + m, cj + n))
-        M = _spd(rng, m)
-        S[ci:, cj:] = M[:, :n]
-        C = b.mat(ci + m, cj + n, "data", S)
-        D = b.mat(di + m, dj + n, "const", -1.0)
-        recs.append(dict(m=m, n=n, ci=ci, cj=cj, di=di, dj=dj, sC=C, sD=D, ipiv=0))
+    recs = chol_cases.to_batch(b, chol_cases.potrf(ps, op, chol_cases.POTRF_SIZES, 300 + ps))
     buf = b.host()
     f = oracle.fn(prec, op)
 
@@ -83,7 +68,10 @@ def test_potrf(gpu, prec, ps, op):
 
     argdt = bfa.DFACT_ARG
     tol = 1e-11 if prec == "d" else 2e-3
-    g, gd, rr, rd = run_both(b, buf, argdt, recs, prec + op, ocall, (280, 280, 0))
+    # potrf_l_mn above m = 128: potrf_l of the n x n top block, then the rows
+    # below solved with its reciprocal diagonal (n bounded by 256)
+    hint = (280, 280, 0) if op == "potrf_l" else (280, 256, 0)
+    g, gd, rr, rd = run_both(b, buf, argdt, recs, prec + op, ocall, hint)
     compare_buffers(b, g, rr, "%s%s ps=%d" % (prec, op, ps), tol=tol)
     assert (gd["use_dA"] == rd["use_dA"]).all()
 
@@ -165,21 +153,11 @@ def test_potrf_inplace_and_failures(gpu, ps):
 
 @pytest.mark.parametrize("ps", [4, 8])
 def test_syrk_dpotrf_ln(gpu, ps):
-    pytest.skip("this case ran the global-memory form of dsyrk_dpotrf_ln, which ended in a "
-                "memory-aperture fault whose cause was not found and is not launched; "
-                "test_syrk_dpotrf_ln_lds covers the LDS-resident form")
-    rng = np.random.default_rng(400 + ps)
+    """dsyrk_dpotrf_ln(_mn) for m up to 299 (n <= 256): C + A B^T formed in D
+    (syrk_ln_mn), then the trapezoid factored in place; n = m and n < m,
+    k = 0 .. 69, row offsets on A, B and D."""
     b = Batch("d", ps, seed=3)
-    recs = []
-    for r in range(12):
-        m = int(rng.integers(1, 120))
-        k = int(rng.integers(0, 70))
-        ai, bi = (int(x) for x in rng.integers(0, 2 * ps, 2))
-        A = b.mat(ai + m, k + 1)
-        C = b.mat(m, m, "data", m * np.eye(m) + 0.01 * rng.standard_normal((m, m)))
-        D = b.mat(m, m, "const", -1.0)
-        recs.append(dict(m=m, n=m, k=k, ai=ai, aj=1, bi=ai, bj=1, ci=0, cj=0, di=0, dj=0,
-                         sA=A, sB=A, sC=C, sD=D))
+    recs = chol_cases.to_batch(b, chol_cases.syrk_potrf(ps, 400 + ps))
     buf = b.host()
     f = oracle.fn("d", "syrk_potrf_ln_mn")
 
@@ -188,8 +166,26 @@ def test_syrk_dpotrf_ln(gpu, ps):
           h.desc_addr(r["sB"]), r["bi"], r["bj"], h.desc_addr(r["sC"]), r["ci"], r["cj"],
           h.desc_addr(r["sD"]), r["di"], r["dj"])
 
-    g, gd, rr, rd = run_both(b, buf, bfa.DSYRK_DPOTRF_ARG, recs, "dsyrk_dpotrf_ln", ocall, (128, 128, 80))
+    g, gd, rr, rd = run_both(b, buf, bfa.DSYRK_DPOTRF_ARG, recs, "dsyrk_dpotrf_ln", ocall, (300, 256, 80))
     compare_buffers(b, g, rr, "dsyrk_dpotrf_ln ps=%d" % ps)
+    assert (gd["use_dA"] == rd["use_dA"]).all()
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_potrf_large(gpu, ps):
+    """potrf_l for n > 256 (16-column panels at full height in LDS, the
+    earlier panels read back from D), with offsets and in place."""
+    b = Batch("d", ps, seed=19)
+    recs = chol_cases.to_batch(b, chol_cases.potrf(ps, "potrf_l", chol_cases.POTRF_LARGE_SIZES, 760 + ps, inplace_at=5))
+    buf = b.host()
+    f = oracle.fn("d", "potrf_l")
+
+    def ocall(h, r):
+        f(ps, r["m"], h.desc_addr(r["sC"]), r["ci"], r["cj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    g, gd, rr, rd = run_both(b, buf, bfa.DFACT_ARG, recs, "dpotrf_l", ocall, (420, 420, 0))
+    compare_buffers(b, g, rr, "dpotrf_l large ps=%d" % ps)
+    assert (gd["use_dA"] == rd["use_dA"]).all()
 
 
 @pytest.mark.parametrize("ps", [4, 8])

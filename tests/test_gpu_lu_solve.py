@@ -193,3 +193,36 @@ def test_getrf_np(gpu, ps):
     g, gd, rr, rd = run_both(b, buf, bfa.DFACT_ARG, recs, "dgetrf_np", ocall, (300, 300, 0))
     compare_buffers(b, g, rr, "dgetrf_np ps=%d" % ps)
     assert (gd["use_dA"] == rd["use_dA"]).all()
+
+
+@pytest.mark.parametrize("thin", [True, False])
+@pytest.mark.parametrize("op", ["trsm_llnn", "trsm_lltn", "trsm_llnu", "trsm_lunn"])
+def test_trsm_left_nonfinite(gpu, op, thin):
+    """A zero diagonal entry (reciprocal Inf) or a NaN in B: every element,
+    NaN and Inf included, equals the oracle's (rows solved before the
+    non-finite one keep their values)."""
+    import trsm_cases
+    from chol_cases import to_batch
+    ps = 4
+    b = Batch("d", ps, seed=5)
+    recs = to_batch(b, trsm_cases.nonfinite(ps, op, 970 + 3 * len(op) + thin))
+    buf = b.host()
+    f = oracle.fn("d", op)
+
+    def ocall(h, r):
+        f(ps, r["m"], r["n"], r["alpha"], h.desc_addr(r["sA"]), r["ai"], r["aj"],
+          h.desc_addr(r["sB"]), r["bi"], r["bj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
+
+    print("%s nonfinite thin=%d" % (op, thin), flush=True)
+    assert bfa.lib().blasfeo_amd_tune_set(b"trsm_thin", 0, 1 if thin else 2) == 0
+    try:
+        g, gd, rr, rd = run_both(b, buf, bfa.DTRSM_ARG, recs, "d" + op, ocall, (256, 4, 0))
+    finally:
+        bfa.lib().blasfeo_amd_tune_set(b"trsm_thin", 0, 0)
+    lay = b.layout
+    for i, r in enumerate(recs):
+        gv, rv = lay.view(g, r["sD"]), lay.view(rr, r["sD"])
+        same = (gv == rv) | (np.isnan(gv) & np.isnan(rv)) | (np.abs(gv - rv) <= 1e-11 * (np.abs(gv) + np.abs(rv)))
+        assert same.all(), "%s instance %d: %d elements differ, first %s gpu=%r ref=%r" % (
+            op, i, int((~same).sum()), np.argwhere(~same)[:1].tolist(), gv[~same][:1], rv[~same][:1])
+    assert (gd["use_dA"] == rd["use_dA"]).all()

@@ -42,14 +42,18 @@ def _run(c, name, cm_init, pm_init):
 @pytest.mark.parametrize("name", ["dpack", "dpack_tran"])
 def test_pack(gpu, ps, name):
     rng = np.random.default_rng(800 + ps + (name == "dpack_tran"))
-    c = make_cases(rng, 40, ps, tran=(name == "dpack_tran"))
+    # instances 5..8 have empty blocks: pack leaves their use_dA (5) alone,
+    # pack_tran clears it (auxiliary/d_aux_lib4.c:1609, :2090)
+    c = make_cases(rng, 40, ps, tran=(name == "dpack_tran"), empties=(5, 6, 7, 8))
     cm = rng.standard_normal(c["cm_len"])
     pm = np.full(c["pm_bytes"] // 8, SENTINEL)
     gcm, gpm, gdesc, rcm, rpm, rdesc = _run(c, name, cm, pm)
     np.testing.assert_array_equal(gpm, rpm)
     np.testing.assert_array_equal(gcm, cm)
     np.testing.assert_array_equal(gdesc["use_dA"], rdesc["use_dA"])
-    assert np.all(gdesc["use_dA"] == 0)
+    empty = (c["ms"] == 0) | (c["ns"] == 0)
+    assert np.all(gdesc["use_dA"][~empty] == 0)
+    assert np.all(gdesc["use_dA"][empty] == (0 if name == "dpack_tran" else 5))
 
 
 @pytest.mark.parametrize("ps", [4, 8])

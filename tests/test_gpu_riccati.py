@@ -125,3 +125,74 @@ def test_riccati_tuned_shapes(gpu, ps):
                                  1e-11, "threads %d instance %d stage %d dA" % (nt, i, k))
     finally:
         autotune.reset()
+
+
+def test_riccati_graph_replay_after_growth(gpu):
+    """A graph captured on a stream, then an eager sweep of a larger batch on
+    the same stream (its plan needs a larger workspace, which replaces the
+    stream's block), then the graph replayed: the replay still equals the
+    first eager result, since a replaced workspace block stays allocated."""
+    import torch
+    rng = np.random.default_rng(650)
+    small = RiccatiBatch(list(rng.integers(1, 65, 16)), list(rng.integers(1, 17, 16)), N=4, ps=4)
+    sinp = small.fill_inputs(rng)
+    nbig = 40000
+    big = RiccatiBatch(list(rng.integers(1, 9, nbig)), list(rng.integers(1, 4, nbig)), N=1, ps=4)
+    binp = big.fill_inputs(rng)
+    dsmall = DeviceRiccati(small, sinp)
+    dbig = DeviceRiccati(big, binp)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        dsmall.run(s.cuda_stream)
+    torch.cuda.synchronize()
+    ref = dsmall.outbuf.clone()
+    g = dsmall.capture(s)
+    torch.cuda.synchronize()
+    print("riccati eager batch=%d after capture" % nbig, flush=True)
+    with torch.cuda.stream(s):
+        dbig.run(s.cuda_stream)
+    torch.cuda.synchronize()
+    for rep in range(2):
+        dsmall.outbuf.zero_()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(dsmall.outbuf, ref), "replay %d after the larger eager call differs" % rep
+
+
+def test_riccati_horizon20_full(gpu):
+    """The Riccati workload at full size: 8192 instances, horizon N = 20,
+    nx ~ U{8..64}, nu ~ U{2..16}, A = I + 0.1 N(0,1), B ~ N(0,1), Q = R = I,
+    one batched call.  Every stage of every instance has backward error
+    ||M_k - L_k L_k^T||_F <= 10 n_k eps ||M_k||_F with M_k rebuilt from the
+    computed L_{k+1}; the first 32 instances' stage-N and stage-(N-1)
+    factors (before the recursion amplifies rounding) equal the oracle's at
+    the 1e-11 rule."""
+    import torch
+    from blasfeo_amd.riccati import stage_backward_errors
+    rng = np.random.default_rng(1234)
+    batch = 8192
+    nxs = rng.integers(8, 65, batch)
+    nus = rng.integers(2, 17, batch)
+    rb = RiccatiBatch(nxs, nus, N=20, ps=4)
+    inp = rb.fill_inputs(rng)
+    dev = DeviceRiccati(rb, inp)
+    print("riccati batch=%d N=20" % batch, flush=True)
+    dev.run()
+    torch.cuda.synchronize()
+    gbuf = dev.outbuf.cpu().numpy()
+    e = stage_backward_errors(rb, inp, gbuf, range(batch))
+    print("worst stage backward error %.3f n eps" % e.max(), flush=True)
+    assert e.max() <= 10.0, "instance %d stage %d: %.2f n eps" % (np.unravel_index(e.argmax(), e.shape) + (e.max(),))
+    sub = RiccatiBatch(nxs[:32], nus[:32], N=20, ps=4)
+    sinp = sub.inp.host_buffer()
+    for i in range(32):
+        for a, b in ((sub.i_babt[i], rb.i_babt[i]), (sub.i_rsq[i], rb.i_rsq[i]), (sub.i_qn[i], rb.i_qn[i])):
+            sub.inp.view(sinp, a)[:] = rb.inp.view(inp, b)
+    host = HostRiccati(sub, sinp)
+    oracle.lib().or_dric_trf_batch(4, 32, host.argv.ctypes.data)
+    for i in range(32):
+        for k in (20, 19):
+            g = np.tril(rb.out.get(gbuf, rb.i_L[i][k]))
+            assert_close(g, np.tril(host.L(i, k)), 1e-11, "instance %d stage %d" % (i, k))

@@ -133,3 +133,39 @@ def test_pm_dgecp_between_panel_sizes():
     a0 = bfa.pm_unpack(buf[offs[0] // 8:], 4, int(d["cn"][0]), m, n)
     a2 = bfa.pm_unpack(buf[offs[2] // 8:], 4, int(d["cn"][2]), m, n)
     np.testing.assert_array_equal(a0, a2)
+
+
+@pytest.mark.parametrize("tran", [False, True])
+def test_pack_empty_block_cache(tran):
+    """An empty block (m = 0 or n = 0): blasfeo_pack_dmat returns before it
+    touches the cache (auxiliary/d_aux_lib4.c:1609), blasfeo_pack_tran_dmat
+    clears use_dA first (:2090); nothing else is written.  Checked for the
+    oracle and for the library's host blasfeo_pack(_tran)_dmat."""
+    rng = np.random.default_rng(77 + tran)
+    c = make_cases(rng, 6, 4, tran, maxdim=20, empties=(1, 2, 3, 4))
+    cm, pm, desc, rec = _host(c)
+    pm0 = pm.copy()
+    L = oracle.lib()
+    (L.or_dpack_tran_batch if tran else L.or_dpack_batch)(4, c["batch"], rec.ctypes.data)
+    empty = (c["ms"] == 0) | (c["ns"] == 0)
+    assert empty.sum() == 4
+    assert np.all(desc["use_dA"][empty] == (0 if tran else 5))
+    assert np.all(desc["use_dA"][~empty] == 0)
+    # the empty instances wrote nothing: only the non-empty ones' blocks changed
+    for i in np.nonzero(empty)[0]:
+        p0 = (int(desc["pA"][i]) - pm.ctypes.data) // 8
+        n = int(desc["memsize"][i]) // 8
+        np.testing.assert_array_equal(pm[p0:p0 + n], pm0[p0:p0 + n])
+    # the library's host routine follows the same rule
+    lib = bfa.lib()
+    lib.blasfeo_amd_set_ps(4)
+    d = np.zeros(1, dtype=bfa.DMAT)
+    buf = np.zeros(bfa.memsize(np.float64, 4, 8, 8) // 8)
+    lib.blasfeo_create_dmat(8, 8, ctypes.c_void_p(d.ctypes.data), ctypes.c_void_p(buf.ctypes.data))
+    d["use_dA"] = 3
+    A = np.ones(4)
+    f = lib.blasfeo_pack_tran_dmat if tran else lib.blasfeo_pack_dmat
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    f(0, 4, A.ctypes.data, 1, d.ctypes.data, 0, 0)
+    assert int(d["use_dA"][0]) == (0 if tran else 3)
+    assert not buf.any()

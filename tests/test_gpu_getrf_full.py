@@ -1,7 +1,8 @@
 """The LU workload at full size: 16384 blasfeo_dgetrf_rp calls, n ~ U{16..256},
 entries U[-1,1] (seeded), ps = 4, in place.  Every instance's pivot vector
-must equal the oracle's bit for bit (FP64), and the factors of the first 512
-instances must match the oracle's at the reference's rule (tol 1e-11).  For
+must equal the oracle's bit for bit (FP64), and the factors of the first 2048
+instances must be backward stable, ||P A - L U||_F <= 10 n eps ||A||_F, with dA
+the reciprocals of U's diagonal.  For
 the FP32 twin the test counts the instances whose pivots differ from the
 oracle's (a different summation order flips near-ties in single precision)
 and requires each of them to be a backward-stable factorisation,
@@ -50,7 +51,7 @@ def _batch(prec, seed):
     hrec["sD"] = hrec["sC"]
     hrec["ipiv"] = hpiv.ctypes.data + 4 * piv_off
     getattr(oracle.lib(), "or_%sgetrf_rp_batch" % prec)(4, BATCH, hrec.ctypes.data)
-    return nn, off, host, gout, hbuf, gpiv, hpiv, piv_off
+    return nn, off, host, gout, hbuf, gpiv, hpiv, piv_off.astype(np.int64)
 
 
 def test_getrf_rp_full_batch_pivots_fp64(gpu):
@@ -59,15 +60,31 @@ def test_getrf_rp_full_batch_pivots_fp64(gpu):
            if not np.array_equal(gpiv[piv_off[i]: piv_off[i] + nn[i]], hpiv[piv_off[i]: piv_off[i] + nn[i]])]
     print("fp64: %d of %d pivot vectors differ" % (len(bad), BATCH), flush=True)
     assert not bad, "instances with differing pivots: %s" % bad[:10]
-    for i in range(512):
+    # the factors: backward error ||P A - L U||_F <= 10 n eps ||A||_F (the FP64
+    # bar; elementwise agreement with the oracle degrades with n for LU, whose
+    # later columns accumulate n rounding steps in a different order), and dA
+    # the reciprocal of the computed U's diagonal
+    eps = np.finfo(np.float64).eps
+    worst = 0.0
+    for i in range(2048):
         n = int(nn[i])
-        cn = -(-n // 4) * 4
+        cn = pm = -(-n // 4) * 4
         o = int(off[i]) // 8
-        g = gout[o: o + n * cn + n]     # panels and dA
-        r = hbuf[o: o + n * cn + n]
-        d = np.abs(g - r)
-        badel = (d > 1e-11 * (np.abs(g) + np.abs(r))) & (d > 1e-11)
-        assert not badel.any(), "instance %d (n=%d): %d elements differ" % (i, n, int(badel.sum()))
+        A = bfa.pm_unpack(host[o: o + pm * cn], 4, cn, n, n)
+        F = bfa.pm_unpack(gout[o: o + pm * cn], 4, cn, n, n)
+        gp = gpiv[piv_off[i]: piv_off[i] + n]
+        PA = A.copy()
+        for j in range(n):
+            p = int(gp[j])
+            PA[[j, p]] = PA[[p, j]]
+        L = np.tril(F, -1) + np.eye(n)
+        U = np.triu(F)
+        r = np.linalg.norm(PA - L @ U) / (np.linalg.norm(A) * n * eps)
+        worst = max(worst, r)
+        assert r <= 10.0, "instance %d (n=%d): backward error %.2f n eps" % (i, n, r)
+        dA = gout[o + pm * cn: o + pm * cn + n]
+        np.testing.assert_array_equal(dA, 1.0 / np.diag(U), err_msg="instance %d dA" % i)
+    print("fp64: worst backward error of the first 2048 factors %.3f n eps" % worst, flush=True)
 
 
 def test_getrf_rp_full_batch_pivots_fp32(gpu):
@@ -80,10 +97,10 @@ def test_getrf_rp_full_batch_pivots_fp32(gpu):
         if np.array_equal(gp, hpiv[piv_off[i]: piv_off[i] + n]):
             continue
         bad.append(i)
-        cn = -(-n // 4) * 4
+        cn = pm = -(-n // 4) * 4
         o = int(off[i]) // 4
-        A = bfa.pm_unpack(host[o: o + n * cn].astype(np.float64), 4, cn, n, n)
-        F = bfa.pm_unpack(gout[o: o + n * cn].astype(np.float64), 4, cn, n, n)
+        A = bfa.pm_unpack(host[o: o + pm * cn].astype(np.float64), 4, cn, n, n)
+        F = bfa.pm_unpack(gout[o: o + pm * cn].astype(np.float64), 4, cn, n, n)
         L = np.tril(F, -1) + np.eye(n)
         U = np.triu(F)
         PA = A.copy()

@@ -102,6 +102,8 @@ def lib():
         f = getattr(L, "blasfeo_amd_%s_batched" % name)
         f.argtypes = [ip, ip, vp, ctypes.POINTER(HINT), vp]
         f.restype = ip
+    L.blasfeo_amd_d_back_ric_trf_batched_noalloc.argtypes = [ip, ip, vp, ctypes.POINTER(HINT), vp]
+    L.blasfeo_amd_d_back_ric_trf_batched_noalloc.restype = ip
     L.blasfeo_amd_dposv_mixed_batched.argtypes = [ip, ip, vp, ctypes.c_double, ip,
                                                   ctypes.POINTER(HINT), vp]
     L.blasfeo_amd_dposv_mixed_batched.restype = ip

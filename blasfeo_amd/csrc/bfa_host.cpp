@@ -43,24 +43,33 @@ int check_launch(const char *what)
 // beyond it aborts the queue at dispatch instead of failing the launch call.
 int lds_check(const void *fn, size_t dyn, const char *what)
 {
+	// static LDS per (device, kernel), in a fixed table (no allocation: the
+	// allocation-free entries use this check too)
+	struct Entry { int dev; const void *fn; size_t stat; };
+	constexpr int NCACHE = 256;
 	static std::mutex mu;
-	static std::map<std::pair<int, const void *>, size_t> cache;  // static LDS per (device, kernel)
+	static Entry cache[NCACHE];
+	static int ncache = 0;
 	int dev = 0;
 	if (hipGetDevice(&dev) != hipSuccess)
 		return set_error(BLASFEO_AMD_ELAUNCH, "%s: no device", what);
 	size_t stat = 0, limit = 0;
 	{
 		std::lock_guard<std::mutex> lock(mu);
-		auto key = std::make_pair(dev, fn);
-		auto it = cache.find(key);
-		if (it == cache.end())
+		int i = 0;
+		while (i < ncache && !(cache[i].dev == dev && cache[i].fn == fn))
+			i++;
+		if (i == ncache)
 		{
 			hipFuncAttributes at;
 			if (hipFuncGetAttributes(&at, fn) != hipSuccess)
 				return check_launch(what);
-			it = cache.emplace(key, at.sharedSizeBytes).first;
+			if (ncache < NCACHE)
+				cache[ncache++] = Entry{dev, fn, at.sharedSizeBytes};
+			stat = at.sharedSizeBytes;
 		}
-		stat = it->second;
+		else
+			stat = cache[i].stat;
 	}
 	// the per-block attribute can report less than a CU's LDS, which one
 	// workgroup may use whole on gfx950; take the larger of the two

@@ -553,13 +553,12 @@ static RicFork *ric_fork_state()
 	return f;
 }
 
+// ws: the plan workspace, (RIC_NCLASS + 1) * batch + 64 ints.  fork: run the
+// class launches side by side on helper streams (else all on st, in order).
 template <int PS>
-static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, int maxfoot, hipStream_t st, const char *name)
+static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, int maxfoot, int *ws, bool fork,
+	hipStream_t st, const char *name)
 {
-	const size_t wsbytes = sizeof(int) * ((size_t)(RIC_NCLASS + 1) * batch + 64);
-	int *ws = (int *)workspace(wsbytes, st);
-	if (!ws)
-		return set_error(BLASFEO_AMD_ENOMEM, "%s: cannot allocate the %zu-byte plan workspace", name, wsbytes);
 	int *counts = ws;
 	int *lists = ws + 64;
 	hipLaunchKernelGGL(k_ric_zero, dim3(1), dim3(64), 0, st, counts);
@@ -568,18 +567,23 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 	if (rc)
 		return rc;
 	const int cus = num_cus() > 0 ? num_cus() : 256;
-	RicFork *fk = ric_fork_state();
-	if (!fk)
-		return set_error(BLASFEO_AMD_ELAUNCH, "%s: cannot create the helper streams", name);
-	std::lock_guard<std::mutex> lock(fk->mu);
-	hipStream_t ss[1 + RIC_AUX] = {st};
-	if (hipEventRecord(fk->fork, st) != hipSuccess)
-		return check_launch(name);
-	for (int i = 0; i < RIC_AUX; i++)
+	RicFork *fk = nullptr;
+	std::unique_lock<std::mutex> lock;
+	hipStream_t ss[1 + RIC_AUX] = {st, st, st, st};
+	if (fork)
 	{
-		ss[1 + i] = fk->aux[i];
-		if (hipStreamWaitEvent(fk->aux[i], fk->fork, 0) != hipSuccess)
+		fk = ric_fork_state();
+		if (!fk)
+			return set_error(BLASFEO_AMD_ELAUNCH, "%s: cannot create the helper streams", name);
+		lock = std::unique_lock<std::mutex>(fk->mu);
+		if (hipEventRecord(fk->fork, st) != hipSuccess)
 			return check_launch(name);
+		for (int i = 0; i < RIC_AUX; i++)
+		{
+			ss[1 + i] = fk->aux[i];
+			if (hipStreamWaitEvent(fk->aux[i], fk->fork, 0) != hipSuccess)
+				return check_launch(name);
+		}
 	}
 	// general class first (largest instances), then resident classes largest
 	// first, dealt round-robin over the caller's and the helper streams
@@ -638,11 +642,44 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 	}
 	// join back whatever was launched (also after a launch error, so that
 	// the caller's stream never runs ahead of the helpers)
-	for (int i = 0; i < RIC_AUX; i++)
+	for (int i = 0; fork && i < RIC_AUX; i++)
 		if (hipEventRecord(fk->join[i], fk->aux[i]) != hipSuccess ||
 			hipStreamWaitEvent(st, fk->join[i], 0) != hipSuccess)
 			return check_launch(name);
 	return rc ? rc : check_launch(name);
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free entry: the plan's counts, queue heads and class lists
+// live in a file-scope device array instead of the per-stream workspace, and
+// the class launches run in order on the caller's stream (no helper
+// streams), so that a call creates nothing and allocates nothing.
+// ---------------------------------------------------------------------------
+constexpr int RIC_NOALLOC_MAX = 65536;   // most instances per call
+static __device__ __align__(256) int scratch[64 + (RIC_NCLASS + 1) * RIC_NOALLOC_MAX];
+
+// Device address of this device's copy of `scratch` (each GPU holds its own),
+// looked up once per device with that device current.
+static int *noalloc_scratch()
+{
+	constexpr int MAXDEV = 64;
+	static std::mutex mu;
+	static int *ptr[MAXDEV];
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAXDEV)
+		return nullptr;
+	std::lock_guard<std::mutex> lock(mu);
+	if (!ptr[dev])
+	{
+		void *p = nullptr;
+		if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess)
+		{
+			(void)hipGetLastError();
+			return nullptr;
+		}
+		ptr[dev] = static_cast<int *>(p);
+	}
+	return ptr[dev];
 }
 
 } // namespace bfa
@@ -668,6 +705,39 @@ extern "C" int blasfeo_amd_d_back_ric_trf_batched(int ps, int batch, const blasf
 	const int nxmax = hint->max_n > 0 ? hint->max_n : mmax;
 	const int maxfoot = ric_footprint(mmax, min(nxmax, mmax));
 	hipStream_t st = (hipStream_t)stream;
-	return ps == 4 ? launch_ric<4>(batch, args, mmax, maxfoot, st, name)
-	               : launch_ric<8>(batch, args, mmax, maxfoot, st, name);
+	const size_t wsbytes = sizeof(int) * ((size_t)(RIC_NCLASS + 1) * batch + 64);
+	int *ws = (int *)workspace(wsbytes, st);
+	if (!ws)
+		return set_error(BLASFEO_AMD_ENOMEM, "%s: cannot allocate the %zu-byte plan workspace", name, wsbytes);
+	return ps == 4 ? launch_ric<4>(batch, args, mmax, maxfoot, ws, true, st, name)
+	               : launch_ric<8>(batch, args, mmax, maxfoot, ws, true, st, name);
+}
+
+extern "C" int blasfeo_amd_d_back_ric_trf_batched_noalloc(int ps, int batch, const blasfeo_amd_riccati_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream)
+{
+	const char *name = "blasfeo_amd_d_back_ric_trf_batched_noalloc";
+	if (batch < 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: batch=%d < 0", name, batch);
+	if (batch == 0)
+		return BLASFEO_AMD_OK;
+	if (batch > RIC_NOALLOC_MAX)
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: at most %d instances per call (its fixed scratch), got %d",
+			name, RIC_NOALLOC_MAX, batch);
+	if (!args)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: args is NULL", name);
+	if (ps != 4 && ps != 8)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: unsupported panel size %d", name, ps);
+	if (!hint || hint->max_m <= 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: hint->max_m (largest nu[k]+nx[k]) is required", name);
+	const int mmax = hint->max_m;
+	if (mmax > fact_mmax<double>())
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: nu+nx up to %d supported, got %d", name, fact_mmax<double>(), mmax);
+	const int nxmax = hint->max_n > 0 ? hint->max_n : mmax;
+	const int maxfoot = ric_footprint(mmax, min(nxmax, mmax));
+	int *ws = noalloc_scratch();
+	if (!ws)
+		return set_error(BLASFEO_AMD_ELAUNCH, "%s: no device address for the scratch array", name);
+	hipStream_t st = (hipStream_t)stream;
+	return ps == 4 ? launch_ric<4>(batch, args, mmax, maxfoot, ws, false, st, name)
+	               : launch_ric<8>(batch, args, mmax, maxfoot, ws, false, st, name);
 }

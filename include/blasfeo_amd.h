@@ -16,8 +16,14 @@
  *     device-resident array of argument records; record i holds exactly the
  *     arguments of one reference call (sizes, scalars, pointers to
  *     device-resident descriptors and their row/column offsets).  They run
- *     asynchronously on the given HIP stream, allocate nothing, perform no host
- *     synchronisation and can be captured into a HIP graph.  They return an
+ *     asynchronously on the given HIP stream and perform no host
+ *     synchronisation.  The Riccati sweep, the trapezoid factorisations above
+ *     128 rows and the mixed-precision solve keep their per-instance plan in a
+ *     per-(device, stream) workspace that the library allocates on the first
+ *     call (and enlarges for a larger batch); every other entry allocates
+ *     nothing.  After one eager call of the same size on a stream, a call can
+ *     be captured into a HIP graph; blasfeo_amd_d_back_ric_trf_batched_noalloc
+ *     allocates nothing at all.  They return an
  *     int status for argument/launch errors only; numerical failures follow
  *     the reference's conventions (zero pivot => zero reciprocal in dA).
  *
@@ -410,6 +416,17 @@ struct blasfeo_amd_riccati_arg
  * one launch (the per-stage trmm_rlnn + syrk_dpotrf_ln chain of
  * examples/example_d_riccati_recursion.c:120-138). */
 int blasfeo_amd_d_back_ric_trf_batched(int ps, int batch, const struct blasfeo_amd_riccati_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+
+/* The same sweep for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc); it has no
+ * counterpart in the reference.  It allocates and creates nothing: the plan's
+ * class counts, queue heads and lists live in a fixed device array of the
+ * library (2.4 MB per GPU, held until the process ends), and the class
+ * launches run in order on `stream` instead of side by side on helper
+ * streams.  At most 65536 instances per call (more: BLASFEO_AMD_ETOOBIG).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams, or other host threads) race and must not be issued. */
+int blasfeo_amd_d_back_ric_trf_batched_noalloc(int ps, int batch, const struct blasfeo_amd_riccati_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 
 /* One instance of d_back_ric_trs_libstr(N, nx, nu, hsBAbt, hsb, hsrq, hsL, hsPb,
  * hsux, hspi, hswork_vec) (examples/example_d_riccati_recursion.c:203-267):

@@ -61,6 +61,17 @@ def riccati_hbm_bytes(nxs, nus, N):
     return float(8 * (rd + wr).sum())
 
 
+def roof(flops, algo_bytes, ms, fp_peak):
+    """The roof a kernel is held to, from its algorithmic arithmetic
+    intensity against the ridge point fp_peak / HBM peak: (bound, achieved,
+    peak, unit), achieved in TFLOP/s for the matrix-core roof and in GB/s
+    (algorithmic bytes) for the HBM roof."""
+    ridge = fp_peak * 1e12 / (HBM_PEAK_GBS * 1e9)
+    if flops / algo_bytes >= ridge:
+        return "mfma", flops / (ms * 1e-3) / 1e12, fp_peak, "TFLOP/s"
+    return "hbm", algo_bytes / (ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s"
+
+
 def read_traffic(kernel, workload):
     """HBM bytes per launch from profiles/hbm_traffic.txt (rocprofv3 FETCH_SIZE
     and WRITE_SIZE runs, scaled by a copy-kernel calibration), or None."""
@@ -246,6 +257,7 @@ def run_riccati(args, torch, bfa):
             % (nchk, berr, fdiff))
 
     traffic = read_traffic("k_ric_class", "riccati-%dx%d" % (args.batch, args.horizon))
+    rbound, rach, rpeak, runit = roof(flops, algo_bytes, ms, FP64_PEAK_TFLOPS)
     out = {
         "metric": METRIC,
         "value": solves,
@@ -264,8 +276,8 @@ def run_riccati(args, torch, bfa):
                    "parallelism": "one workgroup per instance, 1 GPU",
                    "launch": "hip graph replay" if args.graph else "eager",
                    "tuning": tuning},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+        "roofline": {"bound": rbound, "achieved": rach, "peak": rpeak, "unit": runit,
+                     "frac": rach / rpeak, "traffic": traffic,
                      "algorithmic_bytes": algo_bytes, "flops_per_launch": flops,
                      "kernel": "k_ric_class<ps,256> + k_ric_class<ps,128> (the class launches of one batched call, run side by side)"},
         "cpu_baseline": cpu,
@@ -747,8 +759,8 @@ def run_other(args, torch, bfa):
         algo = batch * 8704.0
         value = batch / (ms * 1e-3)
         unit = "factorizations/s"
-        bound, peak, punit = "hbm", HBM_PEAK_GBS, "GB/s"
-        achieved = algo / (ms * 1e-3) / 1e9
+        bound, achieved, peak, punit = roof(flops, algo, ms, FP64_PEAK_TFLOPS)
+        traffic = read_traffic("k_potrf_lds", "potrf-4096x32")
         kernel = "k_potrf_lds<double,%d,64>" % ps
         wl = {"workload": "dpotrf_l_batched", "batch": batch, "n": n, "ps": ps, "precision": "fp64"}
         # check: backward error of every factor
@@ -809,9 +821,9 @@ def run_other(args, torch, bfa):
         algo = float(8 * ((mf * kf + nf * kf + 2 * mf * nf).sum() + (sm * sk + sm * (sm + 1)).sum()))
         value = 2 * batch / (ms * 1e-3)
         unit = "ops/s (dgemm_nt + dsyrk_ln calls)"
-        bound, peak, punit = "hbm", HBM_PEAK_GBS, "GB/s"
-        achieved = algo / (ms * 1e-3) / 1e9
-        kernel = "k_gemm<double,%d,OP_NT> + k_gemm<double,%d,OP_SYRK>" % (ps, ps)
+        bound, achieved, peak, punit = roof(flops, algo, ms, FP64_PEAK_TFLOPS)
+        traffic = read_traffic("k_gemm_wave", "gemm-syrk-100000")
+        kernel = "k_gemm_wave<double,%d,OP_NT> + k_gemm_wave<double,%d,OP_SYRK>" % (ps, ps)
         wl = {"workload": "dgemm_nt+dsyrk_ln_batched", "batch": "100000 + 100000",
               "m,n,k": "U{4..128}", "ps": ps, "precision": "fp64"}
     else:
@@ -864,9 +876,8 @@ def run_other(args, torch, bfa):
         algo = float((np.dtype(dt).itemsize * 2 * nf ** 2 + 12 * nf).sum())
         value = batch / (ms * 1e-3)
         unit = "factorizations/s"
-        bound, punit = "mfma", "TFLOP/s"
-        peak = FP64_PEAK_TFLOPS if prec == "d" else 157.3
-        achieved = flops / (ms * 1e-3) / 1e12
+        bound, achieved, peak, punit = roof(flops, algo, ms, FP64_PEAK_TFLOPS if prec == "d" else 157.3)
+        traffic = read_traffic("k_getrf", "getrf-16384") if args.workload == "getrf" else None
         kernel = "k_getrf<%s,%d%s>" % ("double" if prec == "d" else "float", ps,
                                        ",PIV=false" if args.workload == "getrf_np" else "")
         if args.workload == "getrf_np":
@@ -905,7 +916,7 @@ def run_other(args, torch, bfa):
            "vs_baseline": None, "dtype": "f32" if args.workload == "sgetrf" else "f64",
            "data": "synthetic, seeded (%d)" % args.seed, "config": wl,
            "roofline": {"bound": bound, "achieved": achieved, "peak": peak, "unit": punit,
-                        "frac": achieved / peak, "traffic": None, "algorithmic_bytes": algo,
+                        "frac": achieved / peak, "traffic": traffic, "algorithmic_bytes": algo,
                         "flops_per_launch": flops, "kernel": kernel},
            "cpu_baseline": cpu, "checks": checks}
     print(json.dumps(out), flush=True)
@@ -960,7 +971,9 @@ def run_posv(args, torch, bfa):
     A = torch.empty((batch, n, n), dtype=f64, device=dev)
     for b0 in range(0, batch, 256):
         G = torch.randn((256, n, n), dtype=f64, device=dev, generator=gen)
-        Q, R = torch.linalg.qr(G)
+        # the QR on the host (no GPU library kernels in a profiled run)
+        Q, R = torch.linalg.qr(G.cpu())
+        Q, R = Q.to(dev), R.to(dev)
         Q = Q * torch.sign(torch.diagonal(R, dim1=1, dim2=2)).unsqueeze(1)
         lc = torch.rand(256, dtype=f64, device=dev, generator=gen) * 6 + 2
         lam = torch.pow(10.0, -lc[:, None] * torch.arange(n, dtype=f64, device=dev)[None, :] / (n - 1))
@@ -1061,7 +1074,7 @@ def run_posv(args, torch, bfa):
            "data": "synthetic, seeded (%d)" % args.seed, "config": wl,
            "fp64_solves_per_s": v64, "fp64_ms_per_step": ms64, "mixed_over_fp64": value / v64,
            "roofline": {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
-                        "frac": achieved / 157.3, "traffic": None,
+                        "frac": achieved / 157.3, "traffic": read_traffic("k_posv_mixed", "posv-2048x256"),
                         "flops_per_launch": flops_fact, "kernel": "k_posv_mixed<%d>" % ps,
                         "note": "factorisation flops n^3/3 per instance over the whole mixed step, against the FP32 matrix-core peak"},
            "cpu_baseline": cpu, "checks": checks}

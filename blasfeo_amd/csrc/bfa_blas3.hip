@@ -6,6 +6,7 @@
 // so a batch of small instances uses one workgroup each and a batch of large
 // ones spreads its 64x64 tiles over several (tile engine: bfa_gemm_core.hpp).
 #include "bfa_gemm_core.hpp"
+#include "bfa_gemm_wave.hpp"
 #include "bfa_chol_core.hpp"
 #include "bfa_internal.hpp"
 
@@ -535,6 +536,27 @@ static int launch_gemm(int ps, int batch, const typename GemmArg<T>::type *args,
 	int rc = check_common(ps, batch, args, sizeof(T) == 4, name);
 	if (rc || batch == 0)
 		return rc;
+	// sizes bounded by 256 (at most 64 tiles of 32 x 32 per instance): one
+	// wavefront per output tile, operands straight from global memory; the
+	// LDS-staged 64 x 64 engine for larger or unbounded sizes.  The
+	// "gemm_wave" knob forces the wave form off (2).
+	const int gwk = tune_value(TUNE_GEMM_WAVE, 0);
+	if (gwk != 2 && hint && hint->max_m > 0 && hint->max_m <= 256 &&
+	    (OP == OP_SYRK || (hint->max_n > 0 && hint->max_n <= 256)))
+	{
+		const int mt = (hint->max_m + GW_T - 1) / GW_T;
+		const int nt = OP == OP_SYRK ? mt : (hint->max_n + GW_T - 1) / GW_T;
+		int gy = (OP == OP_SYRK || OP == OP_SYRK_MN) ? mt * (mt + 1) / 2 : mt * nt;
+		gy = gy < 1 ? 1 : gy;
+		dim3 gridw(batch, gy);
+		if (ps == 4)
+			hipLaunchKernelGGL((k_gemm_wave<T, 4, OP>), gridw, dim3(WAVE), 0, st, args);
+		else if (ps == 8)
+			hipLaunchKernelGGL((k_gemm_wave<T, 8, OP>), gridw, dim3(WAVE), 0, st, args);
+		else
+			hipLaunchKernelGGL((k_gemm_wave<T, 16, OP>), gridw, dim3(WAVE), 0, st, args);
+		return check_launch(name);
+	}
 	dim3 grid(batch, tiles_y(hint, TM, TN));
 	if (ps == 4)
 		hipLaunchKernelGGL((k_gemm<T, 4, OP>), grid, dim3(NT3), 0, st, args);

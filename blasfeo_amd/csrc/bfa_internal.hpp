@@ -30,7 +30,7 @@ void *workspace(size_t bytes, hipStream_t stream);
 int trsm_rltn_given_dA(int ps, int batch, const blasfeo_amd_dtrsm_arg *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name);
 
 // Launch-shape knobs (blasfeo_amd_tune_set); 0 means the built-in choice.
-enum TuneKnob { TUNE_RIC_PER_CU = 0, TUNE_RIC_THREADS = 1, TUNE_GEMM_GRID_Y = 2, TUNE_POTRF_BW = 3, TUNE_TRSM_THIN = 4 };
+enum TuneKnob { TUNE_RIC_PER_CU = 0, TUNE_RIC_THREADS = 1, TUNE_GEMM_GRID_Y = 2, TUNE_POTRF_BW = 3, TUNE_TRSM_THIN = 4, TUNE_GEMM_WAVE = 5 };
 int tune_value(TuneKnob knob, int index);
 
 } // namespace bfa

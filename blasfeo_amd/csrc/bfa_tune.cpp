@@ -15,6 +15,7 @@ std::atomic<int> g_ric_threads[NCLASS];
 std::atomic<int> g_gemm_grid_y{0};
 std::atomic<int> g_potrf_bw{0};
 std::atomic<int> g_trsm_thin{0};
+std::atomic<int> g_gemm_wave{0};
 
 std::atomic<int> *slot(const char *name, int index)
 {
@@ -30,6 +31,8 @@ std::atomic<int> *slot(const char *name, int index)
 		return index == 0 ? &g_potrf_bw : nullptr;
 	if (!std::strcmp(name, "trsm_thin"))
 		return index == 0 ? &g_trsm_thin : nullptr;
+	if (!std::strcmp(name, "gemm_wave"))
+		return index == 0 ? &g_gemm_wave : nullptr;
 	return nullptr;
 }
 
@@ -43,7 +46,8 @@ bool valid(const char *name, int index, int value)
 	if (!std::strcmp(name, "potrf_bw"))
 		return value == 1 || value == 2 || value == 4;
 	// trsm_thin: 1 = always the one-wavefront left solve when it applies, 2 = never
-	if (!std::strcmp(name, "trsm_thin"))
+	// gemm_wave: 1 = the one-wavefront-per-tile gemm when the sizes allow it, 2 = never
+	if (!std::strcmp(name, "trsm_thin") || !std::strcmp(name, "gemm_wave"))
 		return value == 1 || value == 2;
 	// ric_threads: one wave only for the classes whose stages have nu+nx <= 64
 	return value == 128 || value == 256 || (value == 64 && index <= 4);
@@ -64,6 +68,8 @@ int tune_value(TuneKnob knob, int index)
 		return g_potrf_bw.load();
 	case TUNE_TRSM_THIN:
 		return g_trsm_thin.load();
+	case TUNE_GEMM_WAVE:
+		return g_gemm_wave.load();
 	}
 	return 0;
 }

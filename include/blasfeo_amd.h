@@ -140,7 +140,12 @@ const char *blasfeo_amd_last_error(void);
  *   "trsm_thin",   index 0: left-side solves (llnn, lltn, llnu, lunn) with
  *                  hint->max_n <= 4 and m <= 256 on one wavefront per instance:
  *                  1 always, 2 never; built-in: when the batch has at least
- *                  16 instances per CU.
+ *                  16 instances per CU;
+ *   "gemm_wave",   index 0: gemm_nt/nn and syrk_ln(_mn) with size bounds
+ *                  up to 256 on one wavefront per 32 x 32 output tile
+ *                  (operands from global memory): 1 or 0 (built-in) when the
+ *                  hint bounds m and n by 256, 2 never (the LDS-staged
+ *                  64 x 64 engine).
  * blasfeo_amd_tune_set returns BLASFEO_AMD_OK or BLASFEO_AMD_EINVAL;
  * blasfeo_amd_tune_get returns the value (0 = built-in) or BLASFEO_AMD_EINVAL. */
 int blasfeo_amd_tune_set(const char *name, int index, int value);

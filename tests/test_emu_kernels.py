@@ -66,3 +66,13 @@ def test_trsm_left_nonfinite(emu_build, op, thin):
     test test_trsm_left_nonfinite): Inf and NaN only where the reference has
     them, on the one-wavefront (thin) and the four-wave kernel."""
     run_case("case_trsm.py", "nonfinite", op, thin)
+
+
+@pytest.mark.parametrize("wave", [1, 2])
+@pytest.mark.parametrize("ps", [4, 8])
+@pytest.mark.parametrize("op", ["gemm_nt", "gemm_nn", "syrk_ln", "syrk_ln_mn"])
+def test_gemm_family(emu_build, op, ps, wave):
+    """gemm_nt / gemm_nn / syrk_ln / syrk_ln_mn on uneven sizes up to 139 with
+    offsets, in place on every fifth instance, k = 0 and edge shapes:
+    wave = 1 one wavefront per 32 x 32 tile, wave = 2 the LDS-staged engine."""
+    run_case("case_gemm.py", op, ps, wave)

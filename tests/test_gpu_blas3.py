@@ -38,10 +38,14 @@ def _gemm_records(b, rng, nrec, op, maxdim=140):
     return recs
 
 
+@pytest.mark.parametrize("engine", ["wave", "lds"])
 @pytest.mark.parametrize("prec", ["d", "s"])
 @pytest.mark.parametrize("ps", [4, 8])
 @pytest.mark.parametrize("op", ["gemm_nt", "gemm_nn", "syrk_ln", "syrk_ln_mn"])
-def test_gemm_family(gpu, prec, ps, op):
+def test_gemm_family(gpu, prec, ps, op, engine):
+    """engine: 'wave' (size bounds <= 256: one wavefront per 32 x 32 tile, the
+    built-in choice here) or 'lds' (the LDS-staged 64 x 64 engine, forced with
+    the gemm_wave knob)."""
     if prec == "s" and op == "syrk_ln_mn":
         pytest.skip("no FP32 syrk_ln_mn entry")
     rng = np.random.default_rng(100 + ps)
@@ -62,8 +66,12 @@ def test_gemm_family(gpu, prec, ps, op):
 
     argdt = bfa.DGEMM_ARG if prec == "d" else bfa.SGEMM_ARG
     hint = (160, 160, 160)
-    g, gd, rr, rd = run_both(b, buf, argdt, recs, prec + op, ocall, hint)
-    compare_buffers(b, g, rr, "%s%s ps=%d" % (prec, op, ps))
+    assert bfa.lib().blasfeo_amd_tune_set(b"gemm_wave", 0, 0 if engine == "wave" else 2) == 0
+    try:
+        g, gd, rr, rd = run_both(b, buf, argdt, recs, prec + op, ocall, hint)
+    finally:
+        bfa.lib().blasfeo_amd_tune_set(b"gemm_wave", 0, 0)
+    compare_buffers(b, g, rr, "%s%s ps=%d %s" % (prec, op, ps, engine))
     assert (gd["use_dA"] == rd["use_dA"]).all()
 
 

@@ -21,7 +21,7 @@ import numpy as np
 
 __all__ = ["lib", "lib_path", "Layout", "DMAT", "SMAT", "DGEMM_ARG", "SGEMM_ARG",
            "DTRSM_ARG", "STRSM_ARG", "DFACT_ARG", "SFACT_ARG", "DSYRK_DPOTRF_ARG",
-           "RICCATI_ARG", "RICCATI_TRS_ARG", "DVEC", "DPACK_ARG", "DROWPE_ARG", "PM_DMAT", "PM_DGECP_ARG", "pm_dgecp", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
+           "RICCATI_ARG", "RICCATI_TRS_ARG", "RICCATI_SV_ARG", "DVEC", "DPACK_ARG", "DROWPE_ARG", "PM_DMAT", "PM_DGECP_ARG", "pm_dgecp", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libblasfeo_amd.so")
@@ -65,6 +65,8 @@ DPACK_ARG = np.dtype([(f, "i4") for f in ("m", "n", "lda", "ai", "aj")] +
 DROWPE_ARG = np.dtype([("kmax", "i4"), ("ipiv", "u8"), ("sA", "u8")], align=True)
 RICCATI_TRS_ARG = np.dtype([("N", "i4")] + [(f, "u8") for f in (
     "nx", "nu", "BAbt", "b", "rq", "L", "Pb", "ux", "pi", "work")], align=True)
+RICCATI_SV_ARG = np.dtype([("N", "i4")] + [(f, "u8") for f in (
+    "nx", "nu", "BAbt", "RSQrq", "L", "ux", "pi", "work_mat", "work_vec")], align=True)
 DPOSV_ARG = np.dtype([("n", "i4"), ("nrhs", "i4")] +
                      [(f, "i4") for f in ("ai", "aj", "bi", "bj", "xi", "xj", "li", "lj")] +
                      [("sA", "u8"), ("sB", "u8"), ("sX", "u8"), ("sL", "u8"), ("info", "u8")], align=True)
@@ -80,7 +82,7 @@ _BATCHED = [
     "dtrsm_llnn", "dtrsm_lltn", "dtrsm_llnu", "dtrsm_lunn", "drowpe",
     "dpotrf_l", "dpotrf_l_mn", "dsyrk_dpotrf_ln", "dgetrf_rp", "dgetrf_np",
     "sgemm_nt", "sgemm_nn", "ssyrk_ln", "strsm_rltn", "spotrf_l", "sgetrf_rp",
-    "d_back_ric_trf", "d_back_ric_trs", "dpack", "dpack_tran", "dunpack",
+    "d_back_ric_trf", "d_back_ric_trs", "d_back_ric_sv", "dpack", "dpack_tran", "dunpack",
 ]
 
 

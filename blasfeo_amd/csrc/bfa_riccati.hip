@@ -148,7 +148,11 @@ __device__ __forceinline__ void ric_trmm_w(const TileLower M, const TileRect W, 
 // M = lower(gR) + W W^T instead, the accumulators started from the nk x nk
 // lower triangle of gR in global memory (zero elsewhere), so that M needs no
 // separate staging pass.
-template <int NT, int PS = 4, bool FROMG = false>
+// AUG: the (nk+1) x nk trapezoid of the fused sweep (one more row of W and
+// M, the linear-term row): row tiles of nk+1 rows, column tiles of nk
+// columns, and the columns from nk on written as zero (the Cholesky needs its
+// padding columns zero; W's row nk is not padding).
+template <int NT, int PS = 4, bool FROMG = false, bool AUG = false>
 __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, int nkt, int n1,
 	const PM<double, PS> gR = PM<double, PS>{}, int nk = 0)
 {
@@ -156,15 +160,17 @@ __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, 
 	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
 	const int w = wave_id();
+	const int mrt = AUG ? (nk + 1 + 15) / 16 : nkt;   // row tiles
+	const int nrows = AUG ? nk + 1 : nk;
 	int ntask = 0;
 	for (int bj = 0; bj < nkt; bj++)
-		ntask += (nkt - bj + 3) / 4;
+		ntask += (mrt - bj + 3) / 4;
 	for (int task = w; task < ntask; task += NW)
 	{
 		int bj = 0, rem = task;
-		while (rem >= (nkt - bj + 3) / 4)
+		while (rem >= (mrt - bj + 3) / 4)
 		{
-			rem -= (nkt - bj + 3) / 4;
+			rem -= (mrt - bj + 3) / 4;
 			bj++;
 		}
 		const int g = bj + 4 * rem;
@@ -172,13 +178,13 @@ __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, 
 #pragma unroll
 		for (int t = 0; t < 4; t++)
 		{
-			if (g + t < nkt)
+			if (g + t < mrt)
 #pragma unroll
 				for (int r = 0; r < 4; r++)
 				{
 					const int row = (g + t) * 16 + acc_row<double>(lane, r), col = bj * 16 + (lane & 15);
 					if (FROMG)
-						acc[t][r] = (row < nk && col < nk && row >= col) ? gR.at(row, col) : 0.0;
+						acc[t][r] = (row < nrows && col < nk && row >= col) ? gR.at(row, col) : 0.0;
 					else
 						acc[t][r] = M(row, col);
 				}
@@ -190,7 +196,7 @@ __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, 
 #pragma unroll
 			for (int t = 0; t < 4; t++)
 			{
-				if (g + t < nkt)
+				if (g + t < mrt)
 				{
 					const double av = W((g + t) * 16 + (lane & 15), lk);
 					acc[t] = mfma16(av, bv, acc[t]);
@@ -200,10 +206,13 @@ __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, 
 #pragma unroll
 		for (int t = 0; t < 4; t++)
 		{
-			if (g + t < nkt)
+			if (g + t < mrt)
 #pragma unroll
 				for (int r = 0; r < 4; r++)
-					M((g + t) * 16 + acc_row<double>(lane, r), bj * 16 + (lane & 15)) = acc[t][r];
+				{
+					const int col = bj * 16 + (lane & 15);
+					M((g + t) * 16 + acc_row<double>(lane, r), col) = (AUG && col >= nk) ? 0.0 : acc[t][r];
+				}
 		}
 	}
 }
@@ -228,10 +237,21 @@ __device__ __attribute__((noinline)) void ric_chol_call(const TileLower M, int n
 	chol_acc<double, NT, NMAXC, TileLower, false>(TileLower{lds_opaque(M.p)}, nk, nk, lds_opaque(invd));
 }
 
-template <int PS, int NT>
-__device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
+// the work matrix of a record: trf's `work`, the fused sweep's `work_mat`
+__device__ __forceinline__ blasfeo_dmat *ric_work(const blasfeo_amd_riccati_arg &a) { return a.work; }
+__device__ __forceinline__ blasfeo_dmat *ric_work(const blasfeo_amd_riccati_sv_arg &a) { return a.work_mat; }
+
+// AUG = true: the backward part of the fused sweep (d_back_ric_sv_libstr,
+// examples/example_d_riccati_recursion.c:58-79): every stage matrix has one
+// more row, the linear terms (BAbt[k]'s row nk = b^T, RSQrq[k]'s row nk =
+// [r; q]^T), carried through the trmm, the syrk and the trapezoid Cholesky
+// (L[k] is (nk+1) x nk); before the syrk the row of the previous factor is
+// added to W's last row (the reference's blasfeo_dgead).
+template <int PS, int NT, bool AUG = false, typename ArgT = blasfeo_amd_riccati_arg>
+__device__ void ric_sweep_lds(const ArgT &a, double *smem)
 {
 	constexpr int NMAXC = (NT == WAVE) ? 64 : 128;
+	constexpr int XR = AUG ? 1 : 0;   // extra row of the augmented form
 	const int N = a.N;
 	const int *nx = a.nx;
 	const int *nu = a.nu;
@@ -244,29 +264,31 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 		nkmax = max(nkmax, nu[k] + nx[k]);
 		nxmax = max(nxmax, nx[k]);
 	}
-	const int nkt_max = (nkmax + 15) / 16;
+	const int nkt_max = (nkmax + XR + 15) / 16;
 	const int nxt_max = (nxmax + 15) / 16;
 	const int nlow = nkt_max * (nkt_max + 1) / 2;
 	const TileLower M{smem};
 	const TileRect W{smem + (size_t)nlow * RT_SZ, nxt_max};
 	double *invd = smem + (size_t)(nlow + nkt_max * nxt_max) * RT_SZ;
 
-	// s = -1 is the last stage, L[N] = chol(Q_N) (potrf_l); s >= 0 is stage
-	// k = N-1-s.  One call site of the in-LDS Cholesky.  Inlined, it addresses
-	// the kernel's dynamic LDS directly; out of line, every LDS access of the
-	// callee first loads the kernel's LDS offset from a table and waits for it.
+	// s = -1 is the last stage, L[N] = chol(Q_N) (potrf_l; potrf_l_mn of the
+	// (nx+1) x nx trapezoid when AUG); s >= 0 is stage k = N-1-s.  One call
+	// site of the in-LDS Cholesky.  Inlined, it addresses the kernel's dynamic
+	// LDS directly; out of line, every LDS access of the callee first loads
+	// the kernel's LDS offset from a table and waits for it.
 	for (int s = -1; s < N; s++)
 	{
 		const bool last = s < 0;
 		const int k = last ? N : N - 1 - s;
 		const int nk = last ? nx[N] : nu[k] + nx[k];
-		const int nkt = (nk + 15) / 16;
+		const int nkt = (nk + 15) / 16;         // column tiles
+		const int mrt = (nk + XR + 15) / 16;    // row tiles
 		if (!last)
 		{
 			const int n1 = nx[k + 1];
 			const int u1 = nu[k + 1];
 			const int n1t = (n1 + 15) / 16;
-			// stage BAbt[k] (nk x n1) into W; M still holds L[k+1].  The
+			// stage BAbt[k] (nk(+1) x n1) into W; M still holds L[k+1].  The
 			// multi-wave kernels loaded it during the previous stage's
 			// Cholesky (its first panel, on the waves other than 0), so W and
 			// M are complete at the Cholesky's barriers; without a barrier
@@ -274,19 +296,27 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 			// overlap the trmm (which only reads M and writes W)
 			if constexpr (NT == WAVE)
 			{
-				g2s_acc<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, nkt * 16, n1t * 16, nk, n1, W);
+				g2s_acc<double, PS, NT, false>(view<double, PS>(&BAbt[k]), 0, 0, mrt * 16, n1t * 16, nk + XR, n1, W);
 				__syncthreads();
 			}
 			if constexpr (NT == WAVE)
-				ric_trmm_call<NT>(M, W, nkt, n1, n1t, u1);
+				ric_trmm_call<NT>(M, W, mrt, n1, n1t, u1);
 			else
-				ric_trmm_w<NT>(M, W, nkt, n1, n1t, u1);
+				ric_trmm_w<NT>(M, W, mrt, n1, n1t, u1);
 			__syncthreads();
+			if constexpr (AUG)
+			{
+				// W(nk, :) += L[k+1](nu1 + nx1, nu1:) (blasfeo_dgead): the linear
+				// term's row of the previous factor, still in M
+				for (int j = threadIdx.x; j < n1; j += NT)
+					W(nk, j) += M(u1 + n1, u1 + j);
+				__syncthreads();
+			}
 		}
 		// M <- lower(RSQ[k]) + W W^T (L[k+1] is no longer needed); the
 		// multi-wave kernels start the syrk accumulators from RSQ[k] itself
 		if (last || NT == WAVE)
-			g2s_acc<double, PS, NT, true>(view<double, PS>(&RSQ[k]), 0, 0, nkt * 16, nkt * 16, nk, nk, M);
+			g2s_acc<double, PS, NT, true>(view<double, PS>(&RSQ[k]), 0, 0, mrt * 16, nkt * 16, nk + XR, nk, M);
 		if (!last)
 		{
 			if constexpr (NT == WAVE)
@@ -295,7 +325,7 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 				ric_syrk_call<NT>(M, W, nkt, nx[k + 1]);
 			}
 			else
-				ric_syrk_m<NT, PS, true>(M, W, nkt, nx[k + 1], view<double, PS>(&RSQ[k]), nk);
+				ric_syrk_m<NT, PS, true, AUG>(M, W, nkt, nx[k + 1], view<double, PS>(&RSQ[k]), nk);
 		}
 		// L[k] = chol(M); W is free (the syrk has read it), so the waves
 		// other than 0 load the next stage's BAbt into it meanwhile
@@ -309,20 +339,21 @@ __device__ void ric_sweep_lds(const blasfeo_amd_riccati_arg &a, double *smem)
 				{
 					const int nkn = nu[kn] + nx[kn], n1n = nx[kn + 1];
 					g2s_acc<double, PS, NT, false, TileRect, 1>(view<double, PS>(&BAbt[kn]), 0, 0,
-						(nkn + 15) / 16 * 16, (n1n + 15) / 16 * 16, nkn, n1n, W);
+						(nkn + XR + 15) / 16 * 16, (n1n + 15) / 16 * 16, nkn + XR, n1n, W);
 				}
 			};
-			chol_acc<double, NT, NMAXC, TileLower, true, decltype(side)>(M, nk, nk, invd, side);
+			chol_acc<double, NT, NMAXC, TileLower, true, decltype(side)>(M, nk + XR, nk, invd, side);
 		}
-		s2g_acc<double, PS, NT, true>(view<double, PS>(&L[k]), 0, 0, nk, nk, M, 0);
+		s2g_acc<double, PS, NT, true>(view<double, PS>(&L[k]), 0, 0, nk + XR, nk, M, 0);
 		for (int j = threadIdx.x; j < nk; j += NT)
 			L[k].dA[j] = invd[j];
 		if (threadIdx.x == 0)
 		{
-			// potrf_l marks the whole diagonal cached, syrk_dpotrf_ln one entry
-			L[k].use_dA = last ? nk : 1;
+			// potrf_l marks the whole diagonal cached; syrk_dpotrf_ln and the
+			// trapezoid forms (potrf_l_mn, syrk_dpotrf_ln_mn) one entry
+			L[k].use_dA = (last && !AUG) ? nk : 1;
 			if (!last)
-				a.work->use_dA = 0;
+				ric_work(a)->use_dA = 0;
 		}
 	}
 	__syncthreads();
@@ -395,13 +426,22 @@ __global__ void k_ric_zero(int *counts)
 // class list.  Instances with a negative horizon or size are dropped (the
 // reference has no defined behaviour for them); those larger than mmax are
 // dropped from the general class, which cannot hold them.
-__global__ void k_ric_plan(const blasfeo_amd_riccati_arg *__restrict__ args, int batch, int mmax,
+// The argument record of the sweep: trf's, or the fused sweep's (AUG).
+template <bool AUG> struct RicArg { typedef blasfeo_amd_riccati_arg type; };
+template <> struct RicArg<true> { typedef blasfeo_amd_riccati_sv_arg type; };
+
+// AUG: stage matrices have one more row; no global-memory sweep serves them,
+// so instances that fit no LDS class are dropped (the entry refuses size
+// bounds that allow them).
+template <bool AUG = false>
+__global__ void k_ric_plan(const typename RicArg<AUG>::type *__restrict__ args, int batch, int mmax,
 	int *counts, int *lists)
 {
+	constexpr int XR = AUG ? 1 : 0;
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= batch)
 		return;
-	const blasfeo_amd_riccati_arg &a = args[i];
+	const typename RicArg<AUG>::type &a = args[i];
 	const int N = a.N;
 	if (N < 0)
 		return;
@@ -417,15 +457,15 @@ __global__ void k_ric_plan(const blasfeo_amd_riccati_arg *__restrict__ args, int
 	if (bad)
 		return;
 	int cls = RIC_GENERAL;
-	if (nxmax <= 64 && nkmax <= 128)
+	if (nxmax <= 64 && nkmax + XR <= 128)
 	{
 		// one-wave classes (cap <= 40960) factor with one panel row per lane: nkmax <= 64
-		const int f = ric_footprint(nkmax, nxmax);
+		const int f = ric_footprint(nkmax + XR, nxmax);
 		for (int c = RIC_NCLASS - 1; c >= 0; c--)
-			if (f <= c_ric_cap[c] && (c_ric_cap[c] > 40960 || nkmax <= 64))
+			if (f <= c_ric_cap[c] && (c_ric_cap[c] > 40960 || nkmax + XR <= 64))
 				cls = c;
 	}
-	if (cls == RIC_GENERAL && nkmax > mmax)
+	if (cls == RIC_GENERAL && (AUG || nkmax > mmax))
 		return;
 	const int slot = atomicAdd(&counts[cls], 1);
 	lists[(size_t)cls * batch + slot] = i;
@@ -456,16 +496,17 @@ __device__ __forceinline__ int ric_next(int *head, int *slot)
 	}
 }
 
-template <int PS, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_ric_class(const blasfeo_amd_riccati_arg *__restrict__ args,
+template <int PS, int NT, bool AUG = false>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_ric_class(const typename RicArg<AUG>::type *__restrict__ args,
 	const int *__restrict__ count, const int *__restrict__ list, int *head)
 {
+	static_assert(!(AUG && NT == WAVE), "the fused sweep runs on two or four waves per instance");
 	extern __shared__ double smem_d[];
 	int *slot = reinterpret_cast<int *>(smem_d);
 	double *work = smem_d + RIC_QSLOT / sizeof(double);
 	const int cnt = *count;
 	for (int t = ric_next<NT>(head, slot); t < cnt; t = ric_next<NT>(head, slot))
-		ric_sweep_lds<PS, NT>(args[list[t]], work);
+		ric_sweep_lds<PS, NT, AUG>(args[list[t]], work);
 }
 
 // one wave per instance (selected by the tuning knobs, not by default):
@@ -500,21 +541,33 @@ __global__ __launch_bounds__(NT3) void k_ric_general(const blasfeo_amd_riccati_a
 		ric_sweep_general<PS>(args[list[t]], P, LDP, mmax, As, Bs, L11, invs);
 }
 
-template <int PS, int NT>
-static int launch_class(int cls, int grid, const blasfeo_amd_riccati_arg *args, int *counts, const int *lists,
+template <int PS, int NT, bool AUG = false>
+static int launch_class(int cls, int grid, const typename RicArg<AUG>::type *args, int *counts, const int *lists,
 	int batch, hipStream_t st, const char *name)
 {
-	const void *fn = NT == WAVE ? reinterpret_cast<const void *>(&k_ric_class1<PS>)
-	                            : reinterpret_cast<const void *>(&k_ric_class<PS, NT>);
-	if (const int rc = lds_check(fn, h_ric_cap[cls], name))
-		return rc;
-	if (NT == WAVE)
-		hipLaunchKernelGGL((k_ric_class1<PS>), dim3(grid), dim3(WAVE), h_ric_cap[cls], st, args,
+	if constexpr (AUG)
+	{
+		static_assert(NT != WAVE, "the fused sweep runs on two or four waves per instance");
+		if (const int rc = lds_check(reinterpret_cast<const void *>(&k_ric_class<PS, NT, true>), h_ric_cap[cls], name))
+			return rc;
+		hipLaunchKernelGGL((k_ric_class<PS, NT, true>), dim3(grid), dim3(NT), h_ric_cap[cls], st, args,
 			counts + cls, lists + (size_t)cls * batch, counts + RIC_HEAD + cls);
+		return check_launch(name);
+	}
 	else
-		hipLaunchKernelGGL((k_ric_class<PS, NT>), dim3(grid), dim3(NT), h_ric_cap[cls], st, args,
-			counts + cls, lists + (size_t)cls * batch, counts + RIC_HEAD + cls);
-	return check_launch(name);
+	{
+		const void *fn = NT == WAVE ? reinterpret_cast<const void *>(&k_ric_class1<PS>)
+		                            : reinterpret_cast<const void *>(&k_ric_class<PS, NT>);
+		if (const int rc = lds_check(fn, h_ric_cap[cls], name))
+			return rc;
+		if (NT == WAVE)
+			hipLaunchKernelGGL((k_ric_class1<PS>), dim3(grid), dim3(WAVE), h_ric_cap[cls], st, args,
+				counts + cls, lists + (size_t)cls * batch, counts + RIC_HEAD + cls);
+		else
+			hipLaunchKernelGGL((k_ric_class<PS, NT>), dim3(grid), dim3(NT), h_ric_cap[cls], st, args,
+				counts + cls, lists + (size_t)cls * batch, counts + RIC_HEAD + cls);
+		return check_launch(name);
+	}
 }
 
 // Helper streams (default priority) on which the class launches run side by
@@ -555,14 +608,14 @@ static RicFork *ric_fork_state()
 
 // ws: the plan workspace, (RIC_NCLASS + 1) * batch + 64 ints.  fork: run the
 // class launches side by side on helper streams (else all on st, in order).
-template <int PS>
-static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, int maxfoot, int *ws, bool fork,
+template <int PS, bool AUG = false>
+static int launch_ric(int batch, const typename RicArg<AUG>::type *args, int mmax, int maxfoot, int *ws, bool fork,
 	hipStream_t st, const char *name)
 {
 	int *counts = ws;
 	int *lists = ws + 64;
 	hipLaunchKernelGGL(k_ric_zero, dim3(1), dim3(64), 0, st, counts);
-	hipLaunchKernelGGL(k_ric_plan, dim3((batch + 255) / 256), dim3(256), 0, st, args, batch, mmax, counts, lists);
+	hipLaunchKernelGGL((k_ric_plan<AUG>), dim3((batch + 255) / 256), dim3(256), 0, st, args, batch, mmax, counts, lists);
 	int rc = check_launch(name);
 	if (rc)
 		return rc;
@@ -588,6 +641,7 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 	// general class first (largest instances), then resident classes largest
 	// first, dealt round-robin over the caller's and the helper streams
 	int q = 0;
+	if constexpr (!AUG)
 	{
 		const size_t shm = sizeof(double) * PW * ((mmax + 15) / 16 * 16 + 1);
 		const int grid = min(batch, cus);
@@ -631,12 +685,12 @@ static int launch_ric(int batch, const blasfeo_amd_riccati_arg *args, int mmax, 
 			nt = v;
 		const int grid = min(batch, cus * per_cu);
 		hipStream_t sc = ss[q++ % (1 + RIC_AUX)];
-		if (nt == 64)
-			rc = launch_class<PS, 64>(cls, grid, args, counts, lists, batch, sc, name);
-		else if (nt == 128)
-			rc = launch_class<PS, 128>(cls, grid, args, counts, lists, batch, sc, name);
+		if (nt == 64 && !AUG)
+			rc = launch_class<PS, AUG ? 128 : 64, AUG>(cls, grid, args, counts, lists, batch, sc, name);
+		else if (nt <= 128)
+			rc = launch_class<PS, 128, AUG>(cls, grid, args, counts, lists, batch, sc, name);
 		else
-			rc = launch_class<PS, 256>(cls, grid, args, counts, lists, batch, sc, name);
+			rc = launch_class<PS, 256, AUG>(cls, grid, args, counts, lists, batch, sc, name);
 		if (rc)
 			break;
 	}
@@ -740,4 +794,197 @@ extern "C" int blasfeo_amd_d_back_ric_trf_batched_noalloc(int ps, int batch, con
 	hipStream_t st = (hipStream_t)stream;
 	return ps == 4 ? launch_ric<4>(batch, args, mmax, maxfoot, ws, false, st, name)
 	               : launch_ric<8>(batch, args, mmax, maxfoot, ws, false, st, name);
+}
+
+// ---------------------------------------------------------------------------
+// The fused sweep's forward substitution (d_back_ric_sv_libstr,
+// examples/example_d_riccati_recursion.c:81-107), one wavefront per instance,
+// k = 0 .. N-1 (nk = nu[k] + nx[k], ns = nk on stage 0 and nu[k] after it):
+//   ux[k](0:ns) = -L[k](nk, 0:ns)                         (the factor's last row)
+//   ux[k] = trsv_ltn_mn(nk, ns) of L[k] on ux[k]            (ux[k](ns:) from stage k-1)
+//   ux[k+1](nu1:) = BAbt[k](nk, :)^T + BAbt[k]^T ux[k]      (gemv_t; the row nk is b)
+//   pi[k] = Lxx(k+1) (Lxx(k+1)^T ux[k+1](nu1:) + L[k+1](nk1, nu1:)^T)
+// The substitution follows the Riccati solve's (bfa_ric_trs.hip): lanes own
+// rows, the column sums are wave reductions, one column's coefficients and
+// reciprocal loaded ahead of the chain; rows up to 128 (two per lane).
+// ---------------------------------------------------------------------------
+namespace bfa {
+
+constexpr int SV_MAX = 2 * WAVE;
+
+template <int PS>
+__global__ __launch_bounds__(WAVE) void k_ric_sv_fwd(const blasfeo_amd_riccati_sv_arg *__restrict__ args, int batch)
+{
+	__shared__ double s_x[SV_MAX];
+	__shared__ double s_t[SV_MAX];
+	const int inst = blockIdx.x;
+	const int lane = threadIdx.x;
+	if (inst >= batch)
+		return;
+	const blasfeo_amd_riccati_sv_arg &a = args[inst];
+	const int N = a.N;
+	for (int k = 0; k < N; k++)
+	{
+		const int nu0 = a.nu[k], n0 = nu0 + a.nx[k];
+		const int nu1 = a.nu[k + 1], nx1 = a.nx[k + 1], nk1 = nu1 + nx1;
+		const blasfeo_dmat &Lk = a.L[k];
+		const blasfeo_dmat &Lk1 = a.L[k + 1];
+		const blasfeo_dmat &Bk = a.BAbt[k];
+		const PM<double, PS> L0{Lk.pA, Lk.cn}, L1{Lk1.pA, Lk1.cn}, B{Bk.pA, Bk.cn};
+		double *ux0 = a.ux[k].pa;
+		double *ux1 = a.ux[k + 1].pa;
+		double *pi = a.pi[k].pa;
+		const int ns = k == 0 ? n0 : nu0;
+		double z[2];
+#pragma unroll
+		for (int q = 0; q < 2; q++)
+		{
+			const int i = lane + q * WAVE;
+			z[q] = i < ns ? -L0.at(n0, i) : (i < n0 ? ux0[i] : 0.0);
+		}
+		// z(j) = (z(j) - sum_{i > j} L(i, j) z(i)) / L(j, j), j = ns-1 .. 0
+		const bool cached = Lk.use_dA == 1;
+		const double *dA = Lk.dA;
+		auto coef = [&](int j, double (&e)[2]) {
+#pragma unroll
+			for (int q = 0; q < 2; q++)
+			{
+				const int i = lane + q * WAVE;
+				e[q] = (i > j && i < n0) ? L0.at(i, j) : 0.0;
+			}
+		};
+		auto recip = [&](int j) { return cached ? dA[j] : 1.0 / L0.at(j, j); };
+		double e[2] = {0.0, 0.0}, d = 0.0;
+		if (ns > 0)
+		{
+			coef(ns - 1, e);
+			d = recip(ns - 1);
+		}
+		for (int j = ns - 1; j >= 0; j--)
+		{
+			double en[2] = {0.0, 0.0}, dn = 0.0;
+			const bool more = j > 0;
+			if (more)
+			{
+				coef(j - 1, en);
+				dn = recip(j - 1);
+			}
+			const int i0 = lane, i1 = lane + WAVE;
+			const double part = (i0 > j && i0 < n0 ? e[0] * z[0] : 0.0) + (i1 > j && i1 < n0 ? e[1] * z[1] : 0.0);
+			const double sum = readlane(wave_sum_d(part), WAVE - 1);
+			if (!cached && lane == 0)
+				Lk.dA[j] = d;
+#pragma unroll
+			for (int q = 0; q < 2; q++)
+				if (lane + q * WAVE == j)
+					z[q] = (z[q] - sum) * d;
+			if (more)
+			{
+				e[0] = en[0];
+				e[1] = en[1];
+				d = dn;
+			}
+		}
+		if (!cached && n0 > 0 && lane == 0)
+			a.L[k].use_dA = 1;
+#pragma unroll
+		for (int q = 0; q < 2; q++)
+		{
+			const int i = lane + q * WAVE;
+			if (i < n0)
+			{
+				ux0[i] = z[q];
+				s_x[i] = z[q];
+			}
+		}
+		__syncthreads();
+		// x(k+1) = b + BAbt^T z (gemv_t), one column per lane; pi starts as it
+		for (int j = lane; j < nx1; j += WAVE)
+		{
+			double acc = 0.0;
+#pragma unroll 8
+			for (int i = 0; i < n0; i++)
+				acc += B.at(i, j) * s_x[i];
+			const double y = B.at(n0, j) + acc;
+			ux1[nu1 + j] = y;
+			s_t[j] = y;
+		}
+		__syncthreads();
+		// t = Lxx^T x + L[k+1](nk1, nu1:)^T (trmv_ltn, axpy), one column per lane
+		double tv[2];
+#pragma unroll
+		for (int q = 0; q < 2; q++)
+		{
+			const int j = lane + q * WAVE;
+			double acc = 0.0;
+			if (j < nx1)
+			{
+				for (int i = j; i < nx1; i++)
+					acc += L1.at(nu1 + i, nu1 + j) * s_t[i];
+				acc += L1.at(nk1, nu1 + j);
+			}
+			tv[q] = acc;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int q = 0; q < 2; q++)
+		{
+			const int j = lane + q * WAVE;
+			if (j < nx1)
+				s_x[j] = tv[q];
+		}
+		__syncthreads();
+		// pi = Lxx t (trmv_lnn), one row per lane
+#pragma unroll
+		for (int q = 0; q < 2; q++)
+		{
+			const int i = lane + q * WAVE;
+			if (i < nx1)
+			{
+				double acc = 0.0;
+				for (int j = 0; j <= i; j++)
+					acc += L1.at(nu1 + i, nu1 + j) * s_x[j];
+				pi[i] = acc;
+			}
+		}
+		__syncthreads();
+	}
+}
+
+} // namespace bfa
+
+extern "C" int blasfeo_amd_d_back_ric_sv_batched(int ps, int batch, const blasfeo_amd_riccati_sv_arg *args, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream)
+{
+	using namespace bfa;
+	const char *name = "blasfeo_amd_d_back_ric_sv_batched";
+	if (batch < 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: batch=%d < 0", name, batch);
+	if (batch == 0)
+		return BLASFEO_AMD_OK;
+	if (!args)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: args is NULL", name);
+	if (ps != 4 && ps != 8)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: unsupported panel size %d", name, ps);
+	if (!hint || hint->max_m <= 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: hint->max_m (largest nu[k]+nx[k]) is required", name);
+	const int mmax = hint->max_m;
+	const int nxmax = hint->max_n > 0 ? hint->max_n : mmax;
+	if (mmax + 1 > 128 || nxmax > 64)
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: nu+nx up to 127 and nx up to 64 supported, got %d and %d",
+			name, mmax, nxmax);
+	hipStream_t st = (hipStream_t)stream;
+	const size_t wsbytes = sizeof(int) * ((size_t)(RIC_NCLASS + 1) * batch + 64);
+	int *ws = (int *)workspace(wsbytes, st);
+	if (!ws)
+		return set_error(BLASFEO_AMD_ENOMEM, "%s: cannot allocate the %zu-byte plan workspace", name, wsbytes);
+	const int maxfoot = ric_footprint(mmax + 1, min(nxmax, mmax));
+	int rc = ps == 4 ? launch_ric<4, true>(batch, args, mmax, maxfoot, ws, true, st, name)
+	                 : launch_ric<8, true>(batch, args, mmax, maxfoot, ws, true, st, name);
+	if (rc)
+		return rc;
+	if (ps == 4)
+		hipLaunchKernelGGL(k_ric_sv_fwd<4>, dim3(batch), dim3(WAVE), 0, st, args, batch);
+	else
+		hipLaunchKernelGGL(k_ric_sv_fwd<8>, dim3(batch), dim3(WAVE), 0, st, args, batch);
+	return check_launch(name);
 }

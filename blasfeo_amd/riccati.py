@@ -16,15 +16,23 @@ Outputs: L[0..N-1] (nu+nx) x (nu+nx), L[N] nx x nx, plus one work matrix.
 """
 import numpy as np
 
-from . import DMAT, DVEC, Layout, RICCATI_ARG, RICCATI_TRS_ARG
+from . import DMAT, DVEC, Layout, RICCATI_ARG, RICCATI_TRS_ARG, RICCATI_SV_ARG
 
 
 class RiccatiBatch:
-    def __init__(self, nxs, nus, N, ps=4):
+    """aug: every matrix has one more row, as d_back_ric_sv_libstr takes them
+    (examples/example_d_riccati_recursion.c:58-111): BAbt's last row holds
+    b^T, RSQ's and QN's the linear term [r; q]^T, and the factors are
+    (nu+nx+1) x (nu+nx) trapezoids.  The factorisation alone ignores the
+    extra row, so one augmented batch serves both forms."""
+
+    def __init__(self, nxs, nus, N, ps=4, aug=False):
         self.nxs = np.asarray(nxs, dtype=np.int64)
         self.nus = np.asarray(nus, dtype=np.int64)
         self.N = int(N)
         self.ps = ps
+        self.aug = bool(aug)
+        a = 1 if aug else 0
         self.batch = len(self.nxs)
         self.inp = Layout(np.float64, ps)
         self.out = Layout(np.float64, ps)
@@ -32,27 +40,33 @@ class RiccatiBatch:
         self.i_L, self.i_work = [], []
         for nx, nu in zip(self.nxs, self.nus):
             nx, nu = int(nx), int(nu)
-            self.i_babt.append(self.inp.add(nu + nx, nx))
-            self.i_rsq.append(self.inp.add(nu + nx, nu + nx))
-            self.i_qn.append(self.inp.add(nx, nx))
-            Ls = [self.out.add(nu + nx, nu + nx) for _ in range(self.N)]
-            Ls.append(self.out.add(nx, nx))
+            self.i_babt.append(self.inp.add(nu + nx + a, nx))
+            self.i_rsq.append(self.inp.add(nu + nx + a, nu + nx))
+            self.i_qn.append(self.inp.add(nx + a, nx))
+            Ls = [self.out.add(nu + nx + a, nu + nx) for _ in range(self.N)]
+            Ls.append(self.out.add(nx + a, nx))
             self.i_L.append(Ls)
-            self.i_work.append(self.out.add(nu + nx, nx))
+            self.i_work.append(self.out.add(nu + nx + a, nx))
         self.max_nk = int((self.nxs + self.nus).max()) if self.batch else 0
         self.max_nx = int(self.nxs.max()) if self.batch else 0
 
     # -- data ---------------------------------------------------------------
     def fill_inputs(self, rng, A_scale=0.1, R=None, Q=None):
-        """A = I + A_scale*N(0,1), B ~ N(0,1), R = Q = I, S = 0 (host buffer)."""
+        """A = I + A_scale*N(0,1), B ~ N(0,1), R = Q = I, S = 0 (host buffer);
+        with aug, b, r and q ~ N(0,1) in the extra rows."""
         buf = self.inp.host_buffer()
         for i in range(self.batch):
             nx, nu = int(self.nxs[i]), int(self.nus[i])
             A = np.eye(nx) + A_scale * rng.standard_normal((nx, nx))
             B = rng.standard_normal((nx, nu))
-            self.inp.put(buf, self.i_babt[i], np.vstack([B.T, A.T]))
-            self.inp.put(buf, self.i_rsq[i], np.eye(nu + nx))
-            self.inp.put(buf, self.i_qn[i], np.eye(nx))
+            babt, rsq, qn = np.vstack([B.T, A.T]), np.eye(nu + nx), np.eye(nx)
+            if self.aug:
+                babt = np.vstack([babt, rng.standard_normal((1, nx))])
+                rsq = np.vstack([rsq, rng.standard_normal((1, nu + nx))])
+                qn = np.vstack([qn, rng.standard_normal((1, nx))])
+            self.inp.put(buf, self.i_babt[i], babt)
+            self.inp.put(buf, self.i_rsq[i], rsq)
+            self.inp.put(buf, self.i_qn[i], qn)
         return buf
 
     def fill_custom(self, fn):
@@ -205,10 +219,11 @@ def trs_backward_error(rb, inp, rs, sin, sol, i):
     ||BAbt_k|| ||z_k|| + ||b_k|| + ||x_{k+1}|| (infinity norms)."""
     eps = np.finfo(np.float64).eps
     N, nx, nu = rb.N, int(rb.nxs[i]), int(rb.nus[i])
-    C = rb.inp.get(inp, rb.i_babt[i])
-    H = np.tril(rb.inp.get(inp, rb.i_rsq[i]))
+    # square parts (an augmented batch has the linear terms in one more row)
+    C = rb.inp.get(inp, rb.i_babt[i])[:nu + nx]
+    H = np.tril(rb.inp.get(inp, rb.i_rsq[i])[:nu + nx])
     H = H + np.tril(H, -1).T
-    HN = np.tril(rb.inp.get(inp, rb.i_qn[i]))
+    HN = np.tril(rb.inp.get(inp, rb.i_qn[i])[:nx])
     HN = HN + np.tril(HN, -1).T
     nrm = lambda v: float(np.abs(v).max()) if np.size(v) else 0.0
     nC, nH, nHN = np.abs(C).sum(1).max(), np.abs(H).sum(1).max(), np.abs(HN).sum(1).max()
@@ -234,6 +249,54 @@ def trs_backward_error(rb, inp, rs, sin, sol, i):
             den = np.abs(C).sum(0).max() * nrm(z[k]) + nrm(b) + nrm(x1)
             worst = max(worst, nrm(r) / (den * eps))
     return worst
+
+
+class RiccatiSv:
+    """The output vectors ux[0..N] and pi[0..N-1] of the fused sweep
+    (d_back_ric_sv_libstr) for every instance of an augmented RiccatiBatch,
+    in the vector layout of a RiccatiSolve (its ux, pi and work entries)."""
+
+    def __init__(self, rb):
+        assert rb.aug, "the fused sweep takes an augmented RiccatiBatch"
+        self.rb = rb
+        self.rs = RiccatiSolve(rb)
+
+    def args(self, tab, addr_babt, addr_rsq, addr_L, addr_work, addr_nx, addr_nu):
+        rb, rs, N = self.rb, self.rs, self.rb.N
+        a = np.zeros(rb.batch, dtype=RICCATI_SV_ARG)
+        idx = np.arange(rb.batch, dtype=np.uint64)
+        a["N"] = N
+        a["nx"] = addr_nx + idx * 4 * (N + 1)
+        a["nu"] = addr_nu + idx * 4 * (N + 1)
+        a["BAbt"] = addr_babt + idx * DMAT.itemsize * max(N, 1)
+        a["RSQrq"] = addr_rsq + idx * DMAT.itemsize * (N + 1)
+        a["L"] = addr_L + idx * DMAT.itemsize * (N + 1)
+        a["work_mat"] = addr_work + idx * DMAT.itemsize
+        base = np.uint64(int(tab)) + idx * np.uint64(DVEC.itemsize * rs.per)
+        a["ux"] = base + np.uint64(DVEC.itemsize * rs.start["ux"])
+        a["pi"] = base + np.uint64(DVEC.itemsize * rs.start["pi"])
+        a["work_vec"] = base + np.uint64(DVEC.itemsize * rs.start["work"])
+        return a
+
+    def solve_inputs(self, inp):
+        """A RiccatiSolve input buffer whose b[k] and rq[k] are the extra rows
+        of the augmented inputs, for d_back_ric_trs_libstr on the same problem."""
+        rb, rs = self.rb, self.rs
+        buf = np.zeros(rs.nbytes, dtype=np.uint8)
+        f = buf.view(np.float64)
+        for i in range(rb.batch):
+            nx, nk = int(rb.nxs[i]), int(rb.nxs[i] + rb.nus[i])
+            b = rb.inp.get(inp, rb.i_babt[i])[nk]
+            rq = rb.inp.get(inp, rb.i_rsq[i])[nk]
+            q = rb.inp.get(inp, rb.i_qn[i])[nx]
+            for k in range(rb.N + 1):
+                if k < rb.N:
+                    o = int(rs.offs[rs.index(i, "b", k)]) // 8
+                    f[o: o + nx] = b
+                o = int(rs.offs[rs.index(i, "rq", k)]) // 8
+                v = rq if k < rb.N else q
+                f[o: o + len(v)] = v
+        return buf
 
 
 class DeviceRiccati:
@@ -296,6 +359,44 @@ class DeviceRiccatiSolve:
         return self.buf.cpu().numpy()
 
 
+class DeviceRiccatiSv:
+    """An augmented RiccatiBatch and its RiccatiSv vectors resident in GPU
+    memory, for ``blasfeo_amd_d_back_ric_sv_batched``."""
+
+    def __init__(self, rb, in_host_buf):
+        import torch
+        from .device import to_device
+        self.rb, self.sv = rb, RiccatiSv(rb)
+        self.fac = DeviceRiccati(rb, in_host_buf)
+        self.vbuf = torch.zeros(self.sv.rs.nbytes, dtype=torch.uint8, device="cuda")
+        self.tab = to_device(self.sv.rs.table(self.vbuf.data_ptr()))
+        babt, rsq, L, work, nx, nu = self.fac.t_dev
+        self.args = to_device(self.sv.args(self.tab.data_ptr(), babt.data_ptr(), rsq.data_ptr(), L.data_ptr(),
+                                           work.data_ptr(), nx.data_ptr(), nu.data_ptr()))
+
+    def run(self, stream=0):
+        from . import call
+        call("d_back_ric_sv", self.rb.ps, self.rb.batch, self.args.data_ptr(),
+             (self.rb.max_nk, self.rb.max_nx, 0), stream)
+
+    def vectors(self):
+        return self.vbuf.cpu().numpy()
+
+
+class HostRiccatiSv:
+    """The same augmented batch and vectors in host memory for the CPU oracle
+    (``or_dric_sv_batch``)."""
+
+    def __init__(self, rb, in_host_buf):
+        self.rb, self.sv = rb, RiccatiSv(rb)
+        self.fac = HostRiccati(rb, in_host_buf)
+        self.vbuf = np.zeros(self.sv.rs.nbytes, dtype=np.uint8)
+        self.tab = self.sv.rs.table(self.vbuf.ctypes.data)
+        babt, rsq, L, work, nx, nu = self.fac.t
+        self.argv = self.sv.args(self.tab.ctypes.data, babt.ctypes.data, rsq.ctypes.data, L.ctypes.data,
+                                 work.ctypes.data, nx.ctypes.data, nu.ctypes.data)
+
+
 class HostRiccati:
     """The same batch in host memory for the CPU oracle."""
 
@@ -325,12 +426,12 @@ def stage_backward_errors(rb, inp, gbuf, instances):
     out = []
     for i in instances:
         nx, nu = int(rb.nxs[i]), int(rb.nus[i])
-        babt = rb.inp.get(inp, rb.i_babt[i])
-        rsq = np.tril(rb.inp.get(inp, rb.i_rsq[i]))
+        babt = rb.inp.get(inp, rb.i_babt[i])[:nu + nx]
+        rsq = np.tril(rb.inp.get(inp, rb.i_rsq[i])[:nu + nx])
         rsq = rsq + np.tril(rsq, -1).T
-        qn = np.tril(rb.inp.get(inp, rb.i_qn[i]))
+        qn = np.tril(rb.inp.get(inp, rb.i_qn[i])[:nx])
         qn = qn + np.tril(qn, -1).T
-        L = [np.tril(rb.out.get(gbuf, rb.i_L[i][k])) for k in range(rb.N + 1)]
+        L = [np.tril(rb.out.get(gbuf, rb.i_L[i][k]))[:nu + nx if k < rb.N else nx] for k in range(rb.N + 1)]
         row = np.empty(rb.N + 1)
         row[rb.N] = np.linalg.norm(qn - L[rb.N] @ L[rb.N].T) / (np.linalg.norm(qn) * max(nx, 1) * eps)
         for k in range(rb.N):

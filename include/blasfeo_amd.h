@@ -466,6 +466,39 @@ struct blasfeo_amd_riccati_trs_arg
  * nu[k]+nx[k] and hint->max_n bounds nx[k], both <= 128. */
 int blasfeo_amd_d_back_ric_trs_batched(int ps, int batch, const struct blasfeo_amd_riccati_trs_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 
+/* One instance of d_back_ric_sv_libstr(N, nx, nu, hsBAbt, hsRSQrq, hsL, hsux,
+ * hspi, hswork_mat, hswork_vec) (examples/example_d_riccati_recursion.c:58-111):
+ * the factorisation and the solve in one recursion, on matrices augmented by
+ * one row that carries the linear terms (nk = nu[k] + nx[k]):
+ *   BAbt[k]  (nk+1) x nx[k+1]: [B_k^T; A_k^T; b_k^T];
+ *   RSQrq[k] (nk+1) x nk: the lower triangle of [R S; S^T Q] and the row [r; q]^T
+ *            (RSQrq[N]: (nx[N]+1) x nx[N], [Q_N; q_N^T]);
+ *   L[k]     (nk+1) x nk outputs (the factor and the row of the linear term);
+ *   ux[0..N] (nk) and pi[0..N-1] (nx[k+1]) outputs.
+ * work_mat and work_vec are accepted for the reference's argument list and not
+ * written: the sweep keeps those intermediates on chip. */
+struct blasfeo_amd_riccati_sv_arg
+	{
+	int N;
+	const int *nx;
+	const int *nu;
+	struct blasfeo_dmat *BAbt;
+	struct blasfeo_dmat *RSQrq;
+	struct blasfeo_dmat *L;
+	struct blasfeo_dvec *ux;
+	struct blasfeo_dvec *pi;
+	struct blasfeo_dmat *work_mat;
+	struct blasfeo_dvec *work_vec;
+	};
+
+/* Batched d_back_ric_sv_libstr: the backward sweep on the augmented matrices
+ * (the LDS-resident sweep of blasfeo_amd_d_back_ric_trf_batched, the factor's
+ * last row carrying the linear terms), then one wavefront per instance for the
+ * forward substitution.  hint->max_m bounds nu[k]+nx[k] (at most 127) and
+ * hint->max_n bounds nx[k] (at most 64); larger bounds return
+ * BLASFEO_AMD_ETOOBIG. */
+int blasfeo_amd_d_back_ric_sv_batched(int ps, int batch, const struct blasfeo_amd_riccati_sv_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Cholesky solve A X = B with A symmetric positive definite                */
 /* ------------------------------------------------------------------------ */

@@ -30,7 +30,7 @@ _SIGS = {
     "gemv_n": "iiRpiipiRpipi", "gemv_t": "iiRpiipiRpipi",
     "trmv_lnn": "ipiipipi", "trmv_ltn": "ipiipipi",
     "trsv_lnn_mn": "iipiipipi", "trsv_ltn_mn": "iipiipipi",
-    "ric_trs": "ippppppppppp",
+    "ric_trs": "ippppppppppp", "ric_sv": "ipppppppppp",
     "pack": "iipipii", "pack_tran": "iipipii", "unpack": "iipiipi",
 }
 
@@ -52,7 +52,7 @@ def lib():
                 f.argtypes = [_I] + [{"i": _I, "p": _V, "R": real}[c] for c in sig]
                 f.restype = None
         for name in ("dgemm_nt", "dgemm_nn", "dsyrk_ln", "dpotrf_l", "dgetrf_rp",
-                     "sgetrf_rp", "dric_trf", "dric_trs", "dposv",
+                     "sgetrf_rp", "dric_trf", "dric_trs", "dric_sv", "dposv",
                      "dpack", "dpack_tran", "dunpack"):
             f = getattr(L, "or_%s_batch" % name)
             f.argtypes = [_I, _I, _V]

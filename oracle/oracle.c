@@ -149,6 +149,13 @@ void or_dric_trs_batch(int ps, int batch, const struct blasfeo_amd_riccati_trs_a
 		or_dric_trs(ps, a[i].N, a[i].nx, a[i].nu, a[i].BAbt, a[i].b, a[i].rq, a[i].L, a[i].Pb, a[i].ux, a[i].pi, a[i].work);
 	}
 
+void or_dric_sv_batch(int ps, int batch, const struct blasfeo_amd_riccati_sv_arg *a)
+	{
+#pragma omp parallel for schedule(dynamic, 4)
+	for(int i = 0; i < batch; i++)
+		or_dric_sv(ps, a[i].N, a[i].nx, a[i].nu, a[i].BAbt, a[i].RSQrq, a[i].L, a[i].ux, a[i].pi, a[i].work_mat, a[i].work_vec);
+	}
+
 /* The FP64 Cholesky solve a reference caller writes:
  * blasfeo_dpotrf_l(A -> L); blasfeo_dtrsm_llnn(L, B -> X); blasfeo_dtrsm_lltn(L, X -> X). */
 void or_dposv_batch(int ps, int batch, const struct blasfeo_amd_dposv_arg *a)

@@ -773,4 +773,46 @@ void PFX(ric_trs)(int ps, int N, const int *nx, const int *nu, MAT *BAbt, VEC *b
 		}
 	}
 
+/* d_back_ric_sv_libstr (examples/example_d_riccati_recursion.c:58-111): the
+ * backward factorisation on matrices augmented by one row, which carries the
+ * linear terms through the recursion (BAbt[k]: row nk = b[k]^T; RSQrq[k]: row
+ * nk = [r; q]^T; L[k] the (nk+1) x nk trapezoid, nk = nu[k] + nx[k]), then the
+ * forward substitution for ux[0..N] and pi[0..N-1].  work: (nk+1) x nx[k+1]
+ * for every stage; wvec: nx[k+1]. */
+void PFX(ric_sv)(int ps, int N, const int *nx, const int *nu, MAT *BAbt, MAT *RSQrq, MAT *L, VEC *ux, VEC *pi, MAT *work, VEC *wvec)
+	{
+	if(N < 0)
+		return;
+	PFX(potrf_l_mn)(ps, nx[N]+1, nx[N], &RSQrq[N], 0, 0, &L[N], 0, 0);
+	for(int k = N-1; k >= 0; k--)
+		{
+		int nk = nu[k] + nx[k], nx1 = nx[k+1], nu1 = nu[k+1];
+		PFX(trmm_rlnn)(ps, nk+1, nx1, (REAL)1.0, &L[k+1], nu1, nu1, &BAbt[k], 0, 0, work, 0, 0);
+		/* blasfeo_dgead(1, nx1, 1.0, L[k+1], nu1+nx1, nu1, work, nk, 0) */
+		for(int j = 0; j < nx1; j++)
+			EL(work, nk, j) = EL(work, nk, j) + EL(&L[k+1], nu1+nx1, nu1+j);
+		PFX(syrk_potrf_ln_mn)(ps, nk+1, nk, nx1, work, 0, 0, work, 0, 0, &RSQrq[k], 0, 0, &L[k], 0, 0);
+		}
+	for(int k = 0; k < N; k++)
+		{
+		int nk = nu[k] + nx[k], nx1 = nx[k+1], nu1 = nu[k+1];
+		int ns = k == 0 ? nk : nu[k];
+		/* blasfeo_drowex(ns, -1.0, L[k], nk, 0, ux[k], 0) */
+		for(int j = 0; j < ns; j++)
+			ux[k].pa[j] = -EL(&L[k], nk, j);
+		PFX(trsv_ltn_mn)(ps, nk, ns, &L[k], 0, 0, &ux[k], 0, &ux[k], 0);
+		/* blasfeo_drowex(nx1, 1.0, BAbt[k], nk, 0, ux[k+1], nu1) */
+		for(int j = 0; j < nx1; j++)
+			ux[k+1].pa[nu1+j] = EL(&BAbt[k], nk, j);
+		PFX(gemv_t)(ps, nk, nx1, (REAL)1.0, &BAbt[k], 0, 0, &ux[k], 0, (REAL)1.0, &ux[k+1], nu1, &ux[k+1], nu1);
+		PFX(veccp)(nx1, &ux[k+1], nu1, &pi[k], 0);
+		/* blasfeo_drowex(nx1, 1.0, L[k+1], nu1+nx1, nu1, wvec, 0) */
+		for(int j = 0; j < nx1; j++)
+			wvec->pa[j] = EL(&L[k+1], nu1+nx1, nu1+j);
+		PFX(trmv_ltn)(ps, nx1, &L[k+1], nu1, nu1, &pi[k], 0, &pi[k], 0);
+		PFX(axpy)(nx1, (REAL)1.0, wvec, 0, &pi[k], 0, &pi[k], 0);
+		PFX(trmv_lnn)(ps, nx1, &L[k+1], nu1, nu1, &pi[k], 0, &pi[k], 0);
+		}
+	}
+
 #undef EL

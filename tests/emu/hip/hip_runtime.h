@@ -27,6 +27,7 @@
 
 #define __global__
 #define __device__
+#define __constant__
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
@@ -157,6 +158,14 @@ inline void emu_launch(void (*k)(KA...), dim3 grid, dim3 block, size_t shm, A...
 }
 #define hipLaunchKernelGGL(k, grid, block, shm, stream, ...) emu_launch(k, dim3(grid), dim3(block), (size_t)(shm), __VA_ARGS__)
 
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = nullptr; return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = nullptr; return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipPeekAtLastError() { return hipSuccess; }
 inline const char *hipGetErrorString(hipError_t) { return "emulated"; }

@@ -203,3 +203,17 @@ def test_trapezoid_entries_reject_large_bounds():
         h = bfa.HINT(*mx)
         assert L.blasfeo_amd_dpotrf_l_mn_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
         assert L.blasfeo_amd_dsyrk_dpotrf_ln_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
+
+
+def test_riccati_sv_bounds():
+    """The fused Riccati factorisation and solve keeps every stage resident in
+    LDS: nu+nx up to 127 (its augmented stage, 128 rows, still fits the
+    largest footprint class) and nx up to 64; larger bounds are refused
+    before any launch with BLASFEO_AMD_ETOOBIG."""
+    from blasfeo_amd import autotune as at
+    assert at.ric_class(128, 64) < 8
+    L = bfa.lib()
+    for mx in ((128, 64, 0), (100, 65, 0)):
+        h = bfa.HINT(*mx)
+        assert L.blasfeo_amd_d_back_ric_sv_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
+    assert L.blasfeo_amd_d_back_ric_sv_batched(4, 1, ctypes.c_void_p(8), None, None) == -1

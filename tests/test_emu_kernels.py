@@ -76,3 +76,10 @@ def test_gemm_family(emu_build, op, ps, wave):
     offsets, in place on every fifth instance, k = 0 and edge shapes:
     wave = 1 one wavefront per 32 x 32 tile, wave = 2 the LDS-staged engine."""
     run_case("case_gemm.py", op, ps, wave)
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_riccati_sv(emu_build, ps):
+    """The fused Riccati factorisation and solve over every LDS footprint class
+    up to nu+nx = 127, nx = 64, each matrix and vector in its own block."""
+    run_case("case_ric.py", "sv", ps)

@@ -250,3 +250,53 @@ def test_riccati_trs_zero_inputs_give_zero():
     argv = rs.args(tab.ctypes.data, babt.ctypes.data, L.ctypes.data, nx.ctypes.data, nu.ctypes.data)
     oracle.lib().or_dric_trs_batch(4, rb.batch, argv.ctypes.data)
     assert not np.any(sbuf.view(np.float64))
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_riccati_sv_matches_trf_trs(ps):
+    """The oracle's fused sweep (ric_sv, d_back_ric_sv_libstr) on matrices
+    augmented by the linear-term row, against its factorisation + solve
+    (ric_trf then ric_trs, d_back_ric_trf_libstr + d_back_ric_trs_libstr)
+    on the same problem: the square part of every factor is the same bits
+    (the extra row changes no other row's arithmetic), ux and pi agree to
+    rounding, and the fused solution satisfies the KKT equations to the
+    same backward error (the trs path is pinned by the dense KKT solve above)."""
+    from blasfeo_amd.riccati import RiccatiBatch, RiccatiSv, HostRiccati, trs_backward_error
+    rng = np.random.default_rng(880 + ps)
+    rb = RiccatiBatch([3, 6, 9, 1, 17], [2, 1, 4, 3, 5], N=5, ps=ps, aug=True)
+    inp = rb.fill_inputs(rng)
+    sv = RiccatiSv(rb)
+    rs = sv.rs
+    L = oracle.lib()
+    # factorisation + solve
+    h1 = HostRiccati(rb, inp)
+    L.or_dric_trf_batch(ps, rb.batch, h1.argv.ctypes.data)
+    sin = sv.solve_inputs(inp)
+    sol1 = sin.copy()
+    babt, rsq, Lt, work, nx, nu = h1.t
+    tab1 = rs.table(sol1.ctypes.data)
+    targs = rs.args(tab1.ctypes.data, babt.ctypes.data, Lt.ctypes.data, nx.ctypes.data, nu.ctypes.data)
+    L.or_dric_trs_batch(ps, rb.batch, targs.ctypes.data)
+    # fused
+    h2 = HostRiccati(rb, inp)
+    sol2 = np.zeros(rs.nbytes, dtype=np.uint8)
+    tab2 = rs.table(sol2.ctypes.data)
+    babt2, rsq2, L2, work2, nx2, nu2 = h2.t
+    argv = sv.args(tab2.ctypes.data, babt2.ctypes.data, rsq2.ctypes.data, L2.ctypes.data, work2.ctypes.data,
+                   nx2.ctypes.data, nu2.ctypes.data)
+    L.or_dric_sv_batch(ps, rb.batch, argv.ctypes.data)
+    for i in range(rb.batch):
+        nk = int(rb.nxs[i] + rb.nus[i])
+        for k in range(rb.N + 1):
+            n = nk if k < rb.N else int(rb.nxs[i])
+            a = np.tril(h1.L(i, k)[:n, :n])
+            b = np.tril(h2.L(i, k)[:n, :n])
+            np.testing.assert_array_equal(a, b, err_msg="instance %d stage %d" % (i, k))
+        for kind, ks in (("ux", range(rb.N + 1)), ("pi", range(rb.N))):
+            for k in ks:
+                u1, u2 = rs.vec(sol1, i, kind, k), rs.vec(sol2, i, kind, k)
+                np.testing.assert_allclose(u2, u1, rtol=1e-9, atol=1e-9 * np.abs(u1).max(),
+                                           err_msg="instance %d %s[%d]" % (i, kind, k))
+        e1 = trs_backward_error(rb, inp, rs, sin, sol1, i)
+        e2 = trs_backward_error(rb, inp, rs, sin, sol2, i)
+        assert e2 <= 10 * e1 + 100, (i, e1, e2)

@@ -79,6 +79,13 @@ def test_gemm_family(emu_build, op, ps, wave):
 
 
 @pytest.mark.parametrize("ps", [4, 8])
+def test_riccati_trf(emu_build, ps):
+    """The LDS-resident Riccati factorisation sweep over every footprint class,
+    with the built-in launch shapes and with one wave per instance."""
+    run_case("case_ric.py", "trf", ps)
+
+
+@pytest.mark.parametrize("ps", [4, 8])
 def test_riccati_sv(emu_build, ps):
     """The fused Riccati factorisation and solve over every LDS footprint class
     up to nu+nx = 127, nx = 64, each matrix and vector in its own block."""

@@ -171,10 +171,11 @@ namespace bfa {
 // Rows m .. ceil16(m)-1 and columns n .. ceil16(n)-1 must hold zeros on entry
 // (with PADCOLS the padding columns hold unspecified values on exit).
 // Per 16-column block: wave 0 factors the whole panel (diagonal block and the
-// rows below, two rows per lane, one when NMAX <= 64) in registers, broadcasting L(jj,j) with
-// v_readlane, so the panel needs no workgroup barrier; then the waves apply
-// the rank-16 trailing update to the lower tiles on the matrix cores, one
-// block column of tiles at a time with independent accumulators.
+// rows below, two rows per lane, one when NMAX <= 64) in registers,
+// broadcasting L(jj,j) with v_readlane, so the panel needs no workgroup
+// barrier; then the waves apply the rank-16 trailing update to the lower tiles
+// on the matrix cores, one block column of tiles at a time with independent
+// accumulators.
 // While wave 0 factors the first panel, the other waves call side(): work of
 // the caller's that touches nothing the factorisation reads or writes (a
 // no-op by default; with one wave, or no panel, every wave calls it once).
@@ -185,9 +186,48 @@ struct NoSide
 	__device__ __forceinline__ void operator()() const {}
 };
 
+// Rank-16 update of NTL tiles (g..g+NTL-1, bj) of the trailing matrix by the
+// panel at column c0: all operands (accumulators, the panel's A values of each
+// tile, its B values of block column bj) are loaded before the first
+// matrix-core step, so a wave waits once per task rather than once per step.
+template <typename T, int NTL, typename ACC>
+__device__ __forceinline__ void chol_update_tiles(const ACC M, int g, int bj, int c0)
+{
+	typedef typename Vec4<T>::type V4;
+	const int lane = lane_id();
+	const int lq = lane >> 4, lr = lane & 15;
+	V4 acc[NTL];
+	T a[NTL][4], b[4];
+#pragma unroll
+	for (int t = 0; t < NTL; t++)
+#pragma unroll
+		for (int r = 0; r < 4; r++)
+			acc[t][r] = M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + lr);
+#pragma unroll
+	for (int s = 0; s < 4; s++)
+		b[s] = M(bj * 16 + lr, c0 + 4 * s + lq);
+#pragma unroll
+	for (int t = 0; t < NTL; t++)
+#pragma unroll
+		for (int s = 0; s < 4; s++)
+			a[t][s] = M((g + t) * 16 + lr, c0 + 4 * s + lq);
+#pragma unroll
+	for (int s = 0; s < 4; s++)
+#pragma unroll
+		for (int t = 0; t < NTL; t++)
+			acc[t] = mfma16(-a[t][s], b[s], acc[t]);
+#pragma unroll
+	for (int t = 0; t < NTL; t++)
+#pragma unroll
+		for (int r = 0; r < 4; r++)
+			M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + lr) = acc[t][r];
+}
+
 template <typename T, int NT, int NMAX, typename ACC, bool PADCOLS = true, typename SIDE = NoSide>
 __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, const SIDE &side = SIDE{})
 {
+	m = __builtin_amdgcn_readfirstlane(m);
+	n = __builtin_amdgcn_readfirstlane(n);
 	typedef typename Vec4<T>::type V4;
 	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
@@ -332,38 +372,15 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, con
 					bj++;
 				}
 				const int g = bj + 4 * rem;
-				V4 acc[4];
-#pragma unroll
-				for (int t = 0; t < 4; t++)
-				{
-					if (g + t < mt)
-#pragma unroll
-						for (int r = 0; r < 4; r++)
-							acc[t][r] = M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + (lane & 15));
-				}
-#pragma unroll
-				for (int kk = 0; kk < 16; kk += 4)
-				{
-					const int kq = c0 + kk + (lane >> 4);
-					const T b = M(bj * 16 + (lane & 15), kq);
-#pragma unroll
-					for (int t = 0; t < 4; t++)
-					{
-						if (g + t < mt)
-						{
-							const T a = M((g + t) * 16 + (lane & 15), kq);
-							acc[t] = mfma16(-a, b, acc[t]);
-						}
-					}
-				}
-#pragma unroll
-				for (int t = 0; t < 4; t++)
-				{
-					if (g + t < mt)
-#pragma unroll
-						for (int r = 0; r < 4; r++)
-							M((g + t) * 16 + acc_row<T>(lane, r), bj * 16 + (lane & 15)) = acc[t][r];
-				}
+				const int ntl = min(4, mt - g);
+				if (ntl == 4)
+					chol_update_tiles<T, 4>(M, g, bj, c0);
+				else if (ntl == 3)
+					chol_update_tiles<T, 3>(M, g, bj, c0);
+				else if (ntl == 2)
+					chol_update_tiles<T, 2>(M, g, bj, c0);
+				else
+					chol_update_tiles<T, 1>(M, g, bj, c0);
 			}
 		}
 		__syncthreads();

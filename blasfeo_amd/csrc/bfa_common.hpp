@@ -87,18 +87,18 @@ __device__ __forceinline__ PM<T, PS> view(const typename MatOf<T>::type *s)
 // matrix that this thread copies: the waves take the covering panels in turn
 // and the lanes of a wave walk one panel's contiguous run of C*PS elements, so
 // a wave reads contiguous memory and no index needs a runtime division.  The
-// elements go eight at a time: a group's global loads are all issued before
-// any of its stores, so a thread keeps up to eight loads in flight instead of
-// waiting for each.  Elements with i >= Rv or c >= Cv (and, with LOWER, those
-// above the diagonal, i < c) are passed on as zero and never read; with
-// SKIPTILES, elements whose 16x16 tile lies above the diagonal are skipped.
+// elements go B at a time: a group's global loads are all issued before any
+// of its stores, so a thread keeps up to B loads in flight instead of waiting
+// for each (the second pass recomputes the indices, so a group costs B values'
+// registers).  Elements with i >= Rv or c >= Cv (and, with LOWER, those above
+// the diagonal, i < c) are passed on as zero and never read; with SKIPTILES,
+// elements whose 16x16 tile lies above the diagonal are skipped.
 // put(i, c, v) stores element (i, c) of the block.
-template <typename T, int PS, int NT, bool LOWER, bool SKIPTILES, typename PUT, int W0 = 0>
+template <typename T, int PS, int NT, bool LOWER, bool SKIPTILES, typename PUT, int W0 = 0, int B = 8>
 __device__ __forceinline__ void g2s_walk(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, PUT put)
 {
 	// waves W0 .. NT/WAVE - 1 take part (all of them by default)
 	constexpr int NW = NT / WAVE - W0;
-	constexpr int B = 8;
 	const int rr = r0 & (PS - 1);
 	const int np = (rr + R + PS - 1) / PS;
 	const int run = PS * C;
@@ -109,18 +109,15 @@ __device__ __forceinline__ void g2s_walk(const PM<T, PS> g, int r0, int c0, int 
 		return;
 	while (p < np)
 	{
+		const int p0 = p, e0 = e;
 		T v[B];
-		int ii[B], cc[B];
-		bool ok[B];
 #pragma unroll
 		for (int b = 0; b < B; b++)
 		{
 			const int c = (unsigned)e / PS;
 			const int i = p * PS + (e & (PS - 1)) - rr;
-			ok[b] = p < np && e < run && i >= 0 && i < R && (!SKIPTILES || (i >> 4) >= (c >> 4));
-			ii[b] = i;
-			cc[b] = c;
-			const bool ld = ok[b] && i < Rv && c < Cv && (!LOWER || i >= c);
+			const bool ok = p < np && e < run && i >= 0 && i < R && (!SKIPTILES || (i >> 4) >= (c >> 4));
+			const bool ld = ok && i < Rv && c < Cv && (!LOWER || i >= c);
 			v[b] = ld ? base[(size_t)p * PS * g.cn + e] : T(0);
 			e += WAVE;
 			if (e >= run)
@@ -129,10 +126,22 @@ __device__ __forceinline__ void g2s_walk(const PM<T, PS> g, int r0, int c0, int 
 				p += NW;
 			}
 		}
+		p = p0;
+		e = e0;
 #pragma unroll
 		for (int b = 0; b < B; b++)
-			if (ok[b])
-				put(ii[b], cc[b], v[b]);
+		{
+			const int c = (unsigned)e / PS;
+			const int i = p * PS + (e & (PS - 1)) - rr;
+			if (p < np && e < run && i >= 0 && i < R && (!SKIPTILES || (i >> 4) >= (c >> 4)))
+				put(i, c, v[b]);
+			e += WAVE;
+			if (e >= run)
+			{
+				e = lane;
+				p += NW;
+			}
+		}
 	}
 }
 
@@ -190,11 +199,11 @@ __device__ __forceinline__ void s2g(PM<T, PS> g, int r0, int c0, int Rv, int Cv,
 // A(i, c) (element (i, c) of the block).  With LOWER, g2s_acc skips elements
 // whose 16x16 tile lies above the diagonal (packed lower-tile storage has no
 // such tiles) and zero-fills the upper part of diagonal tiles.
-template <typename T, int PS, int NT, bool LOWER, typename ACC, int W0 = 0>
+template <typename T, int PS, int NT, bool LOWER, typename ACC, int W0 = 0, int B = 8>
 __device__ __forceinline__ void g2s_acc(const PM<T, PS> g, int r0, int c0, int R, int C, int Rv, int Cv, const ACC A)
 {
 	auto put = [=](int i, int c, T v) { A(i, c) = v; };
-	g2s_walk<T, PS, NT, LOWER, LOWER, decltype(put), W0>(g, r0, c0, R, C, Rv, Cv, put);
+	g2s_walk<T, PS, NT, LOWER, LOWER, decltype(put), W0, B>(g, r0, c0, R, C, Rv, Cv, put);
 }
 
 template <typename T, int PS, int NT, bool LOWER, typename ACC>

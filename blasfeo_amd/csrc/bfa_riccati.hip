@@ -67,8 +67,9 @@ struct TileRect
 };
 
 // The first RIC_QSLOT bytes of a class kernel's dynamic LDS hold the work-queue
-// slot; the class kernels declare no static LDS, so a launch asks for exactly
-// its class cap (at most the 160 KB of a CU).
+// slot (bytes 0..3) and a double 0.0 (bytes 8..15, just before the sweep's M);
+// the class kernels declare no static LDS, so a launch asks for exactly its
+// class cap (at most the 160 KB of a CU).
 constexpr int RIC_QSLOT = 16;
 
 // LDS bytes of the resident form for an instance whose largest stage is
@@ -102,9 +103,23 @@ __device__ __forceinline__ double *lds_opaque(double *p)
 }
 
 // ---------------------------------------------------------------------------
+// A value every lane holds, moved to a scalar register: loops and branches
+// that it controls are then uniform (scalar branches, no exec masking), so
+// the operand loads of a block of matrix-core steps can all be issued before
+// the first of them waits.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // W <- W * Lxx (trmm_rlnn), Lxx = L[k+1](u1:u1+n1, u1:u1+n1), held in M.
 // 16-row strips per wave, the strip's 16 x 64 result in four accumulators;
-// only this wave reads its strip, so it is overwritten in place.
+// only this wave reads its strip, so it is overwritten in place.  Per 16-wide
+// block C of the inner dimension the block's operands are loaded first (the
+// strip's four A columns, and Lxx's B values for the column tiles ct <= C, the
+// others being zero above the diagonal), then its matrix-core steps run.  The
+// B loads are unconditional (a conditional load would be waited for on its
+// own): row u1 + min(lk, n1 - 1), column u1 + min(j, n1 - 1), inside M's
+// packed tiles, and for an element above Lxx's diagonal the address of the
+// zero kept just before M (ric_zero_slot).  Rows lk >= n1 need no zeroing:
+// W's columns from n1 on (the A operand) are zero.
 template <int NT>
 __device__ __forceinline__ void ric_trmm_w(const TileLower M, const TileRect W, int nkt, int n1, int n1t, int u1)
 {
@@ -112,25 +127,48 @@ __device__ __forceinline__ void ric_trmm_w(const TileLower M, const TileRect W, 
 	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
 	const int w = wave_id();
+	const int lq = lane >> 4, lr = lane & 15;
+	int hcol[4], jv[4];
+#pragma unroll
+	for (int ct = 0; ct < 4; ct++)
+	{
+		jv[ct] = ct * 16 + lr;
+		const int cc = u1 + min(jv[ct], n1 - 1);
+		hcol[ct] = (cc >> 4) * RT_SZ + (cc & 15) * RT_LD;
+	}
 	for (int rt = w; rt < nkt; rt += NW)
 	{
 		V4 acc[4];
 #pragma unroll
 		for (int ct = 0; ct < 4; ct++)
 			acc[ct] = V4{0, 0, 0, 0};
-		for (int l0 = 0; l0 < n1; l0 += 4)
-		{
-			const int lk = l0 + (lane >> 4);
-			const double av = W(rt * 16 + (lane & 15), lk);
+		const double *pa = W.p + (size_t)rt * W.nct * RT_SZ + lq * RT_LD + lr;
+		constexpr int ZOFF = -1;   // M.p[-1] holds 0.0
 #pragma unroll
-			for (int ct = 0; ct < 4; ct++)
+		for (int C = 0; C < 4; C++)
+		{
+			if (C < n1t)
 			{
-				if (ct < n1t && l0 + 3 >= ct * 16)
+				double a[4], b[4][4];
+#pragma unroll
+				for (int s = 0; s < 4; s++)
+					a[s] = pa[C * RT_SZ + s * 4 * RT_LD];
+#pragma unroll
+				for (int s = 0; s < 4; s++)
 				{
-					const int j = ct * 16 + (lane & 15);
-					const double bv = (lk < n1 && lk >= j) ? M(u1 + lk, u1 + j) : 0.0;
-					acc[ct] = mfma16(av, bv, acc[ct]);
+					const int lk = C * 16 + s * 4 + lq;
+					const int rr = u1 + min(lk, n1 - 1);
+					const int R = rr >> 4;
+					const int roff = (R * (R + 1) / 2) * RT_SZ + (rr & 15);
+#pragma unroll
+					for (int ct = 0; ct <= C; ct++)
+						b[ct][s] = M.p[lk >= jv[ct] ? roff + hcol[ct] : ZOFF];
 				}
+#pragma unroll
+				for (int s = 0; s < 4; s++)
+#pragma unroll
+					for (int ct = 0; ct <= C; ct++)
+						acc[ct] = mfma16(a[s], b[ct][s], acc[ct]);
 			}
 		}
 		wave_sync();
@@ -139,19 +177,52 @@ __device__ __forceinline__ void ric_trmm_w(const TileLower M, const TileRect W, 
 			if (ct < n1t)
 #pragma unroll
 				for (int r = 0; r < 4; r++)
-					W(rt * 16 + acc_row<double>(lane, r), ct * 16 + (lane & 15)) = acc[ct][r];
+					W(rt * 16 + acc_row<double>(lane, r), ct * 16 + lr) = acc[ct][r];
 	}
 }
 
-// M += W W^T on the lower tiles (syrk_ln): (block column bj, group of up to
-// 4 tiles bi = g..g+3) per wave, the B operand shared by the group.  FROMG:
-// M = lower(gR) + W W^T instead, the accumulators started from the nk x nk
-// lower triangle of gR in global memory (zero elsewhere), so that M needs no
-// separate staging pass.
+// acc[t] += W(strip I[t]) W(strip J[t])^T over the inner tiles 0..kt-1 of
+// W, for NTL tiles; per inner tile the 8*NTL operands are loaded before its
+// matrix-core steps.
+template <int NTL>
+__device__ __forceinline__ void ric_syrk_tiles(const TileRect W, const int (&I)[4], const int (&J)[4], int kt,
+	d4_t (&acc)[4])
+{
+	const int lane = lane_id();
+	const int lo = (lane >> 4) * RT_LD + (lane & 15);
+	const int rs = W.nct * RT_SZ;   // one strip
+	for (int C = 0; C < kt; C++)
+	{
+		double a[NTL][4], b[NTL][4];
+#pragma unroll
+		for (int t = 0; t < NTL; t++)
+#pragma unroll
+			for (int s = 0; s < 4; s++)
+			{
+				a[t][s] = W.p[I[t] * rs + C * RT_SZ + s * 4 * RT_LD + lo];
+				b[t][s] = W.p[J[t] * rs + C * RT_SZ + s * 4 * RT_LD + lo];
+			}
+#pragma unroll
+		for (int s = 0; s < 4; s++)
+#pragma unroll
+			for (int t = 0; t < NTL; t++)
+				acc[t] = mfma16(a[t][s], b[t][s], acc[t]);
+	}
+}
+
+// M += W W^T on the lower tiles (syrk_ln).  The lower tiles (I, J), I >= J,
+// numbered down the block columns, are dealt round-robin to the waves, which
+// take theirs four at a time, so the waves' shares differ by one tile at
+// most.  FROMG: M = lower(gR) + W W^T instead: the nk x nk lower triangle of
+// gR (global memory, zero elsewhere) is loaded into registers of its own
+// before the matrix-core steps and added after them, so that its latency
+// overlaps them and M needs no separate staging pass.
 // AUG: the (nk+1) x nk trapezoid of the fused sweep (one more row of W and
 // M, the linear-term row): row tiles of nk+1 rows, column tiles of nk
 // columns, and the columns from nk on written as zero (the Cholesky needs its
 // padding columns zero; W's row nk is not padding).
+// W's columns from n1 on are zero (the trmm writes them so), so the inner
+// dimension runs over whole 16-column tiles.
 template <int NT, int PS = 4, bool FROMG = false, bool AUG = false>
 __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, int nkt, int n1,
 	const PM<double, PS> gR = PM<double, PS>{}, int nk = 0)
@@ -159,60 +230,74 @@ __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, 
 	typedef d4_t V4;
 	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
-	const int w = wave_id();
+	const int w = uni(wave_id());
 	const int mrt = AUG ? (nk + 1 + 15) / 16 : nkt;   // row tiles
 	const int nrows = AUG ? nk + 1 : nk;
-	int ntask = 0;
+	const int kt = (n1 + 15) / 16;
+	int ntile = 0;
 	for (int bj = 0; bj < nkt; bj++)
-		ntask += (mrt - bj + 3) / 4;
-	for (int task = w; task < ntask; task += NW)
+		ntile += mrt - bj;
+	const int mine = (ntile - w + NW - 1) / NW;   // tiles w, w + NW, ...
+	for (int c0 = 0; c0 < mine; c0 += 4)
 	{
-		int bj = 0, rem = task;
-		while (rem >= (mrt - bj + 3) / 4)
-		{
-			rem -= (mrt - bj + 3) / 4;
-			bj++;
-		}
-		const int g = bj + 4 * rem;
-		V4 acc[4];
+		const int cnt = min(4, mine - c0);
+		int I[4], J[4];
 #pragma unroll
 		for (int t = 0; t < 4; t++)
 		{
-			if (g + t < mrt)
+			int q = w + NW * (c0 + min(t, cnt - 1)), bj = 0;
+			while (q >= mrt - bj)
+			{
+				q -= mrt - bj;
+				bj++;
+			}
+			I[t] = bj + q;
+			J[t] = bj;
+		}
+		V4 acc[4], rq[4];
+#pragma unroll
+		for (int t = 0; t < 4; t++)
+		{
+			acc[t] = V4{0, 0, 0, 0};
+			rq[t] = V4{0, 0, 0, 0};
+			if (t < cnt)
+			{
+				const int col = J[t] * 16 + (lane & 15);
 #pragma unroll
 				for (int r = 0; r < 4; r++)
 				{
-					const int row = (g + t) * 16 + acc_row<double>(lane, r), col = bj * 16 + (lane & 15);
+					const int row = I[t] * 16 + acc_row<double>(lane, r);
 					if (FROMG)
-						acc[t][r] = (row < nrows && col < nk && row >= col) ? gR.at(row, col) : 0.0;
+					{
+						if (row < nrows && col < nk && row >= col)
+							rq[t][r] = gR.at(row, col);
+					}
 					else
 						acc[t][r] = M(row, col);
 				}
-		}
-		for (int l0 = 0; l0 < n1; l0 += 4)
-		{
-			const int lk = l0 + (lane >> 4);
-			const double bv = W(bj * 16 + (lane & 15), lk);
-#pragma unroll
-			for (int t = 0; t < 4; t++)
-			{
-				if (g + t < mrt)
-				{
-					const double av = W((g + t) * 16 + (lane & 15), lk);
-					acc[t] = mfma16(av, bv, acc[t]);
-				}
 			}
 		}
+		if (cnt == 4)
+			ric_syrk_tiles<4>(W, I, J, kt, acc);
+		else if (cnt == 3)
+			ric_syrk_tiles<3>(W, I, J, kt, acc);
+		else if (cnt == 2)
+			ric_syrk_tiles<2>(W, I, J, kt, acc);
+		else
+			ric_syrk_tiles<1>(W, I, J, kt, acc);
 #pragma unroll
 		for (int t = 0; t < 4; t++)
 		{
-			if (g + t < mrt)
+			if (t < cnt)
+			{
+				const int col = J[t] * 16 + (lane & 15);
 #pragma unroll
 				for (int r = 0; r < 4; r++)
 				{
-					const int col = bj * 16 + (lane & 15);
-					M((g + t) * 16 + acc_row<double>(lane, r), col) = (AUG && col >= nk) ? 0.0 : acc[t][r];
+					const double v = FROMG ? acc[t][r] + rq[t][r] : acc[t][r];
+					M(I[t] * 16 + acc_row<double>(lane, r), col) = (AUG && col >= nk) ? 0.0 : v;
 				}
+			}
 		}
 	}
 }
@@ -252,7 +337,7 @@ __device__ void ric_sweep_lds(const ArgT &a, double *smem)
 {
 	constexpr int NMAXC = (NT == WAVE) ? 64 : 128;
 	constexpr int XR = AUG ? 1 : 0;   // extra row of the augmented form
-	const int N = a.N;
+	const int N = uni(a.N);
 	const int *nx = a.nx;
 	const int *nu = a.nu;
 	blasfeo_dmat *L = a.L;
@@ -264,6 +349,8 @@ __device__ void ric_sweep_lds(const ArgT &a, double *smem)
 		nkmax = max(nkmax, nu[k] + nx[k]);
 		nxmax = max(nxmax, nx[k]);
 	}
+	nkmax = uni(nkmax);
+	nxmax = uni(nxmax);
 	const int nkt_max = (nkmax + XR + 15) / 16;
 	const int nxt_max = (nxmax + 15) / 16;
 	const int nlow = nkt_max * (nkt_max + 1) / 2;
@@ -280,13 +367,13 @@ __device__ void ric_sweep_lds(const ArgT &a, double *smem)
 	{
 		const bool last = s < 0;
 		const int k = last ? N : N - 1 - s;
-		const int nk = last ? nx[N] : nu[k] + nx[k];
+		const int nk = uni(last ? nx[N] : nu[k] + nx[k]);
 		const int nkt = (nk + 15) / 16;         // column tiles
 		const int mrt = (nk + XR + 15) / 16;    // row tiles
 		if (!last)
 		{
-			const int n1 = nx[k + 1];
-			const int u1 = nu[k + 1];
+			const int n1 = uni(nx[k + 1]);
+			const int u1 = uni(nu[k + 1]);
 			const int n1t = (n1 + 15) / 16;
 			// stage BAbt[k] (nk(+1) x n1) into W; M still holds L[k+1].  The
 			// multi-wave kernels loaded it during the previous stage's
@@ -322,10 +409,10 @@ __device__ void ric_sweep_lds(const ArgT &a, double *smem)
 			if constexpr (NT == WAVE)
 			{
 				__syncthreads();
-				ric_syrk_call<NT>(M, W, nkt, nx[k + 1]);
+				ric_syrk_call<NT>(M, W, nkt, uni(nx[k + 1]));
 			}
 			else
-				ric_syrk_m<NT, PS, true, AUG>(M, W, nkt, nx[k + 1], view<double, PS>(&RSQ[k]), nk);
+				ric_syrk_m<NT, PS, true, AUG>(M, W, nkt, uni(nx[k + 1]), view<double, PS>(&RSQ[k]), nk);
 		}
 		// L[k] = chol(M); W is free (the syrk has read it), so the waves
 		// other than 0 load the next stage's BAbt into it meanwhile
@@ -337,7 +424,7 @@ __device__ void ric_sweep_lds(const ArgT &a, double *smem)
 			auto side = [&]() {
 				if (kn >= 0)
 				{
-					const int nkn = nu[kn] + nx[kn], n1n = nx[kn + 1];
+					const int nkn = uni(nu[kn] + nx[kn]), n1n = uni(nx[kn + 1]);
 					g2s_acc<double, PS, NT, false, TileRect, 1>(view<double, PS>(&BAbt[kn]), 0, 0,
 						(nkn + XR + 15) / 16 * 16, (n1n + 15) / 16 * 16, nkn + XR, n1n, W);
 				}
@@ -504,6 +591,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
 	extern __shared__ double smem_d[];
 	int *slot = reinterpret_cast<int *>(smem_d);
 	double *work = smem_d + RIC_QSLOT / sizeof(double);
+	if (threadIdx.x == 0)
+		work[-1] = 0.0;   // the trmm's zero operand (ric_trmm_w)
 	const int cnt = *count;
 	for (int t = ric_next<NT>(head, slot); t < cnt; t = ric_next<NT>(head, slot))
 		ric_sweep_lds<PS, NT, AUG>(args[list[t]], work);
@@ -520,6 +609,8 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3))) void 
 {
 	extern __shared__ double smem_d[];
 	double *work = smem_d + RIC_QSLOT / sizeof(double);
+	if (threadIdx.x == 0)
+		work[-1] = 0.0;   // the trmm's zero operand (ric_trmm_w)
 	const int cnt = *count;
 	for (int t = ric_next<WAVE>(head, nullptr); t < cnt; t = ric_next<WAVE>(head, nullptr))
 		ric_sweep_lds<PS, WAVE>(args[list[t]], work);

@@ -95,6 +95,8 @@ inline int emu_dpp_i(int old, int v, int ctrl, int rm, int bm, bool bc)
 #define __builtin_amdgcn_fence(...) ((void)0)
 #define __builtin_amdgcn_rsq(c) (1.0 / sqrt((double)(c)))
 #define __builtin_amdgcn_rsqf(c) (1.0f / sqrtf((float)(c)))
+#define __builtin_amdgcn_rcp(c) (1.0 / (double)(c))
+#define __builtin_amdgcn_rcpf(c) (1.0f / (float)(c))
 
 typedef double emu_d4 __attribute__((ext_vector_type(4)));
 typedef float emu_f4 __attribute__((ext_vector_type(4)));

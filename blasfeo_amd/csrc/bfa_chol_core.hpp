@@ -173,9 +173,9 @@ namespace bfa {
 // Per 16-column block: wave 0 factors the whole panel (diagonal block and the
 // rows below, two rows per lane, one when NMAX <= 64) in registers,
 // broadcasting L(jj,j) with v_readlane, so the panel needs no workgroup
-// barrier; then the waves apply the rank-16 trailing update to the lower tiles
-// on the matrix cores, one block column of tiles at a time with independent
-// accumulators.
+// barrier; the rank-16 trailing updates run on the matrix cores, a group of up
+// to four tiles of one block column per task with independent accumulators
+// (look-ahead order below).
 // While wave 0 factors the first panel, the other waves call side(): work of
 // the caller's that touches nothing the factorisation reads or writes (a
 // no-op by default; with one wave, or no panel, every wave calls it once).
@@ -239,6 +239,40 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, con
 	// is called after the factorisation
 	if (nt == 0)
 		side();
+	// tiles (g..g+3, bj) (fewer at the bottom) updated by the panel at column cp
+	auto upd = [&](int g, int bj, int cp) {
+		const int ntl = min(4, mt - g);
+		if (ntl == 4)
+			chol_update_tiles<T, 4>(M, g, bj, cp);
+		else if (ntl == 3)
+			chol_update_tiles<T, 3>(M, g, bj, cp);
+		else if (ntl == 2)
+			chol_update_tiles<T, 2>(M, g, bj, cp);
+		else
+			chol_update_tiles<T, 1>(M, g, bj, cp);
+	};
+	// the block columns bj0..nt-1 updated by the panel at column cp: tasks
+	// (block column, group of up to 4 tiles) dealt to waves w0, w0+ws, ...
+	auto upd_rest = [&](int bj0, int cp, int w0, int ws) {
+		int ntask = 0;
+		for (int bj = bj0; bj < nt; bj++)
+			ntask += (mt - bj + 3) / 4;
+		for (int task = w0; task < ntask; task += ws)
+		{
+			int bj = bj0, rem = task;
+			while (rem >= (mt - bj + 3) / 4)
+			{
+				rem -= (mt - bj + 3) / 4;
+				bj++;
+			}
+			upd(bj + 4 * rem, bj, cp);
+		}
+	};
+	// Look-ahead: in step jb, wave 0 applies the previous panel's update to
+	// block column jb alone and factors panel jb, while the other waves apply
+	// the previous panel's update to the block columns after jb (side() in
+	// step 0), so the matrix-core updates run beside the panel's serial chain
+	// and a step needs one barrier.
 	for (int jb = 0; jb < nt; jb++)
 	{
 		const int c0 = jb * 16;
@@ -246,6 +280,15 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, con
 		const int nr = m - c0;
 		if (w == 0)
 		{
+			if (jb > 0)
+			{
+				for (int g = jb; g < mt; g += 4)
+					upd(g, jb, c0 - 16);
+				if (NW == 1)
+					upd_rest(jb + 1, c0 - 16, 0, 1);
+				// the panel below reads what other lanes of this wave stored
+				wave_sync();
+			}
 			T x0[16], x1[16];
 			const bool r0ok = lane < nr, r1ok = NMAX > 64 && lane + 64 < nr;
 #pragma unroll
@@ -356,33 +399,8 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, con
 		}
 		else if (jb == 0)
 			side();
-		__syncthreads();
-		// trailing update: tasks (block column bj > jb, group of up to 4 tiles
-		// bi = g..g+3 >= bj) spread over the waves, independent accumulators
-		{
-			int ntask = 0;
-			for (int bj = jb + 1; bj < nt; bj++)
-				ntask += (mt - bj + 3) / 4;
-			for (int task = w; task < ntask; task += NW)
-			{
-				int bj = jb + 1, rem = task;
-				while (rem >= (mt - bj + 3) / 4)
-				{
-					rem -= (mt - bj + 3) / 4;
-					bj++;
-				}
-				const int g = bj + 4 * rem;
-				const int ntl = min(4, mt - g);
-				if (ntl == 4)
-					chol_update_tiles<T, 4>(M, g, bj, c0);
-				else if (ntl == 3)
-					chol_update_tiles<T, 3>(M, g, bj, c0);
-				else if (ntl == 2)
-					chol_update_tiles<T, 2>(M, g, bj, c0);
-				else
-					chol_update_tiles<T, 1>(M, g, bj, c0);
-			}
-		}
+		else if constexpr (NW > 1)
+			upd_rest(jb + 1, c0 - 16, w - 1, NW - 1);
 		__syncthreads();
 	}
 	if (NW == 1 && nt > 0)

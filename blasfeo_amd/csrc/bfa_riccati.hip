@@ -210,6 +210,66 @@ __device__ __forceinline__ void ric_syrk_tiles(const TileRect W, const int (&I)[
 	}
 }
 
+// One wave's syrk tiles taken four at a time: chunk c0 of the wave's list
+// (tiles w + NW*(c0..c0+3)), and with FROMG the chunk's RSQ values.
+struct RicChunk
+{
+	int I[4], J[4], cnt;
+	d4_t rq[4];
+};
+
+template <int NT, int PS, bool FROMG, bool AUG>
+__device__ __forceinline__ int ric_syrk_mine(int nkt, int nk)
+{
+	constexpr int NW = NT / WAVE;
+	const int w = uni(wave_id());
+	const int mrt = AUG ? (nk + 1 + 15) / 16 : nkt;   // row tiles
+	int ntile = 0;
+	for (int bj = 0; bj < nkt; bj++)
+		ntile += mrt - bj;
+	return (ntile - w + NW - 1) / NW;   // tiles w, w + NW, ...
+}
+
+template <int NT, int PS, bool FROMG, bool AUG>
+__device__ __forceinline__ void ric_syrk_chunk(RicChunk &ch, int c0, int mine, int nkt, int nk,
+	const PM<double, PS> gR)
+{
+	constexpr int NW = NT / WAVE;
+	const int lane = lane_id();
+	const int w = uni(wave_id());
+	const int mrt = AUG ? (nk + 1 + 15) / 16 : nkt;
+	const int nrows = AUG ? nk + 1 : nk;
+	ch.cnt = min(4, mine - c0);
+#pragma unroll
+	for (int t = 0; t < 4; t++)
+	{
+		int q = w + NW * (c0 + min(t, ch.cnt - 1)), bj = 0;
+		while (q >= mrt - bj)
+		{
+			q -= mrt - bj;
+			bj++;
+		}
+		ch.I[t] = bj + q;
+		ch.J[t] = bj;
+	}
+#pragma unroll
+	for (int t = 0; t < 4; t++)
+	{
+		ch.rq[t] = d4_t{0, 0, 0, 0};
+		if (FROMG && t < ch.cnt)
+		{
+			const int col = ch.J[t] * 16 + (lane & 15);
+#pragma unroll
+			for (int r = 0; r < 4; r++)
+			{
+				const int row = ch.I[t] * 16 + acc_row<double>(lane, r);
+				if (row < nrows && col < nk && row >= col)
+					ch.rq[t][r] = gR.at(row, col);
+			}
+		}
+	}
+}
+
 // M += W W^T on the lower tiles (syrk_ln).  The lower tiles (I, J), I >= J,
 // numbered down the block columns, are dealt round-robin to the waves, which
 // take theirs four at a time, so the waves' shares differ by one tile at
@@ -228,74 +288,43 @@ __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, 
 	const PM<double, PS> gR = PM<double, PS>{}, int nk = 0)
 {
 	typedef d4_t V4;
-	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
-	const int w = uni(wave_id());
-	const int mrt = AUG ? (nk + 1 + 15) / 16 : nkt;   // row tiles
-	const int nrows = AUG ? nk + 1 : nk;
 	const int kt = (n1 + 15) / 16;
-	int ntile = 0;
-	for (int bj = 0; bj < nkt; bj++)
-		ntile += mrt - bj;
-	const int mine = (ntile - w + NW - 1) / NW;   // tiles w, w + NW, ...
+	const int mine = ric_syrk_mine<NT, PS, FROMG, AUG>(nkt, nk);
 	for (int c0 = 0; c0 < mine; c0 += 4)
 	{
-		const int cnt = min(4, mine - c0);
-		int I[4], J[4];
-#pragma unroll
-		for (int t = 0; t < 4; t++)
-		{
-			int q = w + NW * (c0 + min(t, cnt - 1)), bj = 0;
-			while (q >= mrt - bj)
-			{
-				q -= mrt - bj;
-				bj++;
-			}
-			I[t] = bj + q;
-			J[t] = bj;
-		}
-		V4 acc[4], rq[4];
+		RicChunk ch;
+		ric_syrk_chunk<NT, PS, FROMG, AUG>(ch, c0, mine, nkt, nk, gR);
+		const int cnt = ch.cnt;
+		V4 acc[4];
 #pragma unroll
 		for (int t = 0; t < 4; t++)
 		{
 			acc[t] = V4{0, 0, 0, 0};
-			rq[t] = V4{0, 0, 0, 0};
-			if (t < cnt)
-			{
-				const int col = J[t] * 16 + (lane & 15);
+			if (!FROMG && t < cnt)
 #pragma unroll
 				for (int r = 0; r < 4; r++)
-				{
-					const int row = I[t] * 16 + acc_row<double>(lane, r);
-					if (FROMG)
-					{
-						if (row < nrows && col < nk && row >= col)
-							rq[t][r] = gR.at(row, col);
-					}
-					else
-						acc[t][r] = M(row, col);
-				}
-			}
+					acc[t][r] = M(ch.I[t] * 16 + acc_row<double>(lane, r), ch.J[t] * 16 + (lane & 15));
 		}
 		if (cnt == 4)
-			ric_syrk_tiles<4>(W, I, J, kt, acc);
+			ric_syrk_tiles<4>(W, ch.I, ch.J, kt, acc);
 		else if (cnt == 3)
-			ric_syrk_tiles<3>(W, I, J, kt, acc);
+			ric_syrk_tiles<3>(W, ch.I, ch.J, kt, acc);
 		else if (cnt == 2)
-			ric_syrk_tiles<2>(W, I, J, kt, acc);
+			ric_syrk_tiles<2>(W, ch.I, ch.J, kt, acc);
 		else
-			ric_syrk_tiles<1>(W, I, J, kt, acc);
+			ric_syrk_tiles<1>(W, ch.I, ch.J, kt, acc);
 #pragma unroll
 		for (int t = 0; t < 4; t++)
 		{
 			if (t < cnt)
 			{
-				const int col = J[t] * 16 + (lane & 15);
+				const int col = ch.J[t] * 16 + (lane & 15);
 #pragma unroll
 				for (int r = 0; r < 4; r++)
 				{
-					const double v = FROMG ? acc[t][r] + rq[t][r] : acc[t][r];
-					M(I[t] * 16 + acc_row<double>(lane, r), col) = (AUG && col >= nk) ? 0.0 : v;
+					const double v = FROMG ? acc[t][r] + ch.rq[t][r] : acc[t][r];
+					M(ch.I[t] * 16 + acc_row<double>(lane, r), col) = (AUG && col >= nk) ? 0.0 : v;
 				}
 			}
 		}

@@ -419,7 +419,8 @@ struct blasfeo_amd_riccati_arg
 
 /* Fused backward factorisation sweep, one workgroup per instance, all stages in
  * one launch (the per-stage trmm_rlnn + syrk_dpotrf_ln chain of
- * examples/example_d_riccati_recursion.c:120-138). */
+ * examples/example_d_riccati_recursion.c:120-138).  hint->max_m bounds
+ * nu[k]+nx[k] (at most 1024) and hint->max_n bounds nx[k] (0: max_m). */
 int blasfeo_amd_d_back_ric_trf_batched(int ps, int batch, const struct blasfeo_amd_riccati_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 
 /* The same sweep for callers that cannot allocate (code under HIP graph

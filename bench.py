@@ -140,6 +140,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-max-instances", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-tol", type=float, default=1e-13,
+                    help="ric_mixed: backward-error target of the refinement")
     ap.add_argument("--tune", action="store_true",
                     help="riccati: pick launch shapes and the panel size with blasfeo_amd.autotune on "
                          "this workload before timing")
@@ -148,7 +150,7 @@ def main():
                          "in a HIP graph (the default; every replay runs the whole call: plan, "
                          "queue reset and class launches)")
     ap.add_argument("--workload", default="riccati",
-                    choices=("riccati", "ric_trs", "ric_sv", "pack", "getrs", "potrf32", "gemm_syrk", "getrf", "sgetrf", "getrf_np",
+                    choices=("riccati", "ric_trs", "ric_sv", "ric_mixed", "pack", "getrs", "potrf32", "gemm_syrk", "getrf", "sgetrf", "getrf_np",
                              "posv256"),
                     help="riccati (default, the headline); the others measure the remaining "
                          "batched configurations")
@@ -184,6 +186,8 @@ def main():
         return run_ric_trs(args, torch, bfa)
     if args.workload == "ric_sv":
         return run_ric_sv(args, torch, bfa)
+    if args.workload == "ric_mixed":
+        return run_ric_mixed(args, torch, bfa)
     if args.workload == "pack":
         return run_pack(args, torch, bfa)
     if args.workload == "getrs":
@@ -429,6 +433,68 @@ def run_ric_trs(args, torch, bfa):
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                         "algorithmic_bytes": algo, "kernel": "k_ric_trs<%d>" % args.ps},
            "cpu_baseline": cpu, "checks": checks}
+    print(json.dumps(out), flush=True)
+
+
+def run_ric_mixed(args, torch, bfa):
+    """Mixed vs FP64 at equal error on the headline Riccati instances with
+    b, rq ~ N(0,1): the mixed-precision solve (FP32 factorisation, FP64
+    refinement of the KKT solution to backward error <= tol, FP64 fallback
+    for the instances that do not get there) against the FP64 factorisation
+    + solve, each step one batched call (the FP64 pair: two)."""
+    from blasfeo_amd.riccati import (RiccatiBatch, DeviceRiccati, RiccatiSolve, DeviceRiccatiSolve,
+                                     DeviceRiccatiMixed, trs_backward_error)
+    rng = np.random.default_rng(args.seed)
+    nxs = rng.integers(8, 65, args.batch)
+    nus = rng.integers(2, 17, args.batch)
+    tol, maxit = args.mixed_tol, 3
+    log("building %d Riccati instances (N=%d) and their solve vectors" % (args.batch, args.horizon))
+    rb = RiccatiBatch(nxs, nus, args.horizon, args.ps)
+    inp = rb.fill_inputs(rng)
+    rs = RiccatiSolve(rb)
+    sin = rs.fill_inputs(rng)
+    mix = DeviceRiccatiMixed(rb, inp, rs, sin)
+    fac = DeviceRiccati(rb, inp)
+    ds = DeviceRiccatiSolve(fac, rs, sin)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        ms = time_steps(torch, stream, lambda: mix.run(tol, maxit, sh), args.steps, args.warmup)
+
+        def fp64():
+            fac.run(sh)
+            ds.run(sh)
+        ms64 = time_steps(torch, stream, fp64, args.steps, args.warmup)
+    torch.cuda.synchronize()
+    info = mix.infos()
+    n_fb = int((info["path"] == 1).sum())
+    solves = args.batch / (ms * 1e-3)
+    log("mixed %.3f ms/step (%.0f solves/s), FP64 %.3f ms/step; %d of %d instances took the FP64 fallback"
+        % (ms, solves, ms64, n_fb, args.batch))
+    nchk = min(256, args.batch)
+    g64 = ds.host()
+    e64 = np.array([trs_backward_error(rb, inp, rs, sin, g64, i) for i in range(nchk)]) * np.finfo(float).eps
+    checks = {"tol": tol, "max_corrections": maxit, "fp64_fallback_instances": n_fb,
+              "corrections_histogram": np.bincount(info["iters"]).tolist(),
+              "mixed_berr_median": float(np.median(info["berr"])), "mixed_berr_max": float(info["berr"].max()),
+              "fp64_berr_median_first_%d" % nchk: float(np.median(e64)),
+              "pass": bool((info["berr"][info["path"] == 0] <= tol).all())}
+    flops = riccati_flops(nxs, nus, args.horizon)
+    out = {"metric": METRIC, "value": solves, "unit": "solves/s (mixed precision, KKT berr <= %g or FP64)" % tol,
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 factorisation, f64 refinement",
+           "data": "synthetic, seeded (numpy PCG64 seed %d): the headline Riccati instances, "
+                   "b and rq ~ N(0,1) per stage" % args.seed,
+           "config": {"workload": "riccati_solve_mixed_batched", "batch": args.batch, "horizon": args.horizon,
+                      "nx": "U{8..64}", "nu": "U{2..16}", "ps": args.ps,
+                      "fp64_trf_trs_ms_per_step": ms64, "mixed_speedup_vs_fp64": ms64 / ms,
+                      "parallelism": "one workgroup per instance, 1 GPU"},
+           "roofline": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+                        "flops_per_launch": flops,
+                        "kernel": "whole call (factorisation flops only, FP64 peak as the reference roof)"},
+           "cpu_baseline": None, "checks": checks}
     print(json.dumps(out), flush=True)
 
 

@@ -21,7 +21,7 @@ import numpy as np
 
 __all__ = ["lib", "lib_path", "Layout", "DMAT", "SMAT", "DGEMM_ARG", "SGEMM_ARG",
            "DTRSM_ARG", "STRSM_ARG", "DFACT_ARG", "SFACT_ARG", "DSYRK_DPOTRF_ARG",
-           "RICCATI_ARG", "RICCATI_TRS_ARG", "RICCATI_SV_ARG", "DVEC", "DPACK_ARG", "DROWPE_ARG", "PM_DMAT", "PM_DGECP_ARG", "pm_dgecp", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
+           "RICCATI_ARG", "RICCATI_TRS_ARG", "RICCATI_SV_ARG", "RICCATI_MIXED_ARG", "DVEC", "DPACK_ARG", "DROWPE_ARG", "PM_DMAT", "PM_DGECP_ARG", "pm_dgecp", "DPOSV_ARG", "SOLVE_INFO", "HINT", "posv_mixed", "ric_solve_mixed", "pm_pack", "pm_unpack", "memsize", "check"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libblasfeo_amd.so")
@@ -67,6 +67,9 @@ RICCATI_TRS_ARG = np.dtype([("N", "i4")] + [(f, "u8") for f in (
     "nx", "nu", "BAbt", "b", "rq", "L", "Pb", "ux", "pi", "work")], align=True)
 RICCATI_SV_ARG = np.dtype([("N", "i4")] + [(f, "u8") for f in (
     "nx", "nu", "BAbt", "RSQrq", "L", "ux", "pi", "work_mat", "work_vec")], align=True)
+RICCATI_MIXED_ARG = np.dtype([("N", "i4")] + [(f, "u8") for f in (
+    "nx", "nu", "BAbt", "RSQrq", "L", "work_mat", "b", "rq", "Pb", "ux", "pi", "work_vec",
+    "res_b", "res_rq", "d_Pb", "d_ux", "d_pi", "info")], align=True)
 DPOSV_ARG = np.dtype([("n", "i4"), ("nrhs", "i4")] +
                      [(f, "i4") for f in ("ai", "aj", "bi", "bj", "xi", "xj", "li", "lj")] +
                      [("sA", "u8"), ("sB", "u8"), ("sX", "u8"), ("sL", "u8"), ("info", "u8")], align=True)
@@ -109,6 +112,9 @@ def lib():
     L.blasfeo_amd_dposv_mixed_batched.argtypes = [ip, ip, vp, ctypes.c_double, ip,
                                                   ctypes.POINTER(HINT), vp]
     L.blasfeo_amd_dposv_mixed_batched.restype = ip
+    L.blasfeo_amd_d_ric_solve_mixed_batched.argtypes = [ip, ip, vp, ctypes.c_double, ip,
+                                                        ctypes.POINTER(HINT), vp]
+    L.blasfeo_amd_d_ric_solve_mixed_batched.restype = ip
     L.blasfeo_amd_tune_set.argtypes = [ctypes.c_char_p, ip, ip]
     L.blasfeo_amd_tune_set.restype = ip
     L.blasfeo_amd_tune_get.argtypes = [ctypes.c_char_p, ip]
@@ -162,6 +168,14 @@ def posv_mixed(ps, batch, args_ptr, tol, max_iter, hint, stream=0):
     check(lib().blasfeo_amd_dposv_mixed_batched(ps, batch, ctypes.c_void_p(int(args_ptr)), float(tol),
                                                 int(max_iter), h, ctypes.c_void_p(int(stream))),
           "dposv_mixed")
+
+
+def ric_solve_mixed(ps, batch, args_ptr, tol, max_iter, hint, stream=0):
+    """blasfeo_amd_d_ric_solve_mixed_batched with device pointers; raises on error."""
+    h = ctypes.byref(HINT(*[int(x) for x in hint]))
+    check(lib().blasfeo_amd_d_ric_solve_mixed_batched(ps, batch, ctypes.c_void_p(int(args_ptr)), float(tol),
+                                                      int(max_iter), h, ctypes.c_void_p(int(stream))),
+          "d_ric_solve_mixed")
 
 
 # ---------------------------------------------------------------------------

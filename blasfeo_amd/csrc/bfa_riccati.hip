@@ -47,38 +47,44 @@ constexpr int RT_SZ = 16 * RT_LD;
 
 __host__ __device__ __forceinline__ int ric_tix(int I, int J) { return I * (I + 1) / 2 + J; }
 
-struct TileLower
+// T: the sweep's arithmetic type (double; float for the mixed-precision form)
+template <typename T>
+struct TileLowerT
 {
-	double *p;
-	__device__ __forceinline__ double &operator()(int r, int c) const
+	T *p;
+	__device__ __forceinline__ T &operator()(int r, int c) const
 	{
 		return p[ric_tix(r >> 4, c >> 4) * RT_SZ + (c & 15) * RT_LD + (r & 15)];
 	}
 };
+typedef TileLowerT<double> TileLower;
 
-struct TileRect
+template <typename T>
+struct TileRectT
 {
-	double *p;
+	T *p;
 	int nct;  // tile columns per tile row
-	__device__ __forceinline__ double &operator()(int r, int c) const
+	__device__ __forceinline__ T &operator()(int r, int c) const
 	{
 		return p[((r >> 4) * nct + (c >> 4)) * RT_SZ + (c & 15) * RT_LD + (r & 15)];
 	}
 };
+typedef TileRectT<double> TileRect;
 
 // The first RIC_QSLOT bytes of a class kernel's dynamic LDS hold the work-queue
-// slot (bytes 0..3) and a double 0.0 (bytes 8..15, just before the sweep's M);
+// slot (bytes 0..3) and a 0.0 of the sweep's type (just before the sweep's M);
 // the class kernels declare no static LDS, so a launch asks for exactly its
 // class cap (at most the 160 KB of a CU).
 constexpr int RIC_QSLOT = 16;
 
 // LDS bytes of the resident form for an instance whose largest stage is
-// nkmax = max(nu+nx) and whose largest state is nxmax (queue slot included).
-__host__ __device__ __forceinline__ int ric_footprint(int nkmax, int nxmax)
+// nkmax = max(nu+nx) and whose largest state is nxmax (queue slot included),
+// with esz-byte elements.
+__host__ __device__ __forceinline__ int ric_footprint(int nkmax, int nxmax, int esz = 8)
 {
 	const int nkt = (nkmax + 15) / 16;
 	const int nxt = (nxmax + 15) / 16;
-	return ((nkt * (nkt + 1) / 2 + nkt * nxt) * RT_SZ + nkt * 16) * (int)sizeof(double) + RIC_QSLOT;
+	return ((nkt * (nkt + 1) / 2 + nkt * nxt) * RT_SZ + nkt * 16) * esz + RIC_QSLOT;
 }
 
 // ---------------------------------------------------------------------------
@@ -120,10 +126,10 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 // packed tiles, and for an element above Lxx's diagonal the address of the
 // zero kept just before M (ric_zero_slot).  Rows lk >= n1 need no zeroing:
 // W's columns from n1 on (the A operand) are zero.
-template <int NT>
-__device__ __forceinline__ void ric_trmm_w(const TileLower M, const TileRect W, int nkt, int n1, int n1t, int u1)
+template <int NT, typename T = double>
+__device__ __forceinline__ void ric_trmm_w(const TileLowerT<T> M, const TileRectT<T> W, int nkt, int n1, int n1t, int u1)
 {
-	typedef d4_t V4;
+	typedef typename Vec4<T>::type V4;
 	constexpr int NW = NT / WAVE;
 	const int lane = lane_id();
 	const int w = wave_id();
@@ -142,14 +148,14 @@ __device__ __forceinline__ void ric_trmm_w(const TileLower M, const TileRect W, 
 #pragma unroll
 		for (int ct = 0; ct < 4; ct++)
 			acc[ct] = V4{0, 0, 0, 0};
-		const double *pa = W.p + (size_t)rt * W.nct * RT_SZ + lq * RT_LD + lr;
+		const T *pa = W.p + (size_t)rt * W.nct * RT_SZ + lq * RT_LD + lr;
 		constexpr int ZOFF = -1;   // M.p[-1] holds 0.0
 #pragma unroll
 		for (int C = 0; C < 4; C++)
 		{
 			if (C < n1t)
 			{
-				double a[4], b[4][4];
+				T a[4], b[4][4];
 #pragma unroll
 				for (int s = 0; s < 4; s++)
 					a[s] = pa[C * RT_SZ + s * 4 * RT_LD];
@@ -177,23 +183,23 @@ __device__ __forceinline__ void ric_trmm_w(const TileLower M, const TileRect W, 
 			if (ct < n1t)
 #pragma unroll
 				for (int r = 0; r < 4; r++)
-					W(rt * 16 + acc_row<double>(lane, r), ct * 16 + lr) = acc[ct][r];
+					W(rt * 16 + acc_row<T>(lane, r), ct * 16 + lr) = acc[ct][r];
 	}
 }
 
 // acc[t] += W(strip I[t]) W(strip J[t])^T over the inner tiles 0..kt-1 of
 // W, for NTL tiles; per inner tile the 8*NTL operands are loaded before its
 // matrix-core steps.
-template <int NTL>
-__device__ __forceinline__ void ric_syrk_tiles(const TileRect W, const int (&I)[4], const int (&J)[4], int kt,
-	d4_t (&acc)[4])
+template <int NTL, typename T>
+__device__ __forceinline__ void ric_syrk_tiles(const TileRectT<T> W, const int (&I)[4], const int (&J)[4], int kt,
+	typename Vec4<T>::type (&acc)[4])
 {
 	const int lane = lane_id();
 	const int lo = (lane >> 4) * RT_LD + (lane & 15);
 	const int rs = W.nct * RT_SZ;   // one strip
 	for (int C = 0; C < kt; C++)
 	{
-		double a[NTL][4], b[NTL][4];
+		T a[NTL][4], b[NTL][4];
 #pragma unroll
 		for (int t = 0; t < NTL; t++)
 #pragma unroll
@@ -212,10 +218,11 @@ __device__ __forceinline__ void ric_syrk_tiles(const TileRect W, const int (&I)[
 
 // One wave's syrk tiles taken four at a time: chunk c0 of the wave's list
 // (tiles w + NW*(c0..c0+3)), and with FROMG the chunk's RSQ values.
+template <typename T>
 struct RicChunk
 {
 	int I[4], J[4], cnt;
-	d4_t rq[4];
+	typename Vec4<T>::type rq[4];
 };
 
 template <int NT, int PS, bool FROMG, bool AUG>
@@ -230,8 +237,8 @@ __device__ __forceinline__ int ric_syrk_mine(int nkt, int nk)
 	return (ntile - w + NW - 1) / NW;   // tiles w, w + NW, ...
 }
 
-template <int NT, int PS, bool FROMG, bool AUG>
-__device__ __forceinline__ void ric_syrk_chunk(RicChunk &ch, int c0, int mine, int nkt, int nk,
+template <int NT, int PS, bool FROMG, bool AUG, typename T>
+__device__ __forceinline__ void ric_syrk_chunk(RicChunk<T> &ch, int c0, int mine, int nkt, int nk,
 	const PM<double, PS> gR)
 {
 	constexpr int NW = NT / WAVE;
@@ -255,16 +262,16 @@ __device__ __forceinline__ void ric_syrk_chunk(RicChunk &ch, int c0, int mine, i
 #pragma unroll
 	for (int t = 0; t < 4; t++)
 	{
-		ch.rq[t] = d4_t{0, 0, 0, 0};
+		ch.rq[t] = typename Vec4<T>::type{0, 0, 0, 0};
 		if (FROMG && t < ch.cnt)
 		{
 			const int col = ch.J[t] * 16 + (lane & 15);
 #pragma unroll
 			for (int r = 0; r < 4; r++)
 			{
-				const int row = ch.I[t] * 16 + acc_row<double>(lane, r);
+				const int row = ch.I[t] * 16 + acc_row<T>(lane, r);
 				if (row < nrows && col < nk && row >= col)
-					ch.rq[t][r] = gR.at(row, col);
+					ch.rq[t][r] = (T)gR.at(row, col);
 			}
 		}
 	}
@@ -283,18 +290,18 @@ __device__ __forceinline__ void ric_syrk_chunk(RicChunk &ch, int c0, int mine, i
 // padding columns zero; W's row nk is not padding).
 // W's columns from n1 on are zero (the trmm writes them so), so the inner
 // dimension runs over whole 16-column tiles.
-template <int NT, int PS = 4, bool FROMG = false, bool AUG = false>
-__device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, int nkt, int n1,
+template <int NT, int PS = 4, bool FROMG = false, bool AUG = false, typename T = double>
+__device__ __forceinline__ void ric_syrk_m(const TileLowerT<T> M, const TileRectT<T> W, int nkt, int n1,
 	const PM<double, PS> gR = PM<double, PS>{}, int nk = 0)
 {
-	typedef d4_t V4;
+	typedef typename Vec4<T>::type V4;
 	const int lane = lane_id();
 	const int kt = (n1 + 15) / 16;
 	const int mine = ric_syrk_mine<NT, PS, FROMG, AUG>(nkt, nk);
 	for (int c0 = 0; c0 < mine; c0 += 4)
 	{
-		RicChunk ch;
-		ric_syrk_chunk<NT, PS, FROMG, AUG>(ch, c0, mine, nkt, nk, gR);
+		RicChunk<T> ch;
+		ric_syrk_chunk<NT, PS, FROMG, AUG, T>(ch, c0, mine, nkt, nk, gR);
 		const int cnt = ch.cnt;
 		V4 acc[4];
 #pragma unroll
@@ -304,16 +311,16 @@ __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, 
 			if (!FROMG && t < cnt)
 #pragma unroll
 				for (int r = 0; r < 4; r++)
-					acc[t][r] = M(ch.I[t] * 16 + acc_row<double>(lane, r), ch.J[t] * 16 + (lane & 15));
+					acc[t][r] = M(ch.I[t] * 16 + acc_row<T>(lane, r), ch.J[t] * 16 + (lane & 15));
 		}
 		if (cnt == 4)
-			ric_syrk_tiles<4>(W, ch.I, ch.J, kt, acc);
+			ric_syrk_tiles<4, T>(W, ch.I, ch.J, kt, acc);
 		else if (cnt == 3)
-			ric_syrk_tiles<3>(W, ch.I, ch.J, kt, acc);
+			ric_syrk_tiles<3, T>(W, ch.I, ch.J, kt, acc);
 		else if (cnt == 2)
-			ric_syrk_tiles<2>(W, ch.I, ch.J, kt, acc);
+			ric_syrk_tiles<2, T>(W, ch.I, ch.J, kt, acc);
 		else
-			ric_syrk_tiles<1>(W, ch.I, ch.J, kt, acc);
+			ric_syrk_tiles<1, T>(W, ch.I, ch.J, kt, acc);
 #pragma unroll
 		for (int t = 0; t < 4; t++)
 		{
@@ -323,8 +330,8 @@ __device__ __forceinline__ void ric_syrk_m(const TileLower M, const TileRect W, 
 #pragma unroll
 				for (int r = 0; r < 4; r++)
 				{
-					const double v = FROMG ? acc[t][r] + ch.rq[t][r] : acc[t][r];
-					M(ch.I[t] * 16 + acc_row<double>(lane, r), col) = (AUG && col >= nk) ? 0.0 : v;
+					const T v = FROMG ? acc[t][r] + ch.rq[t][r] : acc[t][r];
+					M(ch.I[t] * 16 + acc_row<T>(lane, r), col) = (AUG && col >= nk) ? T(0) : v;
 				}
 			}
 		}
@@ -361,9 +368,13 @@ __device__ __forceinline__ blasfeo_dmat *ric_work(const blasfeo_amd_riccati_sv_a
 // [r; q]^T), carried through the trmm, the syrk and the trapezoid Cholesky
 // (L[k] is (nk+1) x nk); before the syrk the row of the previous factor is
 // added to W's last row (the reference's blasfeo_dgead).
-template <int PS, int NT, bool AUG = false, typename ArgT = blasfeo_amd_riccati_arg>
-__device__ void ric_sweep_lds(const ArgT &a, double *smem)
+// T = float: the mixed-precision form -- the stage matrices and factors are
+// held and computed in FP32 (inputs converted on staging, factors written to
+// the FP64 outputs), multi-wave kernels only.
+template <int PS, int NT, bool AUG = false, typename ArgT = blasfeo_amd_riccati_arg, typename T = double>
+__device__ void ric_sweep_lds(const ArgT &a, T *smem)
 {
+	static_assert(sizeof(T) == 8 || NT != WAVE, "the FP32 sweep runs on two or four waves per instance");
 	constexpr int NMAXC = (NT == WAVE) ? 64 : 128;
 	constexpr int XR = AUG ? 1 : 0;   // extra row of the augmented form
 	const int N = uni(a.N);
@@ -383,9 +394,9 @@ __device__ void ric_sweep_lds(const ArgT &a, double *smem)
 	const int nkt_max = (nkmax + XR + 15) / 16;
 	const int nxt_max = (nxmax + 15) / 16;
 	const int nlow = nkt_max * (nkt_max + 1) / 2;
-	const TileLower M{smem};
-	const TileRect W{smem + (size_t)nlow * RT_SZ, nxt_max};
-	double *invd = smem + (size_t)(nlow + nkt_max * nxt_max) * RT_SZ;
+	const TileLowerT<T> M{smem};
+	const TileRectT<T> W{smem + (size_t)nlow * RT_SZ, nxt_max};
+	T *invd = smem + (size_t)(nlow + nkt_max * nxt_max) * RT_SZ;
 
 	// s = -1 is the last stage, L[N] = chol(Q_N) (potrf_l; potrf_l_mn of the
 	// (nx+1) x nx trapezoid when AUG); s >= 0 is stage k = N-1-s.  One call
@@ -418,7 +429,7 @@ __device__ void ric_sweep_lds(const ArgT &a, double *smem)
 			if constexpr (NT == WAVE)
 				ric_trmm_call<NT>(M, W, mrt, n1, n1t, u1);
 			else
-				ric_trmm_w<NT>(M, W, mrt, n1, n1t, u1);
+				ric_trmm_w<NT, T>(M, W, mrt, n1, n1t, u1);
 			__syncthreads();
 			if constexpr (AUG)
 			{
@@ -441,7 +452,7 @@ __device__ void ric_sweep_lds(const ArgT &a, double *smem)
 				ric_syrk_call<NT>(M, W, nkt, uni(nx[k + 1]));
 			}
 			else
-				ric_syrk_m<NT, PS, true, AUG>(M, W, nkt, uni(nx[k + 1]), view<double, PS>(&RSQ[k]), nk);
+				ric_syrk_m<NT, PS, true, AUG, T>(M, W, nkt, uni(nx[k + 1]), view<double, PS>(&RSQ[k]), nk);
 		}
 		// L[k] = chol(M); W is free (the syrk has read it), so the waves
 		// other than 0 load the next stage's BAbt into it meanwhile
@@ -454,11 +465,11 @@ __device__ void ric_sweep_lds(const ArgT &a, double *smem)
 				if (kn >= 0)
 				{
 					const int nkn = uni(nu[kn] + nx[kn]), n1n = uni(nx[kn + 1]);
-					g2s_acc<double, PS, NT, false, TileRect, 1>(view<double, PS>(&BAbt[kn]), 0, 0,
+					g2s_acc<double, PS, NT, false, TileRectT<T>, 1>(view<double, PS>(&BAbt[kn]), 0, 0,
 						(nkn + XR + 15) / 16 * 16, (n1n + 15) / 16 * 16, nkn + XR, n1n, W);
 				}
 			};
-			chol_acc<double, NT, NMAXC, TileLower, true, decltype(side)>(M, nk + XR, nk, invd, side);
+			chol_acc<T, NT, NMAXC, TileLowerT<T>, true, decltype(side)>(M, nk + XR, nk, invd, side);
 		}
 		s2g_acc<double, PS, NT, true>(view<double, PS>(&L[k]), 0, 0, nk + XR, nk, M, 0);
 		for (int j = threadIdx.x; j < nk; j += NT)
@@ -548,12 +559,14 @@ template <> struct RicArg<true> { typedef blasfeo_amd_riccati_sv_arg type; };
 
 // AUG: stage matrices have one more row; no global-memory sweep serves them,
 // so instances that fit no LDS class are dropped (the entry refuses size
-// bounds that allow them).
-template <bool AUG = false>
+// bounds that allow them).  T = float: the classes of the FP32 sweep (half
+// the bytes per element); it has no global-memory form either.
+template <bool AUG = false, typename T = double>
 __global__ void k_ric_plan(const typename RicArg<AUG>::type *__restrict__ args, int batch, int mmax,
 	int *counts, int *lists)
 {
 	constexpr int XR = AUG ? 1 : 0;
+	constexpr bool NOGEN = AUG || sizeof(T) != 8;
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= batch)
 		return;
@@ -576,12 +589,12 @@ __global__ void k_ric_plan(const typename RicArg<AUG>::type *__restrict__ args, 
 	if (nxmax <= 64 && nkmax + XR <= 128)
 	{
 		// one-wave classes (cap <= 40960) factor with one panel row per lane: nkmax <= 64
-		const int f = ric_footprint(nkmax + XR, nxmax);
+		const int f = ric_footprint(nkmax + XR, nxmax, (int)sizeof(T));
 		for (int c = RIC_NCLASS - 1; c >= 0; c--)
 			if (f <= c_ric_cap[c] && (c_ric_cap[c] > 40960 || nkmax + XR <= 64))
 				cls = c;
 	}
-	if (cls == RIC_GENERAL && (AUG || nkmax > mmax))
+	if (cls == RIC_GENERAL && (NOGEN || nkmax > mmax))
 		return;
 	const int slot = atomicAdd(&counts[cls], 1);
 	lists[(size_t)cls * batch + slot] = i;
@@ -612,19 +625,19 @@ __device__ __forceinline__ int ric_next(int *head, int *slot)
 	}
 }
 
-template <int PS, int NT, bool AUG = false>
+template <int PS, int NT, bool AUG = false, typename T = double>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_ric_class(const typename RicArg<AUG>::type *__restrict__ args,
 	const int *__restrict__ count, const int *__restrict__ list, int *head)
 {
 	static_assert(!(AUG && NT == WAVE), "the fused sweep runs on two or four waves per instance");
 	extern __shared__ double smem_d[];
 	int *slot = reinterpret_cast<int *>(smem_d);
-	double *work = smem_d + RIC_QSLOT / sizeof(double);
+	T *work = reinterpret_cast<T *>(smem_d) + RIC_QSLOT / sizeof(T);
 	if (threadIdx.x == 0)
-		work[-1] = 0.0;   // the trmm's zero operand (ric_trmm_w)
+		work[-1] = T(0);   // the trmm's zero operand (ric_trmm_w)
 	const int cnt = *count;
 	for (int t = ric_next<NT>(head, slot); t < cnt; t = ric_next<NT>(head, slot))
-		ric_sweep_lds<PS, NT, AUG>(args[list[t]], work);
+		ric_sweep_lds<PS, NT, AUG, typename RicArg<AUG>::type, T>(args[list[t]], work);
 }
 
 // one wave per instance (selected by the tuning knobs, not by default):
@@ -661,16 +674,16 @@ __global__ __launch_bounds__(NT3) void k_ric_general(const blasfeo_amd_riccati_a
 		ric_sweep_general<PS>(args[list[t]], P, LDP, mmax, As, Bs, L11, invs);
 }
 
-template <int PS, int NT, bool AUG = false>
+template <int PS, int NT, bool AUG = false, typename T = double>
 static int launch_class(int cls, int grid, const typename RicArg<AUG>::type *args, int *counts, const int *lists,
 	int batch, hipStream_t st, const char *name)
 {
-	if constexpr (AUG)
+	if constexpr (AUG || sizeof(T) != 8)
 	{
-		static_assert(NT != WAVE, "the fused sweep runs on two or four waves per instance");
-		if (const int rc = lds_check(reinterpret_cast<const void *>(&k_ric_class<PS, NT, true>), h_ric_cap[cls], name))
+		static_assert(NT != WAVE, "the fused and FP32 sweeps run on two or four waves per instance");
+		if (const int rc = lds_check(reinterpret_cast<const void *>(&k_ric_class<PS, NT, AUG, T>), h_ric_cap[cls], name))
 			return rc;
-		hipLaunchKernelGGL((k_ric_class<PS, NT, true>), dim3(grid), dim3(NT), h_ric_cap[cls], st, args,
+		hipLaunchKernelGGL((k_ric_class<PS, NT, AUG, T>), dim3(grid), dim3(NT), h_ric_cap[cls], st, args,
 			counts + cls, lists + (size_t)cls * batch, counts + RIC_HEAD + cls);
 		return check_launch(name);
 	}
@@ -728,14 +741,14 @@ static RicFork *ric_fork_state()
 
 // ws: the plan workspace, (RIC_NCLASS + 1) * batch + 64 ints.  fork: run the
 // class launches side by side on helper streams (else all on st, in order).
-template <int PS, bool AUG = false>
+template <int PS, bool AUG = false, typename T = double>
 static int launch_ric(int batch, const typename RicArg<AUG>::type *args, int mmax, int maxfoot, int *ws, bool fork,
 	hipStream_t st, const char *name)
 {
 	int *counts = ws;
 	int *lists = ws + 64;
 	hipLaunchKernelGGL(k_ric_zero, dim3(1), dim3(64), 0, st, counts);
-	hipLaunchKernelGGL((k_ric_plan<AUG>), dim3((batch + 255) / 256), dim3(256), 0, st, args, batch, mmax, counts, lists);
+	hipLaunchKernelGGL((k_ric_plan<AUG, T>), dim3((batch + 255) / 256), dim3(256), 0, st, args, batch, mmax, counts, lists);
 	int rc = check_launch(name);
 	if (rc)
 		return rc;
@@ -761,7 +774,7 @@ static int launch_ric(int batch, const typename RicArg<AUG>::type *args, int mma
 	// general class first (largest instances), then resident classes largest
 	// first, dealt round-robin over the caller's and the helper streams
 	int q = 0;
-	if constexpr (!AUG)
+	if constexpr (!AUG && sizeof(T) == 8)
 	{
 		const size_t shm = sizeof(double) * PW * ((mmax + 15) / 16 * 16 + 1);
 		const int grid = min(batch, cus);
@@ -799,18 +812,25 @@ static int launch_ric(int batch, const typename RicArg<AUG>::type *args, int mma
 			nt = 256;
 			per_cu = max(1, min(2, RIC_LDS_MAX / h_ric_cap[cls]));
 		}
+		if constexpr (sizeof(T) != 8)
+		{
+			// FP32 sweep: as many resident instances as LDS and three waves
+			// per SIMD allow
+			per_cu = min(nt == 128 ? 6 : 3, RIC_LDS_MAX / h_ric_cap[cls]);
+		}
 		if (const int v = tune_value(TUNE_RIC_PER_CU, cls))
 			per_cu = v;
 		if (const int v = tune_value(TUNE_RIC_THREADS, cls))
 			nt = v;
 		const int grid = min(batch, cus * per_cu);
 		hipStream_t sc = ss[q++ % (1 + RIC_AUX)];
-		if (nt == 64 && !AUG)
-			rc = launch_class<PS, AUG ? 128 : 64, AUG>(cls, grid, args, counts, lists, batch, sc, name);
+		constexpr bool ONEWAVE = !AUG && sizeof(T) == 8;
+		if (nt == 64 && ONEWAVE)
+			rc = launch_class<PS, ONEWAVE ? 64 : 128, AUG, T>(cls, grid, args, counts, lists, batch, sc, name);
 		else if (nt <= 128)
-			rc = launch_class<PS, 128, AUG>(cls, grid, args, counts, lists, batch, sc, name);
+			rc = launch_class<PS, 128, AUG, T>(cls, grid, args, counts, lists, batch, sc, name);
 		else
-			rc = launch_class<PS, 256, AUG>(cls, grid, args, counts, lists, batch, sc, name);
+			rc = launch_class<PS, 256, AUG, T>(cls, grid, args, counts, lists, batch, sc, name);
 		if (rc)
 			break;
 	}
@@ -1107,4 +1127,249 @@ extern "C" int blasfeo_amd_d_back_ric_sv_batched(int ps, int batch, const blasfe
 	else
 		hipLaunchKernelGGL(k_ric_sv_fwd<8>, dim3(batch), dim3(WAVE), 0, st, args, batch);
 	return check_launch(name);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision Riccati solve: FP32 factorisation, solve on those factors,
+// refinement with FP64 residuals of the KKT equations, FP64 fallback.
+// ---------------------------------------------------------------------------
+namespace bfa {
+
+// the argument records of the steps, from the caller's records
+__global__ void k_mix_setup(const blasfeo_amd_riccati_mixed_arg *__restrict__ ma, int batch,
+	blasfeo_amd_riccati_arg *ra, blasfeo_amd_riccati_trs_arg *tm, blasfeo_amd_riccati_trs_arg *tc)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= batch)
+		return;
+	const blasfeo_amd_riccati_mixed_arg a = ma[i];
+	ra[i] = blasfeo_amd_riccati_arg{a.N, a.nx, a.nu, a.BAbt, a.RSQrq, a.L, a.work_mat};
+	tm[i] = blasfeo_amd_riccati_trs_arg{a.N, a.nx, a.nu, a.BAbt, a.b, a.rq, a.L, a.Pb, a.ux, a.pi, a.work_vec};
+	tc[i] = blasfeo_amd_riccati_trs_arg{a.N, a.nx, a.nu, a.BAbt, a.res_b, a.res_rq, a.L, a.d_Pb, a.d_ux, a.d_pi,
+		a.work_vec};
+	if (a.info)
+		*a.info = blasfeo_amd_solve_info{0, 0, 0.0};
+}
+
+// max over the wave of v >= 0, NaN counted as +inf, in every lane
+__device__ __forceinline__ double mix_wmax(double v)
+{
+	v = (v == v) ? v : __builtin_huge_val();
+	return readlane(wave_max_d(v), WAVE - 1);
+}
+
+// One wave per instance still open (tc[i].N >= 0): mode 1 first applies the
+// correction (ux += d_ux, pi += d_pi); then the KKT residuals of ux, pi are
+// written to res_rq / res_b and the normwise backward error to info->berr;
+// an instance at or below tol is closed (tc[i].N = -1), so the next
+// correction solve and pass skip it.  mode 2 (after the FP64 fallback)
+// only records the backward error.
+template <int PS>
+__global__ __launch_bounds__(WAVE) void k_mix_residual(const blasfeo_amd_riccati_mixed_arg *__restrict__ ma,
+	blasfeo_amd_riccati_trs_arg *tc, int batch, double tol, int mode)
+{
+	const int i = blockIdx.x;
+	if (i >= batch || tc[i].N < 0)
+		return;
+	const blasfeo_amd_riccati_mixed_arg &a = ma[i];
+	const int lane = lane_id();
+	const int N = uni(a.N);
+	if (mode == 1)
+	{
+		for (int k = 0; k <= N; k++)
+		{
+			const int nk = uni(a.nu[k] + a.nx[k]);
+			const int nx1 = k < N ? uni(a.nx[k + 1]) : 0;
+			for (int j = lane; j < nk; j += WAVE)
+				a.ux[k].pa[j] += a.d_ux[k].pa[j];
+			for (int j = lane; j < nx1; j += WAVE)
+				a.pi[k].pa[j] += a.d_pi[k].pa[j];
+		}
+		if (lane == 0 && a.info)
+			a.info->iters++;
+	}
+	double berr = 0.0;
+	for (int k = 0; k <= N; k++)
+	{
+		const int nuk = uni(a.nu[k]), nk = uni(a.nu[k] + a.nx[k]);
+		const int nx1 = k < N ? uni(a.nx[k + 1]) : 0;
+		const double *z = a.ux[k].pa;
+		const double *g = a.rq[k].pa;
+		const double *p1 = k < N ? a.pi[k].pa : nullptr;
+		const double *p0 = k > 0 ? a.pi[k - 1].pa : nullptr;
+		const PM<double, PS> H = view<double, PS>(&a.RSQrq[k]);
+		double rmax = 0.0, hmax = 0.0, cmax = 0.0;
+		for (int r = lane; r < nk; r += WAVE)
+		{
+			double acc = g[r], hs = 0.0, cs = 0.0;
+			for (int j = 0; j < nk; j++)
+			{
+				const double h = j <= r ? H.at(r, j) : H.at(j, r);
+				acc += h * z[j];
+				hs += fabs(h);
+			}
+			if (k < N)
+			{
+				const PM<double, PS> C = view<double, PS>(&a.BAbt[k]);
+				for (int j = 0; j < nx1; j++)
+				{
+					const double c = C.at(r, j);
+					acc += c * p1[j];
+					cs += fabs(c);
+				}
+			}
+			if (k > 0 && r >= nuk)
+				acc -= p0[r - nuk];
+			a.res_rq[k].pa[r] = acc;
+			rmax = fmax(rmax, (acc == acc) ? fabs(acc) : __builtin_huge_val());
+			hmax = fmax(hmax, hs);
+			cmax = fmax(cmax, cs);
+		}
+		double zmax = 0.0, gmax = 0.0, p1max = 0.0, p0max = 0.0;
+		for (int j = lane; j < nk; j += WAVE)
+		{
+			zmax = fmax(zmax, fabs(z[j]));
+			gmax = fmax(gmax, fabs(g[j]));
+		}
+		for (int j = lane; j < nx1; j += WAVE)
+			p1max = fmax(p1max, fabs(p1[j]));
+		if (k > 0)
+			for (int j = lane; j < nk - nuk; j += WAVE)
+				p0max = fmax(p0max, fabs(p0[j]));
+		rmax = mix_wmax(rmax);
+		const double nz = mix_wmax(zmax);
+		double den = mix_wmax(hmax) * nz + mix_wmax(gmax) + mix_wmax(cmax) * mix_wmax(p1max) + mix_wmax(p0max);
+		berr = fmax(berr, rmax > 0.0 ? rmax / den : 0.0);
+		if (k < N)
+		{
+			// dynamics: s = BAbt^T ux_k + b_k - x_{k+1}
+			const PM<double, PS> C = view<double, PS>(&a.BAbt[k]);
+			const double *b = a.b[k].pa;
+			const int nu1 = uni(a.nu[k + 1]);
+			const double *x1 = a.ux[k + 1].pa + nu1;
+			double smax = 0.0, ctmax = 0.0, bmax = 0.0, xmax = 0.0;
+			for (int c = lane; c < nx1; c += WAVE)
+			{
+				double acc = b[c] - x1[c], cs = 0.0;
+				for (int r = 0; r < nk; r++)
+				{
+					const double v = C.at(r, c);
+					acc += v * z[r];
+					cs += fabs(v);
+				}
+				a.res_b[k].pa[c] = acc;
+				smax = fmax(smax, (acc == acc) ? fabs(acc) : __builtin_huge_val());
+				ctmax = fmax(ctmax, cs);
+				bmax = fmax(bmax, fabs(b[c]));
+				xmax = fmax(xmax, fabs(x1[c]));
+			}
+			smax = mix_wmax(smax);
+			den = mix_wmax(ctmax) * nz + mix_wmax(bmax) + mix_wmax(xmax);
+			berr = fmax(berr, smax > 0.0 ? smax / den : 0.0);
+		}
+	}
+	if (lane == 0)
+	{
+		if (a.info)
+			a.info->berr = berr;
+		if (mode == 2 || berr <= tol)
+			tc[i].N = -1;
+	}
+}
+
+// the FP64 fallback: instances still open keep their N in ra / tm (and are
+// marked path 1), the others get N = -1 so the FP64 sweep and solve skip them
+__global__ void k_mix_fallback(const blasfeo_amd_riccati_mixed_arg *__restrict__ ma, int batch,
+	blasfeo_amd_riccati_arg *ra, blasfeo_amd_riccati_trs_arg *tm, blasfeo_amd_riccati_trs_arg *tc)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= batch)
+		return;
+	const bool open = tc[i].N >= 0;
+	ra[i].N = open ? ma[i].N : -1;
+	tm[i].N = open ? ma[i].N : -1;
+	if (open && ma[i].info)
+		ma[i].info->path = 1;
+}
+
+} // namespace bfa
+
+extern "C" int blasfeo_amd_d_ric_solve_mixed_batched(int ps, int batch, const blasfeo_amd_riccati_mixed_arg *args,
+	double tol, int maxit, const blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream)
+{
+	using namespace bfa;
+	const char *name = "blasfeo_amd_d_ric_solve_mixed_batched";
+	if (batch < 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: batch=%d < 0", name, batch);
+	if (batch == 0)
+		return BLASFEO_AMD_OK;
+	if (!args)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: args is NULL", name);
+	if (ps != 4 && ps != 8)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: unsupported panel size %d", name, ps);
+	if (!hint || hint->max_m <= 0)
+		return set_error(BLASFEO_AMD_EINVAL, "%s: hint->max_m (largest nu[k]+nx[k]) is required", name);
+	if (maxit < 0 || !(tol >= 0.0))
+		return set_error(BLASFEO_AMD_EINVAL, "%s: maxit=%d, tol=%g", name, maxit, tol);
+	const int mmax = hint->max_m;
+	const int nxmax = hint->max_n > 0 ? hint->max_n : mmax;
+	if (mmax > 128 || nxmax > 64)
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: nu+nx up to 128 and nx up to 64 supported, got %d and %d",
+			name, mmax, nxmax);
+	hipStream_t st = (hipStream_t)stream;
+	// plan workspace, then the three argument arrays
+	const size_t plan = sizeof(int) * ((size_t)(RIC_NCLASS + 1) * batch + 64);
+	const size_t off_ra = (plan + 255) / 256 * 256;
+	const size_t off_tm = off_ra + (sizeof(blasfeo_amd_riccati_arg) * batch + 255) / 256 * 256;
+	const size_t off_tc = off_tm + (sizeof(blasfeo_amd_riccati_trs_arg) * batch + 255) / 256 * 256;
+	const size_t wsbytes = off_tc + sizeof(blasfeo_amd_riccati_trs_arg) * batch;
+	char *ws = (char *)workspace(wsbytes, st);
+	if (!ws)
+		return set_error(BLASFEO_AMD_ENOMEM, "%s: cannot allocate the %zu-byte workspace", name, wsbytes);
+	int *pws = (int *)ws;
+	auto *ra = (blasfeo_amd_riccati_arg *)(ws + off_ra);
+	auto *tm = (blasfeo_amd_riccati_trs_arg *)(ws + off_tm);
+	auto *tc = (blasfeo_amd_riccati_trs_arg *)(ws + off_tc);
+	const dim3 g256((batch + 255) / 256);
+	hipLaunchKernelGGL(k_mix_setup, g256, dim3(256), 0, st, args, batch, ra, tm, tc);
+	int rc = check_launch(name);
+	// FP32 factorisation, solve on it
+	const int foot32 = ric_footprint(mmax, min(nxmax, mmax), 4);
+	if (!rc)
+		rc = ps == 4 ? launch_ric<4, false, float>(batch, ra, mmax, foot32, pws, true, st, name)
+		             : launch_ric<8, false, float>(batch, ra, mmax, foot32, pws, true, st, name);
+	blasfeo_amd_hint th{mmax, nxmax, 0};
+	if (!rc)
+		rc = blasfeo_amd_d_back_ric_trs_batched(ps, batch, tm, &th, stream);
+	auto residual = [&](int mode) {
+		if (ps == 4)
+			hipLaunchKernelGGL(k_mix_residual<4>, dim3(batch), dim3(WAVE), 0, st, args, tc, batch, tol, mode);
+		else
+			hipLaunchKernelGGL(k_mix_residual<8>, dim3(batch), dim3(WAVE), 0, st, args, tc, batch, tol, mode);
+		return check_launch(name);
+	};
+	if (!rc)
+		rc = residual(0);
+	// corrections on the FP32 factors
+	for (int it = 0; it < maxit && !rc; it++)
+	{
+		rc = blasfeo_amd_d_back_ric_trs_batched(ps, batch, tc, &th, stream);
+		if (!rc)
+			rc = residual(1);
+	}
+	// FP64 for whatever is still open
+	if (!rc)
+	{
+		hipLaunchKernelGGL(k_mix_fallback, g256, dim3(256), 0, st, args, batch, ra, tm, tc);
+		rc = check_launch(name);
+	}
+	const int foot64 = ric_footprint(mmax, min(nxmax, mmax));
+	if (!rc)
+		rc = ps == 4 ? launch_ric<4>(batch, ra, mmax, foot64, pws, true, st, name)
+		             : launch_ric<8>(batch, ra, mmax, foot64, pws, true, st, name);
+	if (!rc)
+		rc = blasfeo_amd_d_back_ric_trs_batched(ps, batch, tm, &th, stream);
+	if (!rc)
+		rc = residual(2);
+	return rc;
 }

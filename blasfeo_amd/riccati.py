@@ -383,6 +383,56 @@ class DeviceRiccatiSv:
         return self.vbuf.cpu().numpy()
 
 
+class DeviceRiccatiMixed:
+    """A RiccatiBatch, its RiccatiSolve vectors (b, rq in; Pb, ux, pi out) and
+    a second vector set for the corrections, resident in GPU memory, for
+    ``blasfeo_amd_d_ric_solve_mixed_batched`` (FP32 factorisation, FP64
+    refinement, FP64 fallback)."""
+
+    def __init__(self, rb, in_host_buf, rs, sol_host_buf):
+        import torch
+        from . import RICCATI_MIXED_ARG, SOLVE_INFO
+        from .device import to_device
+        self.rb, self.rs = rb, rs
+        self.fac = DeviceRiccati(rb, in_host_buf)
+        self.vbuf = torch.from_numpy(np.ascontiguousarray(sol_host_buf)).to("cuda")
+        self.cbuf = torch.zeros(rs.nbytes, dtype=torch.uint8, device="cuda")
+        self.tab = to_device(rs.table(self.vbuf.data_ptr()))
+        self.ctab = to_device(rs.table(self.cbuf.data_ptr()))
+        self.info = torch.zeros(rb.batch * SOLVE_INFO.itemsize, dtype=torch.uint8, device="cuda")
+        babt, rsq, L, work, nx, nu = self.fac.t_dev
+        N = rb.N
+        a = np.zeros(rb.batch, dtype=RICCATI_MIXED_ARG)
+        idx = np.arange(rb.batch, dtype=np.uint64)
+        a["N"] = N
+        a["nx"] = nx.data_ptr() + idx * 4 * (N + 1)
+        a["nu"] = nu.data_ptr() + idx * 4 * (N + 1)
+        a["BAbt"] = babt.data_ptr() + idx * DMAT.itemsize * max(N, 1)
+        a["RSQrq"] = rsq.data_ptr() + idx * DMAT.itemsize * (N + 1)
+        a["L"] = L.data_ptr() + idx * DMAT.itemsize * (N + 1)
+        a["work_mat"] = work.data_ptr() + idx * DMAT.itemsize
+        base = np.uint64(self.tab.data_ptr()) + idx * np.uint64(DVEC.itemsize * rs.per)
+        cbase = np.uint64(self.ctab.data_ptr()) + idx * np.uint64(DVEC.itemsize * rs.per)
+        for f, kind in (("b", "b"), ("rq", "rq"), ("Pb", "Pb"), ("ux", "ux"), ("pi", "pi"), ("work_vec", "work")):
+            a[f] = base + np.uint64(DVEC.itemsize * rs.start[kind])
+        for f, kind in (("res_b", "b"), ("res_rq", "rq"), ("d_Pb", "Pb"), ("d_ux", "ux"), ("d_pi", "pi")):
+            a[f] = cbase + np.uint64(DVEC.itemsize * rs.start[kind])
+        a["info"] = np.uint64(self.info.data_ptr()) + idx * np.uint64(SOLVE_INFO.itemsize)
+        self.args = to_device(a)
+
+    def run(self, tol=1e-14, max_iter=3, stream=0):
+        from . import ric_solve_mixed
+        rb = self.rb
+        ric_solve_mixed(rb.ps, rb.batch, self.args.data_ptr(), tol, max_iter, (rb.max_nk, rb.max_nx, 0), stream)
+
+    def vectors(self):
+        return self.vbuf.cpu().numpy()
+
+    def infos(self):
+        from . import SOLVE_INFO
+        return self.info.cpu().numpy().view(SOLVE_INFO)
+
+
 class HostRiccatiSv:
     """The same augmented batch and vectors in host memory for the CPU oracle
     (``or_dric_sv_batch``)."""

@@ -500,6 +500,53 @@ struct blasfeo_amd_riccati_sv_arg
  * BLASFEO_AMD_ETOOBIG. */
 int blasfeo_amd_d_back_ric_sv_batched(int ps, int batch, const struct blasfeo_amd_riccati_sv_arg *args, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
 
+/* Mixed-precision Riccati solve: no single counterpart in the reference; it
+ * computes what d_back_ric_trf_libstr followed by d_back_ric_trs_libstr
+ * (examples/example_d_riccati_recursion.c:115-140, 203-267) compute, the
+ * solution of each instance's KKT system, to normwise backward error <= tol.
+ * The factorisation runs in FP32 (stage matrices and factors held in FP32 in
+ * LDS, half the bytes of the FP64 sweep; FP32 matrix-core trmm/syrk and
+ * trailing updates; the factors written to L in FP64 storage), the solve on
+ * those factors, then iterative refinement: FP64 residuals of the KKT
+ * equations
+ *   r_k = RSQ_k ux_k + rq_k + BAbt_k pi_k - [0; pi_{k-1}],
+ *   s_k = BAbt_k^T ux_k + b_k - x_{k+1},
+ * and a correction solved on the FP32 factors (the same solve with b = s,
+ * rq = r).  The backward error is max_k of ||r_k|| / (||RSQ_k|| ||ux_k|| +
+ * ||rq_k|| + ||BAbt_k|| ||pi_k|| + ||pi_{k-1}||) and ||s_k|| / (||BAbt_k^T||
+ * ||ux_k|| + ||b_k|| + ||x_{k+1}||) (infinity norms).  An instance still
+ * above tol after maxit corrections is factored and solved again in FP64
+ * (its L and vectors overwritten).  Arguments as blasfeo_amd_riccati_arg and
+ * blasfeo_amd_riccati_trs_arg (BAbt, RSQrq, b and rq are read-only inputs;
+ * L, Pb, ux, pi outputs), plus correction vectors of the same shapes
+ * (res_b as b, res_rq as rq, d_Pb as Pb, d_ux as ux, d_pi as pi) and one
+ * info record per instance: iters = corrections applied, path 0 = mixed,
+ * 1 = FP64, berr = final backward error.  hint->max_m bounds nu[k]+nx[k] (at
+ * most 128) and hint->max_n bounds nx[k] (at most 64). */
+struct blasfeo_amd_riccati_mixed_arg
+	{
+	int N;
+	const int *nx;
+	const int *nu;
+	struct blasfeo_dmat *BAbt;
+	struct blasfeo_dmat *RSQrq;
+	struct blasfeo_dmat *L;
+	struct blasfeo_dmat *work_mat;
+	struct blasfeo_dvec *b;
+	struct blasfeo_dvec *rq;
+	struct blasfeo_dvec *Pb;
+	struct blasfeo_dvec *ux;
+	struct blasfeo_dvec *pi;
+	struct blasfeo_dvec *work_vec;
+	struct blasfeo_dvec *res_b;
+	struct blasfeo_dvec *res_rq;
+	struct blasfeo_dvec *d_Pb;
+	struct blasfeo_dvec *d_ux;
+	struct blasfeo_dvec *d_pi;
+	struct blasfeo_amd_solve_info *info;
+	};
+int blasfeo_amd_d_ric_solve_mixed_batched(int ps, int batch, const struct blasfeo_amd_riccati_mixed_arg *args, double tol, int maxit, const struct blasfeo_amd_hint *hint, blasfeo_amd_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Cholesky solve A X = B with A symmetric positive definite                */
 /* ------------------------------------------------------------------------ */

@@ -6,8 +6,12 @@ nx = 64, every matrix and every vector in its own exactly-sized block:
             diagonals, ux and pi against the oracle's d_back_ric_sv;
   trf <ps>  blasfeo_amd_d_back_ric_trf_batched (every class launch shape,
             the one-wave kernels included): factors and reciprocal
-            diagonals against the oracle's d_back_ric_trf.
-Usage: python case_ric.py sv|trf <ps>"""
+            diagonals against the oracle's d_back_ric_trf;
+  mixed <ps> blasfeo_amd_d_ric_solve_mixed_batched (the FP32 sweep, the
+            solve, the FP64 refinement): ux and pi against the oracle's
+            d_back_ric_trf + d_back_ric_trs, and every instance's backward
+            error at or below the tolerance.
+Usage: python case_ric.py sv|trf|mixed <ps>"""
 import ctypes
 import sys
 
@@ -162,5 +166,104 @@ def run(ps):
     print("ok", flush=True)
 
 
+def vec_blocks(rs, buf):
+    """Every vector of a RiccatiSolve buffer in its own block; returns the
+    blocks and the descriptor table pointing at them."""
+    tab = rs.table(0)
+    blocks = []
+    for j, r in enumerate(tab):
+        b = Block(int(r["memsize"]))
+        o = int(rs.offs[j])
+        b.write(np.asarray(buf)[o: o + int(r["memsize"])])
+        r["mem"] = r["pa"] = b.addr
+        blocks.append(b)
+    return blocks, tab
+
+
+def run_mixed(ps):
+    from blasfeo_amd.riccati import HostRiccati, RiccatiSolve
+    nxs = NXS + [2, 12]
+    nus = NUS + [1, 3]
+    rb = RiccatiBatch(nxs, nus, N=3, ps=ps)
+    rs = RiccatiSolve(rb)
+    rng = np.random.default_rng(970 + ps)
+    inp = rb.fill_inputs(rng)
+    sin = rs.fill_inputs(rng)
+    in_blocks, fix_in = split(rb.inp, inp, ("mem", "pA", "dA"))
+    out_blocks, fix_out = split(rb.out, rb.out.host_buffer(), ("mem", "pA", "dA"))
+    babt, rsq, L, work, nx, nu = rb.tables(0, 0)
+    fix_in(babt), fix_in(rsq), fix_out(L), fix_out(work)
+    vblocks, vtab = vec_blocks(rs, sin)
+    cblocks, ctab = vec_blocks(rs, np.zeros(rs.nbytes, dtype=np.uint8))
+    tbl = {}
+    for k, v in (("babt", babt), ("rsq", rsq), ("L", L), ("work", work), ("nx", nx), ("nu", nu), ("v", vtab),
+                 ("c", ctab)):
+        tbl[k] = Block(v.nbytes)
+        tbl[k].write(v)
+    info = Block(bfa.SOLVE_INFO.itemsize * rb.batch)
+    N = rb.N
+    a = np.zeros(rb.batch, dtype=bfa.RICCATI_MIXED_ARG)
+    idx = np.arange(rb.batch, dtype=np.uint64)
+    a["N"] = N
+    a["nx"] = tbl["nx"].addr + idx * 4 * (N + 1)
+    a["nu"] = tbl["nu"].addr + idx * 4 * (N + 1)
+    a["BAbt"] = tbl["babt"].addr + idx * bfa.DMAT.itemsize * N
+    a["RSQrq"] = tbl["rsq"].addr + idx * bfa.DMAT.itemsize * (N + 1)
+    a["L"] = tbl["L"].addr + idx * bfa.DMAT.itemsize * (N + 1)
+    a["work_mat"] = tbl["work"].addr + idx * bfa.DMAT.itemsize
+    base = np.uint64(tbl["v"].addr) + idx * np.uint64(bfa.DVEC.itemsize * rs.per)
+    cbase = np.uint64(tbl["c"].addr) + idx * np.uint64(bfa.DVEC.itemsize * rs.per)
+    for f, kind in (("b", "b"), ("rq", "rq"), ("Pb", "Pb"), ("ux", "ux"), ("pi", "pi"), ("work_vec", "work")):
+        a[f] = base + np.uint64(bfa.DVEC.itemsize * rs.start[kind])
+    for f, kind in (("res_b", "b"), ("res_rq", "rq"), ("d_Pb", "Pb"), ("d_ux", "ux"), ("d_pi", "pi")):
+        a[f] = cbase + np.uint64(bfa.DVEC.itemsize * rs.start[kind])
+    a["info"] = np.uint64(info.addr) + idx * np.uint64(bfa.SOLVE_INFO.itemsize)
+    ablk = Block(a.nbytes)
+    ablk.write(a)
+    E = emu_lib("emu_lapack")
+    E.blasfeo_amd_d_ric_solve_mixed_batched.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_double,
+                                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    # tol 0: no instance can converge, every one takes the FP64 fallback (then
+    # the results are the FP64 factorisation and solve, 1e-11 rule); then
+    # tol 1e-14 from the same inputs: the mixed path
+    for tol, rule in ((0.0, 1e-11), (1e-14, 1e-9)):
+        for b, src in zip(vblocks, vec_blocks(rs, sin)[0]):
+            b.write(src.read())
+        run_mixed_once(E, ps, rb, rs, N, ablk, info, tol, rule, vblocks, inp, sin)
+    print("ok", flush=True)
+
+
+def run_mixed_once(E, ps, rb, rs, N, ablk, info, tol, rule, vblocks, inp, sin):
+    from blasfeo_amd.riccati import HostRiccati
+    print("emu d_ric_solve_mixed ps=%d batch=%d N=%d tol=%g" % (ps, rb.batch, N, tol), flush=True)
+    rc = E.blasfeo_amd_d_ric_solve_mixed_batched(ps, rb.batch, ablk.addr, tol, 4,
+                                                 ctypes.addressof(HINT(rb.max_nk, rb.max_nx, 0)), None)
+    assert rc == 0, E.emu_last_error()
+    infos = info.read(bfa.SOLVE_INFO)
+    if tol == 0.0:
+        # an instance may reach a zero residual exactly (then it is closed)
+        assert ((infos["path"] == 1) & (infos["iters"] == 4) | (infos["berr"] == 0)).all(), infos
+    else:
+        assert (infos["berr"] <= tol).all() and (infos["path"] == 0).all(), infos
+    # oracle: FP64 factorisation and solve
+    host = HostRiccati(rb, inp)
+    oracle.lib().or_dric_trf_batch(ps, rb.batch, host.argv.ctypes.data)
+    hsol = sin.copy()
+    htab = rs.table(hsol.ctypes.data)
+    hb, hr, hL, hw, hnx, hnu = host.t
+    targs = rs.args(htab.ctypes.data, hb.ctypes.data, hL.ctypes.data, hnx.ctypes.data, hnu.ctypes.data)
+    oracle.lib().or_dric_trs_batch(ps, rb.batch, targs.ctypes.data)
+    for i in range(rb.batch):
+        for kind, ks in (("ux", range(N + 1)), ("pi", range(N))):
+            for k in ks:
+                j = rs.index(i, kind, k)
+                got = vblocks[j].read(np.float64, int(rs.sizes[j]))
+                want = rs.vec(hsol, i, kind, k)
+                compare(got, want, rule, "%s[%d] instance %d (path %d, iters %d, berr %.2e)"
+                        % (kind, k, i, infos[i]["path"], infos[i]["iters"], infos[i]["berr"]))
+    print("paths", np.bincount(infos["path"], minlength=2).tolist(), "iters", infos["iters"].tolist(),
+          "max berr %.2e" % infos["berr"].max(), flush=True)
+
+
 if __name__ == "__main__":
-    {"sv": run, "trf": run_trf}[sys.argv[1]](int(sys.argv[2]))
+    {"sv": run, "trf": run_trf, "mixed": run_mixed}[sys.argv[1]](int(sys.argv[2]))

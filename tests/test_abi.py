@@ -217,3 +217,17 @@ def test_riccati_sv_bounds():
         h = bfa.HINT(*mx)
         assert L.blasfeo_amd_d_back_ric_sv_batched(4, 1, ctypes.c_void_p(8), ctypes.byref(h), None) == -4
     assert L.blasfeo_amd_d_back_ric_sv_batched(4, 1, ctypes.c_void_p(8), None, None) == -1
+
+
+def test_riccati_mixed_bounds():
+    """The mixed-precision Riccati solve keeps the FP32 sweep in LDS: nu+nx up
+    to 128 and nx up to 64; larger bounds and a negative maxit or tol are
+    refused before any launch."""
+    L = bfa.lib()
+    f = L.blasfeo_amd_d_ric_solve_mixed_batched
+    for mx in ((129, 64, 0), (100, 65, 0)):
+        assert f(4, 1, ctypes.c_void_p(8), 1e-14, 3, ctypes.byref(bfa.HINT(*mx)), None) == -4
+    h = bfa.HINT(80, 64, 0)
+    assert f(4, 1, ctypes.c_void_p(8), 1e-14, -1, ctypes.byref(h), None) == -1
+    assert f(4, 1, ctypes.c_void_p(8), -1.0, 3, ctypes.byref(h), None) == -1
+    assert f(4, 1, ctypes.c_void_p(8), 1e-14, 3, None, None) == -1

@@ -90,3 +90,12 @@ def test_riccati_sv(emu_build, ps):
     """The fused Riccati factorisation and solve over every LDS footprint class
     up to nu+nx = 127, nx = 64, each matrix and vector in its own block."""
     run_case("case_ric.py", "sv", ps)
+
+
+@pytest.mark.parametrize("ps", [4, 8])
+def test_riccati_mixed(emu_build, ps):
+    """The mixed-precision Riccati solve: the FP32 sweep, the solve on its
+    factors and the FP64 refinement to backward error <= 1e-14 (ux, pi within
+    1e-9 of the oracle's FP64 factorisation + solve), and with tol 0 the FP64
+    fallback for every instance (the oracle's results at the 1e-11 rule)."""
+    run_case("case_ric.py", "mixed", ps)

@@ -429,8 +429,48 @@ __global__ __launch_bounds__(NTH) void k_potrf_tiles(const typename FactArg<T>::
 // pivot search, no swaps, ipiv not touched, and dA = 1/U(j,j) with no guard
 // for a zero pivot (the reference divides unconditionally)
 // ---------------------------------------------------------------------------
+// Launch order: the plan kernel sorts the instances into buckets of 16 rows
+// of max(m, n), largest first, and workgroup b of the launch takes the b-th
+// instance of that order (counts/lists == nullptr: the input order), so the
+// largest instances start first and the launch does not end on one of them
+// running alone.
+constexpr int GETRF_NB = 16;
+template <typename T>
+__global__ void k_getrf_plan(const typename FactArg<T>::type *__restrict__ args, int batch, int *counts, int *lists)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= batch)
+		return;
+	const int s = max(args[i].m, args[i].n);
+	const int b = s <= 0 ? GETRF_NB - 1 : max(0, GETRF_NB - 1 - (s - 1) / 16);
+	const int slot = atomicAdd(&counts[b], 1);
+	lists[(size_t)b * batch + slot] = i;
+}
+
+__global__ void k_zero_ints(int *p, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = 0;
+}
+
+__device__ __forceinline__ int getrf_order(int t, int batch, const int *counts, const int *lists)
+{
+	if (!counts)
+		return t;
+	for (int b = 0; b < GETRF_NB; b++)
+	{
+		const int c = counts[b];
+		if (t < c)
+			return lists[(size_t)b * batch + t];
+		t -= c;
+	}
+	return -1;
+}
+
 template <typename T, int PS, bool PIV = true>
-__global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k_getrf(const typename FactArg<T>::type *__restrict__ args, int mmax)
+__global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k_getrf(const typename FactArg<T>::type *__restrict__ args, int mmax,
+	int batch = 0, const int *__restrict__ counts = nullptr, const int *__restrict__ lists = nullptr)
 {
 	typedef typename Vec4<T>::type V4;
 	extern __shared__ unsigned char smem_raw[];
@@ -443,7 +483,10 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 	T *P = reinterpret_cast<T *>(smem_raw);                              // [PW][LDP]
 	int *perm = reinterpret_cast<int *>(smem_raw + sizeof(T) * PW * LDP);  // [mmax]
 
-	const typename FactArg<T>::type &a = args[blockIdx.x];
+	const int inst = getrf_order(blockIdx.x, batch, counts, lists);
+	if (inst < 0)
+		return;
+	const typename FactArg<T>::type &a = args[inst];
 	const int m = a.m, n = a.n;
 	if (threadIdx.x == 0)
 		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? 1 : 0;
@@ -964,12 +1007,19 @@ static int launch_getrf(int ps, int batch, const typename FactArg<T>::type *args
 	if ((rc = fact_mmax_arg<T>(hint, name, &mmax)))
 		return rc;
 	const size_t shm = sizeof(T) * PW * ((mmax + 15) / 16 * 16 + 1) + sizeof(int) * mmax;
+	// the size order (k_getrf_plan): counts (64 ints) and one list per bucket
+	int *ws = (int *)workspace(sizeof(int) * (64 + (size_t)GETRF_NB * batch), st);
+	if (!ws)
+		return set_error(BLASFEO_AMD_ENOMEM, "%s: workspace allocation failed", name);
+	int *counts = ws, *lists = ws + 64;
+	hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, counts, 64);
+	hipLaunchKernelGGL((k_getrf_plan<T>), dim3((batch + 255) / 256), dim3(256), 0, st, args, batch, counts, lists);
 	if (ps == 4)
-		hipLaunchKernelGGL((k_getrf<T, 4, PIV>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+		hipLaunchKernelGGL((k_getrf<T, 4, PIV>), dim3(batch), dim3(NTL), shm, st, args, mmax, batch, counts, lists);
 	else if (ps == 8)
-		hipLaunchKernelGGL((k_getrf<T, 8, PIV>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+		hipLaunchKernelGGL((k_getrf<T, 8, PIV>), dim3(batch), dim3(NTL), shm, st, args, mmax, batch, counts, lists);
 	else
-		hipLaunchKernelGGL((k_getrf<T, 16, PIV>), dim3(batch), dim3(NTL), shm, st, args, mmax);
+		hipLaunchKernelGGL((k_getrf<T, 16, PIV>), dim3(batch), dim3(NTL), shm, st, args, mmax, batch, counts, lists);
 	return check_launch(name);
 }
 

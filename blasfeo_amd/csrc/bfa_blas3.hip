@@ -548,13 +548,17 @@ static int launch_gemm(int ps, int batch, const typename GemmArg<T>::type *args,
 		const int nt = OP == OP_SYRK ? mt : (hint->max_n + GW_T - 1) / GW_T;
 		int gy = (OP == OP_SYRK || OP == OP_SYRK_MN) ? mt * (mt + 1) / 2 : mt * nt;
 		gy = gy < 1 ? 1 : gy;
-		dim3 gridw(batch, gy);
+		// one wave per (instance, tile slot), XCD-aware order (k_gemm_wave)
+		const long long waves = ((long long)batch * gy + XCDS - 1) / XCDS * XCDS;
+		if (waves > 0x7fffffffLL / WAVE)
+			return set_error(BLASFEO_AMD_ETOOBIG, "%s: batch %d too large for one launch", name, batch);
+		const dim3 gridw((unsigned)waves);
 		if (ps == 4)
-			hipLaunchKernelGGL((k_gemm_wave<T, 4, OP>), gridw, dim3(WAVE), 0, st, args);
+			hipLaunchKernelGGL((k_gemm_wave<T, 4, OP>), gridw, dim3(WAVE), 0, st, args, batch, gy);
 		else if (ps == 8)
-			hipLaunchKernelGGL((k_gemm_wave<T, 8, OP>), gridw, dim3(WAVE), 0, st, args);
+			hipLaunchKernelGGL((k_gemm_wave<T, 8, OP>), gridw, dim3(WAVE), 0, st, args, batch, gy);
 		else
-			hipLaunchKernelGGL((k_gemm_wave<T, 16, OP>), gridw, dim3(WAVE), 0, st, args);
+			hipLaunchKernelGGL((k_gemm_wave<T, 16, OP>), gridw, dim3(WAVE), 0, st, args, batch, gy);
 		return check_launch(name);
 	}
 	dim3 grid(batch, tiles_y(hint, TM, TN));

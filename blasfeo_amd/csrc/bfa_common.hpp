@@ -22,6 +22,19 @@ namespace bfa {
 
 constexpr int WAVE = 64;
 
+// MI355X has 8 XCDs (each with its own L2); the dispatcher deals the
+// workgroups of a launch to them round-robin in workgroup-id order.
+constexpr int XCDS = 8;
+
+// Logical index of this workgroup in a 1-D launch of `total` workgroups
+// (`total` a multiple of XCDS) such that each XCD gets one contiguous range
+// of logical indices, taken in order: workgroups that share data (the tiles
+// of one instance) then run on one XCD, close in time, and hit its L2.
+__device__ __forceinline__ unsigned xcd_local_index(unsigned g, unsigned total)
+{
+	return (g % XCDS) * (total / XCDS) + g / XCDS;
+}
+
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 

@@ -78,17 +78,31 @@ __device__ __forceinline__ void gw_load(const PM<T, PS> &gA, int ai, int aj, con
 	}
 }
 
+// One wave per (instance, tile slot): `slots` slots per instance (the tile
+// count of the hint's largest instance), a 1-D launch of batch * slots
+// workgroups rounded up to a multiple of XCDS.  The slots of one instance
+// map to consecutive logical indices on one XCD (xcd_local_index), so the
+// tiles that re-read the same A row block or B column block run together
+// on one L2; slot t >= the instance's tile count exits at once.  Registers
+// are capped for five waves per SIMD (92 VGPRs, no spills), so that more
+// waves' load latencies overlap: faster than four (124 VGPRs) on the
+// 100000-call uneven batch; six and eight spill and are slower.
 template <typename T, int PS, int OP>
-__global__ __launch_bounds__(WAVE) void k_gemm_wave(const typename GemmArg<T>::type *__restrict__ args)
+__global__ __launch_bounds__(WAVE, 5) void k_gemm_wave(const typename GemmArg<T>::type *__restrict__ args, int batch,
+	int slots)
 {
 	typedef typename Vec4<T>::type V4;
-	const typename GemmArg<T>::type &a = args[blockIdx.x];
+	const unsigned li = xcd_local_index(blockIdx.x, gridDim.x);
+	const int inst = (int)(li / (unsigned)slots), slot = (int)(li % (unsigned)slots);
+	if (inst >= batch)
+		return;
+	const typename GemmArg<T>::type &a = args[inst];
 	const int m = a.m;
 	const int n = (OP == OP_SYRK) ? a.m : a.n;
 	const int k = a.k;
 	if (m <= 0 || n <= 0)
 		return;
-	if (threadIdx.x == 0 && blockIdx.y == 0)
+	if (threadIdx.x == 0 && slot == 0)
 		a.sD->use_dA = 0;
 	const PM<T, PS> gA = view<T, PS>(a.sA), gB = view<T, PS>(a.sB), gC = view<T, PS>(a.sC);
 	PM<T, PS> gD = view<T, PS>(a.sD);
@@ -96,7 +110,7 @@ __global__ __launch_bounds__(WAVE) void k_gemm_wave(const typename GemmArg<T>::t
 	const int lane = threadIdx.x;
 	const int mt = (m + GW_T - 1) / GW_T, nt = (n + GW_T - 1) / GW_T;
 	const int ntiles = gw_ntiles<OP>(mt, nt);
-	for (int t = blockIdx.y; t < ntiles; t += gridDim.y)
+	for (int t = slot; t < ntiles; t += slots)
 	{
 		int ti, tj;
 		gw_tile<OP>(t, mt, nt, ti, tj);

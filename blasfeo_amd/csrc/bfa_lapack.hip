@@ -69,7 +69,8 @@ __global__ __launch_bounds__(NT) void k_potrf_lds(const typename FactArg<T>::typ
 		n = m;
 	const int mt = (m + 15) / 16, nt = (n + 15) / 16;
 	g2s<T, PS, NT, false, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, nt * 16, m, n, M, LDM);
-	chol_acc<T, NT, 128>(ColMajor<T>{M, LDM}, m, n, invd);
+	// the one-wave form serves m <= 32: one panel row per lane, no second row set
+	chol_acc<T, NT, NT == WAVE ? 64 : 128>(ColMajor<T>{M, LDM}, m, n, invd);
 	s2g<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, n, M, LDM, 0);
 	T *dD = a.sD->dA;
 	for (int j = threadIdx.x; j < n; j += NT)

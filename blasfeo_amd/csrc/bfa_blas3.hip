@@ -1,10 +1,16 @@
 // bfa_blas3.hip -- batched level-3 kernels for gfx950: gemm_nt, gemm_nn,
 // syrk_ln(_mn), trmm_rlnn and trsm_rltn on panel-major matrices.
 //
-// Launch model: the grid's x dimension walks the batch (one record per
-// workgroup), y walks output tiles of an instance (tile t, t+gridDim.y, ...),
-// so a batch of small instances uses one workgroup each and a batch of large
-// ones spreads its 64x64 tiles over several (tile engine: bfa_gemm_core.hpp).
+// Launch model: one workgroup per (instance, tile slot), `slots` slots per
+// instance (tile t of an instance goes to slot t mod slots), so a batch of
+// small instances uses one workgroup each and a batch of large ones spreads
+// its 64x64 tiles over several (tile engine: bfa_gemm_core.hpp).  The gemm
+// and trmm launches are 1-D (batch_slot): in XCD-aware order when an
+// instance has more than four slots (the slots of one instance run together
+// on one XCD and share its L2: 4000 gemm_nt of U{64..512} 7.68 -> 6.41 ms,
+// 20000 of U{4..300} 7.62 -> 6.34 ms on MI355X), instance-fastest otherwise
+// (100000 of U{4..128} on this engine: 5.35 against 8.07 ms in XCD order).
+// trsm keeps a 2-D grid (x the instance, y the slot).
 #include "bfa_gemm_core.hpp"
 #include "bfa_gemm_wave.hpp"
 #include "bfa_chol_core.hpp"
@@ -13,37 +19,46 @@
 namespace bfa {
 
 template <typename T, int PS, int OP>
-__global__ __launch_bounds__(NT3) __attribute__((amdgpu_waves_per_eu(3))) void k_gemm(const typename GemmArg<T>::type *__restrict__ args)
+__global__ __launch_bounds__(NT3) __attribute__((amdgpu_waves_per_eu(3))) void k_gemm(const typename GemmArg<T>::type *__restrict__ args,
+	int batch, int slots)
 {
 	__shared__ T As[KC * LDT];
 	__shared__ T Bs[KC * LDT];
-	const typename GemmArg<T>::type &a = args[blockIdx.x];
+	int inst, slot;
+	batch_slot(slots > 4, batch, slots, inst, slot);
+	if (inst >= batch)
+		return;
+	const typename GemmArg<T>::type &a = args[inst];
 	const int m = a.m;
 	const int n = (OP == OP_SYRK) ? a.m : a.n;
 	if (m <= 0 || n <= 0)
 		return;
-	if (threadIdx.x == 0 && blockIdx.y == 0)
+	if (threadIdx.x == 0 && slot == 0)
 		a.sD->use_dA = 0;
 	gemm_core<T, PS, OP>(As, Bs, m, n, a.k, a.alpha, a.beta,
 		view<T, PS>(a.sA), a.ai, a.aj, view<T, PS>(a.sB), a.bi, a.bj,
-		view<T, PS>(a.sC), a.ci, a.cj, view<T, PS>(a.sD), a.di, a.dj);
+		view<T, PS>(a.sC), a.ci, a.cj, view<T, PS>(a.sD), a.di, a.dj, slot, slots);
 }
 
 // D = alpha * B * A, A lower n x n (blasfeo_ref/x_blas3_ref.c:3218-3330)
 template <typename T, int PS>
-__global__ __launch_bounds__(NT3) void k_trmm_rlnn(const typename TrsmArg<T>::type *__restrict__ args)
+__global__ __launch_bounds__(NT3) void k_trmm_rlnn(const typename TrsmArg<T>::type *__restrict__ args, int batch, int slots)
 {
 	__shared__ T As[KC * LDT];
 	__shared__ T Bs[KC * LDT];
-	const typename TrsmArg<T>::type &a = args[blockIdx.x];
+	int inst, slot;
+	batch_slot(slots > 4, batch, slots, inst, slot);
+	if (inst >= batch)
+		return;
+	const typename TrsmArg<T>::type &a = args[inst];
 	if (a.m <= 0 || a.n <= 0)
 		return;
-	if (threadIdx.x == 0 && blockIdx.y == 0)
+	if (threadIdx.x == 0 && slot == 0)
 		a.sD->use_dA = 0;
 	const PM<T, PS> gA = view<T, PS>(a.sA);
 	gemm_core<T, PS, OP_TRMM_RLNN>(As, Bs, a.m, a.n, a.n, a.alpha, T(0),
 		gA, a.ai, a.aj, view<T, PS>(a.sB), a.bi, a.bj,
-		gA, 0, 0, view<T, PS>(a.sD), a.di, a.dj);
+		gA, 0, 0, view<T, PS>(a.sD), a.di, a.dj, slot, slots);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,6 +545,17 @@ static int check_common(int ps, int batch, const void *args, bool is_float, cons
 	return BLASFEO_AMD_OK;
 }
 
+// 1-D grid of batch * slots workgroups rounded up to a multiple of XCDS, for
+// kernels that map their workgroup id through xcd_local_index
+static int xcd_grid(int batch, int slots, int threads, dim3 *grid, const char *name)
+{
+	const long long wgs = ((long long)batch * slots + XCDS - 1) / XCDS * XCDS;
+	if (wgs * threads > 0x7fffffffLL)
+		return set_error(BLASFEO_AMD_ETOOBIG, "%s: batch %d too large for one launch", name, batch);
+	*grid = dim3((unsigned)wgs);
+	return BLASFEO_AMD_OK;
+}
+
 template <typename T, int OP>
 static int launch_gemm(int ps, int batch, const typename GemmArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
 {
@@ -549,10 +575,9 @@ static int launch_gemm(int ps, int batch, const typename GemmArg<T>::type *args,
 		int gy = (OP == OP_SYRK || OP == OP_SYRK_MN) ? mt * (mt + 1) / 2 : mt * nt;
 		gy = gy < 1 ? 1 : gy;
 		// one wave per (instance, tile slot), XCD-aware order (k_gemm_wave)
-		const long long waves = ((long long)batch * gy + XCDS - 1) / XCDS * XCDS;
-		if (waves > 0x7fffffffLL / WAVE)
-			return set_error(BLASFEO_AMD_ETOOBIG, "%s: batch %d too large for one launch", name, batch);
-		const dim3 gridw((unsigned)waves);
+		dim3 gridw;
+		if ((rc = xcd_grid(batch, gy, WAVE, &gridw, name)))
+			return rc;
 		if (ps == 4)
 			hipLaunchKernelGGL((k_gemm_wave<T, 4, OP>), gridw, dim3(WAVE), 0, st, args, batch, gy);
 		else if (ps == 8)
@@ -561,13 +586,16 @@ static int launch_gemm(int ps, int batch, const typename GemmArg<T>::type *args,
 			hipLaunchKernelGGL((k_gemm_wave<T, 16, OP>), gridw, dim3(WAVE), 0, st, args, batch, gy);
 		return check_launch(name);
 	}
-	dim3 grid(batch, tiles_y(hint, TM, TN));
+	const int slots = tiles_y(hint, TM, TN);
+	dim3 grid;
+	if ((rc = xcd_grid(batch, slots, NT3, &grid, name)))
+		return rc;
 	if (ps == 4)
-		hipLaunchKernelGGL((k_gemm<T, 4, OP>), grid, dim3(NT3), 0, st, args);
+		hipLaunchKernelGGL((k_gemm<T, 4, OP>), grid, dim3(NT3), 0, st, args, batch, slots);
 	else if (ps == 8)
-		hipLaunchKernelGGL((k_gemm<T, 8, OP>), grid, dim3(NT3), 0, st, args);
+		hipLaunchKernelGGL((k_gemm<T, 8, OP>), grid, dim3(NT3), 0, st, args, batch, slots);
 	else
-		hipLaunchKernelGGL((k_gemm<T, 16, OP>), grid, dim3(NT3), 0, st, args);
+		hipLaunchKernelGGL((k_gemm<T, 16, OP>), grid, dim3(NT3), 0, st, args, batch, slots);
 	return check_launch(name);
 }
 
@@ -577,13 +605,16 @@ static int launch_trmm(int ps, int batch, const typename TrsmArg<T>::type *args,
 	int rc = check_common(ps, batch, args, sizeof(T) == 4, name);
 	if (rc || batch == 0)
 		return rc;
-	dim3 grid(batch, tiles_y(hint, TM, TN));
+	const int slots = tiles_y(hint, TM, TN);
+	dim3 grid;
+	if ((rc = xcd_grid(batch, slots, NT3, &grid, name)))
+		return rc;
 	if (ps == 4)
-		hipLaunchKernelGGL((k_trmm_rlnn<T, 4>), grid, dim3(NT3), 0, st, args);
+		hipLaunchKernelGGL((k_trmm_rlnn<T, 4>), grid, dim3(NT3), 0, st, args, batch, slots);
 	else if (ps == 8)
-		hipLaunchKernelGGL((k_trmm_rlnn<T, 8>), grid, dim3(NT3), 0, st, args);
+		hipLaunchKernelGGL((k_trmm_rlnn<T, 8>), grid, dim3(NT3), 0, st, args, batch, slots);
 	else
-		hipLaunchKernelGGL((k_trmm_rlnn<T, 16>), grid, dim3(NT3), 0, st, args);
+		hipLaunchKernelGGL((k_trmm_rlnn<T, 16>), grid, dim3(NT3), 0, st, args, batch, slots);
 	return check_launch(name);
 }
 

@@ -35,6 +35,27 @@ __device__ __forceinline__ unsigned xcd_local_index(unsigned g, unsigned total)
 	return (g % XCDS) * (total / XCDS) + g / XCDS;
 }
 
+// (instance, slot) of this workgroup in a 1-D launch of batch * slots
+// workgroups (rounded up to a multiple of XCDS): XCD-aware (the slots of an
+// instance consecutive on one XCD) or instance-fastest (slot s of every
+// instance, then slot s + 1); inst >= batch for the rounding workgroups.
+__device__ __forceinline__ void batch_slot(bool xcd, int batch, int slots, int &inst, int &slot)
+{
+	if (xcd)
+	{
+		const unsigned li = xcd_local_index(blockIdx.x, gridDim.x);
+		inst = (int)(li / (unsigned)slots);
+		slot = (int)(li % (unsigned)slots);
+	}
+	else
+	{
+		inst = (int)(blockIdx.x % (unsigned)batch);
+		slot = (int)(blockIdx.x / (unsigned)batch);
+		if (slot >= slots)
+			inst = batch;
+	}
+}
+
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 

@@ -87,13 +87,13 @@ struct TileLoader
 	}
 };
 
-// One gemm-family instance, tiles blockIdx.y, blockIdx.y + gridDim.y, ...
+// One gemm-family instance, tiles t0, t0 + tstep, ...
 // OP_NT/OP_SYRK*: D = beta*C + alpha*A*B^T (B is n x k);  OP_NN: B is k x n;
 // OP_TRMM_RLNN: D = alpha*B*A with A lower k x k (here A is the right operand).
 template <typename T, int PS, int OP>
 __device__ __forceinline__ void gemm_core(T *As, T *Bs, int m, int n, int k, T alpha, T beta,
 	const PM<T, PS> gA, int ai, int aj, const PM<T, PS> gB, int bi, int bj,
-	const PM<T, PS> gC, int ci, int cj, PM<T, PS> gD, int di, int dj)
+	const PM<T, PS> gC, int ci, int cj, PM<T, PS> gD, int di, int dj, int t0, int tstep)
 {
 	typedef typename Vec4<T>::type V4;
 	const int mt = (m + TM - 1) / TM;
@@ -104,7 +104,7 @@ __device__ __forceinline__ void gemm_core(T *As, T *Bs, int m, int n, int k, T a
 	const int wr = (w >> 1) * 32;  // wave's row offset in the tile
 	const int wc = (w & 1) * 32;   // wave's col offset in the tile
 
-	for (int t = blockIdx.y; t < ntiles; t += gridDim.y)
+	for (int t = t0; t < ntiles; t += tstep)
 	{
 		const int ti = t % mt;
 		const int tj = t / mt;

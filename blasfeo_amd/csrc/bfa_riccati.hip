@@ -520,7 +520,7 @@ __device__ void ric_sweep_general(const blasfeo_amd_riccati_arg &a, double *P, i
 			PM<double, PS> gL1 = view<double, PS>(&L[k + 1]);
 			gemm_core<double, PS, OP_TRMM_RLNN>(As, Bs, nk, nx1, nx1, 1.0, 0.0,
 				gL1, nu1, nu1, view<double, PS>(&BAbt[k]), 0, 0,
-				gL1, 0, 0, view<double, PS>(W), 0, 0);
+				gL1, 0, 0, view<double, PS>(W), 0, 0, 0, 1);
 		}
 		__syncthreads();
 		if (nk > 0)

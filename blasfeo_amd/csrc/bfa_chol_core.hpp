@@ -167,7 +167,8 @@ namespace bfa {
 // of a matrix held in LDS and addressed through M(r, c) (column-major, or
 // 16x16 tiles; m <= NMAX <= 128), in place: M <- L (potrf_l_mn semantics:
 // L11 = chol(C11), L21 = C21 L11^{-T}),
-// invd[j] = 1/sqrt(pivot) with the zero-pivot rule above (L_jj = c*invd[j]).
+// invd[j] = 1/sqrt(pivot) with the zero-pivot rule above (L_jj = c*invd[j]);
+// invd must hold 16*ceil(n/16) entries (a full panel stores all 16).
 // Rows m .. ceil16(m)-1 and columns n .. ceil16(n)-1 must hold zeros on entry
 // (with PADCOLS the padding columns hold unspecified values on exit).
 // Per 16-column block: wave 0 factors the whole panel (diagonal block and the
@@ -304,7 +305,7 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, con
 #pragma unroll
 				for (int j = 0; j < 16; j++)
 					x1[j] = M(c0 + 64 + lane, c0 + j);
-			// lane j keeps 1/sqrt(pivot j), stored once after the panel
+			// guarded form: lane j keeps 1/sqrt(pivot j), stored after the panel
 			T myinv = T(0);
 			// PADCOLS: the padding columns of a partial panel are factored too
 			// (unguarded); otherwise a partial panel takes the guarded form.
@@ -317,6 +318,11 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, con
 				// which no real column reads (column jj is updated only from
 				// columns j < jj) and no store returns; invd is stored for
 				// columns < cw only.
+				// Lane j holds the pivot c itself, so scaling every lane gives
+				// it c * inv without a per-column lane select, and every lane
+				// stores the (uniform) reciprocal: per-column lane masks held
+				// 32 SGPRs across the panel, and the compiler spilled the
+				// broadcast values to VGPR lanes and reloaded them.
 				if (!(NMAX > 64 && nr > 64))
 				{
 #pragma unroll
@@ -324,8 +330,8 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, con
 					{
 						const T c = readlane(x0[j], j);
 						const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
-						x0[j] = (lane == j) ? c * inv : x0[j] * inv;
-						myinv = (lane == j) ? inv : myinv;
+						x0[j] *= inv;
+						invd[c0 + j] = inv;
 #pragma unroll
 						for (int jj = j + 1; jj < 16; jj++)
 						{
@@ -342,9 +348,9 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, con
 					{
 						const T c = readlane(x0[j], j);
 						const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
-						x0[j] = (lane == j) ? c * inv : x0[j] * inv;
+						x0[j] *= inv;
 						x1[j] *= inv;
-						myinv = (lane == j) ? inv : myinv;
+						invd[c0 + j] = inv;
 #pragma unroll
 						for (int jj = j + 1; jj < 16; jj++)
 						{
@@ -386,7 +392,7 @@ __device__ __forceinline__ void chol_acc(const ACC M, int m, int n, T *invd, con
 					}
 				}
 			}
-			if (lane < cw)
+			if (!(PADCOLS || cw == 16) && lane < cw)
 				invd[c0 + lane] = myinv;
 #pragma unroll
 			for (int j = 0; j < 16; j++)

@@ -319,15 +319,17 @@ __global__ __launch_bounds__(NTH) void k_potrf_tiles(const typename FactArg<T>::
 					x[j] = lane < 16 ? D[j * 16 + lane] : T(0);
 				// no per-lane guards: the upper triangle of the tile (lanes < j)
 				// is updated too, but only the lower triangle is read (the
-				// row solve, L11 below its diagonal) or written to D
-				T myinv = T(0);
+				// row solve, L11 below its diagonal) or written to D.  Lane j
+				// holds the pivot c, so scaling every lane gives it c * inv;
+				// every lane stores the uniform reciprocal (no per-column lane
+				// masks, which held SGPRs across the tile)
 #pragma unroll
 				for (int j = 0; j < 16; j++)
 				{
 					const T c = readlane(x[j], j);
 					const T inv = (c > T(0)) ? rsqrt_fast(c) : T(0);
-					x[j] = (lane == j) ? c * inv : x[j] * inv;
-					myinv = (lane == j) ? inv : myinv;
+					x[j] *= inv;
+					invs[j] = inv;
 #pragma unroll
 					for (int jn = j + 1; jn < 16; jn++)
 					{
@@ -335,8 +337,6 @@ __global__ __launch_bounds__(NTH) void k_potrf_tiles(const typename FactArg<T>::
 						x[jn] -= x[j] * l;
 					}
 				}
-				if (lane < 16)
-					invs[lane] = myinv;
 				if (lane < 16)
 #pragma unroll
 					for (int j = 0; j < 16; j++)

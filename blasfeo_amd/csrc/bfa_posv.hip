@@ -235,7 +235,7 @@ __device__ __forceinline__ void posv_diag(float *FKK, int *flag)
 		bad = bad || !ok;
 		const double inv = ok ? rsqrt_fast(c) : 0.0;
 		iv[j] = inv;
-		x[j] = (lane == j) ? c * inv : x[j] * inv;
+		x[j] *= inv;   // lane j holds c: c * inv, without a per-column lane mask
 #pragma unroll
 		for (int jj = j + 1; jj < 16; jj++)
 		{

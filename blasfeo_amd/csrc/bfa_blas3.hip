@@ -116,28 +116,45 @@ __global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::ty
 		{
 			const int c0 = cb * 16;
 			const int cv = min(16, n - c0);
-			// diagonal block of A into LDS (zero outside the valid triangle)
+			// diagonal block of A (zero outside the valid triangle): loaded
+			// here, stored to LDS after the update below, so that its load
+			// latency overlaps the update (NT3 = 256 threads, one element each)
 			__syncthreads();
-			for (int e = threadIdx.x; e < 256; e += NT3)
+			T adv;
 			{
-				const int r = e & 15, c = e >> 4;
-				T v = T(0);
-				if (r < cv && c < cv && r > c)
-					v = gA.at(a.ai + c0 + r, a.aj + c0 + c);
-				Ad[r * 17 + c] = v;
+				const int r = threadIdx.x & 15, c = threadIdx.x >> 4;
+				adv = (r < cv && c < cv && r > c) ? gA.at(a.ai + c0 + r, a.aj + c0 + c) : T(0);
 			}
-			// X(:, c0:c0+16) -= X(:, 0:c0) * A(c0:c0+16, 0:c0)^T ; wave w owns rows 16w..16w+15
+			// X(:, c0:c0+16) -= X(:, 0:c0) * A(c0:c0+16, 0:c0)^T ; wave w owns rows 16w..16w+15.
+			// A is read from global memory four K steps (16 columns) at a time
+			// with the next four in flight; c0 is a multiple of 16, so no
+			// step runs past it.  The products are accumulated in the same
+			// order as one step at a time.
 			if (c0 > 0)
 			{
 				V4 acc = V4{0, 0, 0, 0};
 				const int jrow = a.ai + c0 + (lane & 15);
 				const bool jok = (lane & 15) < cv;
-				for (int kk = 0; kk < c0; kk += 4)
+				T bn[4];
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+					bn[u] = jok ? gA.at(jrow, a.aj + 4 * u + (lane >> 4)) : T(0);
+				for (int kk = 0; kk < c0; kk += 16)
 				{
-					const int kq = kk + (lane >> 4);
-					T av = Xs[kq * LDX + 16 * w + (lane & 15)];
-					T bv = jok ? gA.at(jrow, a.aj + kq) : T(0);
-					acc = mfma16(av, bv, acc);
+					T bv[4], av[4];
+#pragma unroll
+					for (int u = 0; u < 4; u++)
+						bv[u] = bn[u];
+					if (kk + 16 < c0)
+#pragma unroll
+						for (int u = 0; u < 4; u++)
+							bn[u] = jok ? gA.at(jrow, a.aj + kk + 16 + 4 * u + (lane >> 4)) : T(0);
+#pragma unroll
+					for (int u = 0; u < 4; u++)
+						av[u] = Xs[(kk + 4 * u + (lane >> 4)) * LDX + 16 * w + (lane & 15)];
+#pragma unroll
+					for (int u = 0; u < 4; u++)
+						acc = mfma16(av[u], bv[u], acc);
 				}
 #pragma unroll
 				for (int r = 0; r < 4; r++)
@@ -147,6 +164,7 @@ __global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::ty
 					Xs[jl * LDX + il] -= acc[r];
 				}
 			}
+			Ad[(threadIdx.x & 15) * 17 + (threadIdx.x >> 4)] = adv;
 			__syncthreads();
 			// triangular solve of the 16 columns, one row per thread of wave 0
 			if (w == 0)

@@ -110,8 +110,9 @@ __global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::ty
 		// X <- alpha * B(row block, :)
 		g2s<T, PS, NT3, false>(gB, a.bi + i0, a.bj, TM, nb * 16, mv, n, Xs, LDX);
 		__syncthreads();
-		for (int e = threadIdx.x; e < nb * 16 * LDX; e += NT3)
-			Xs[e] *= alpha;
+		if (alpha != T(1))   // (x * 1 == x: nothing to do for alpha = 1)
+			for (int e = threadIdx.x; e < nb * 16 * LDX; e += NT3)
+				Xs[e] *= alpha;
 		for (int cb = 0; cb < nb; cb++)
 		{
 			const int c0 = cb * 16;

@@ -71,19 +71,16 @@ __global__ __launch_bounds__(NT3) void k_trmm_rlnn(const typename TrsmArg<T>::ty
 constexpr int TRSM_NMAX = 256;
 constexpr int LDX = TM + 1;
 
-// DA_GIVEN: the reciprocal diagonal is read from sA->dA whatever use_dA says
-// (the trapezoid Cholesky, whose diagonal block was just factored into the
-// same descriptor with its reciprocals stored there).
-template <typename T, int PS, bool DA_GIVEN = false>
-__global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::type *__restrict__ args, int nmax)
+// One instance of trsm_rltn, row blocks t0, t0 + tstep, ... (all threads of
+// the workgroup call it together).  DA_GIVEN: the reciprocal diagonal is
+// read from sA->dA whatever use_dA says (the trapezoid Cholesky, whose
+// diagonal block was just factored into the same descriptor with its
+// reciprocals stored there).
+template <typename T, int PS, bool DA_GIVEN>
+__device__ __forceinline__ void trsm_rltn_instance(const typename TrsmArg<T>::type &a, int nmax, int t0, int tstep,
+	T *Xs, T *inv_s, T *Ad)
 {
 	typedef typename Vec4<T>::type V4;
-	extern __shared__ unsigned char smem_raw[];
-	T *Xs = reinterpret_cast<T *>(smem_raw);      // [npad][LDX] column-major row block
-	__shared__ T inv_s[TRSM_NMAX];
-	__shared__ T Ad[16 * 17];                       // diagonal 16x16 block of A
-
-	const typename TrsmArg<T>::type &a = args[blockIdx.x];
 	const int m = a.m, n = a.n;
 	if (m <= 0 || n <= 0 || n > nmax)
 		return;
@@ -102,7 +99,7 @@ __global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::ty
 
 	const int mt = (m + TM - 1) / TM;
 	const int nb = (n + 15) / 16;
-	for (int t = blockIdx.y; t < mt; t += gridDim.y)
+	for (int t = t0; t < mt; t += tstep)
 	{
 		const int i0 = t * TM;
 		const int mv = min(TM, m - i0);
@@ -194,6 +191,38 @@ __global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::ty
 		}
 		__syncthreads();
 		s2g<T, PS, NT3, false>(gD, a.di + i0, a.dj, mv, n, Xs, LDX, 0);
+	}
+}
+
+// With a queue (count != nullptr) the launch is persistent: each workgroup
+// takes instances from the list (one atomic per instance on *head) and
+// solves all their row blocks; otherwise instance blockIdx.x, row blocks
+// blockIdx.y, blockIdx.y + gridDim.y, ...
+template <typename T, int PS, bool DA_GIVEN = false>
+__global__ __launch_bounds__(NT3) void k_trsm_rltn(const typename TrsmArg<T>::type *__restrict__ args, int nmax,
+	const int *__restrict__ count, const int *__restrict__ list, int *head)
+{
+	extern __shared__ unsigned char smem_raw[];
+	T *Xs = reinterpret_cast<T *>(smem_raw);      // [npad][LDX] column-major row block
+	__shared__ T inv_s[TRSM_NMAX];
+	__shared__ T Ad[16 * 17];                       // diagonal 16x16 block of A
+	__shared__ int qslot;
+	if (!count)
+	{
+		trsm_rltn_instance<T, PS, DA_GIVEN>(args[blockIdx.x], nmax, blockIdx.y, gridDim.y, Xs, inv_s, Ad);
+		return;
+	}
+	const int cnt = *count;
+	for (;;)
+	{
+		__syncthreads();   // every thread has read the previous slot
+		if (threadIdx.x == 0)
+			qslot = atomicAdd(head, 1);
+		__syncthreads();
+		const int t = qslot;
+		if (t >= cnt)
+			break;
+		trsm_rltn_instance<T, PS, DA_GIVEN>(args[list[t]], nmax, 0, 1, Xs, inv_s, Ad);
 	}
 }
 
@@ -637,13 +666,62 @@ static int launch_trmm(int ps, int batch, const typename TrsmArg<T>::type *args,
 	return check_launch(name);
 }
 
-template <typename T, int PS>
-static int launch_trsm_ps(int batch, const typename TrsmArg<T>::type *args, dim3 grid, size_t shm, int nmax, hipStream_t st, const char *name)
+// trsm_rltn's instances sorted by n into buckets of 64 columns, so that each
+// bucket's launch sizes its LDS row block for that bucket: 64 columns take
+// 37 KB (four workgroups per CU), 128 take 71 KB (two), 256 take 137 KB
+// (one).  counts[0..TRSM_NB-1] and the queue heads counts[16..] start at 0.
+constexpr int TRSM_NB = TRSM_NMAX / 64;
+__global__ void k_trsm_zero(int *ws)
 {
-	hipLaunchKernelGGL((k_trsm_rltn<T, PS>), grid, dim3(NT3), shm, st, args, nmax);
+	if (threadIdx.x < 64)
+		ws[threadIdx.x] = 0;
+}
+template <typename T>
+__global__ void k_trsm_plan(const typename TrsmArg<T>::type *__restrict__ args, int batch, int *counts, int *lists)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= batch)
+		return;
+	const int n = args[i].n;
+	const int b = n <= 64 ? 0 : min(TRSM_NB - 1, (n - 1) / 64);
+	const int slot = atomicAdd(&counts[b], 1);
+	lists[(size_t)b * batch + slot] = i;
+}
+
+// the batched entry: plan, then one persistent launch per n bucket, largest
+// first (measured on 16384 calls of m, n ~ U{16..256} against one launch
+// sized for the hint's n)
+template <typename T, int PS>
+static int launch_trsm_ps(int batch, const typename TrsmArg<T>::type *args, int npad, hipStream_t st, const char *name)
+{
+	const size_t wsbytes = sizeof(int) * (64 + (size_t)TRSM_NB * batch);
+	int *ws = (int *)workspace(wsbytes, st);
+	if (!ws)
+		return set_error(BLASFEO_AMD_ENOMEM, "%s: cannot allocate the %zu-byte plan workspace", name, wsbytes);
+	int *counts = ws, *heads = ws + 16, *lists = ws + 64;
+	hipLaunchKernelGGL(k_trsm_zero, dim3(1), dim3(64), 0, st, ws);
+	hipLaunchKernelGGL((k_trsm_plan<T>), dim3((batch + 255) / 256), dim3(256), 0, st, args, batch, counts, lists);
 	int rc = check_launch(name);
 	if (rc)
 		return rc;
+	const int cus = num_cus() > 0 ? num_cus() : 256;
+	constexpr size_t static_lds = sizeof(T) * (TRSM_NMAX + 16 * 17) + sizeof(int);
+	for (int b = TRSM_NB - 1; b >= 0; b--)
+	{
+		if (64 * b >= npad)
+			continue;   // n beyond the hint: no instance runs (as in the unbucketed launch)
+		const int np = min(64 * (b + 1), npad);
+		const size_t shm = (size_t)np * LDX * sizeof(T);
+		// at most four workgroups per CU (registers: four waves per SIMD)
+		const int per_cu = max(1, min(4, (int)(163840 / (shm + static_lds))));
+		const int grid = min(batch, per_cu * cus);
+		if ((rc = lds_check(reinterpret_cast<const void *>(&k_trsm_rltn<T, PS>), shm, name)))
+			return rc;
+		hipLaunchKernelGGL((k_trsm_rltn<T, PS>), dim3(grid), dim3(NT3), shm, st, args, np, counts + b,
+			lists + (size_t)b * batch, heads + b);
+		if ((rc = check_launch(name)))
+			return rc;
+	}
 	hipLaunchKernelGGL((k_trsm_dA<T, PS, false>), dim3(batch), dim3(64), 0, st, args);
 	return check_launch(name);
 }
@@ -653,7 +731,8 @@ static int launch_trsm_ps(int batch, const typename TrsmArg<T>::type *args, dim3
 template <int PS>
 static int launch_trsm_given(int batch, const blasfeo_amd_dtrsm_arg *args, dim3 grid, size_t shm, int nmax, hipStream_t st, const char *name)
 {
-	hipLaunchKernelGGL((k_trsm_rltn<double, PS, true>), grid, dim3(NT3), shm, st, args, nmax);
+	hipLaunchKernelGGL((k_trsm_rltn<double, PS, true>), grid, dim3(NT3), shm, st, args, nmax, (const int *)nullptr,
+		(const int *)nullptr, (int *)nullptr);
 	return check_launch(name);
 }
 
@@ -680,10 +759,10 @@ static int launch_trsm(int ps, int batch, const typename TrsmArg<T>::type *args,
 		               : launch_trsm_given<8>(batch, d, grid, shm, npad, st, name);
 	}
 	if (ps == 4)
-		return launch_trsm_ps<T, 4>(batch, args, grid, shm, npad, st, name);
+		return launch_trsm_ps<T, 4>(batch, args, npad, st, name);
 	if (ps == 8)
-		return launch_trsm_ps<T, 8>(batch, args, grid, shm, npad, st, name);
-	return launch_trsm_ps<T, 16>(batch, args, grid, shm, npad, st, name);
+		return launch_trsm_ps<T, 8>(batch, args, npad, st, name);
+	return launch_trsm_ps<T, 16>(batch, args, npad, st, name);
 }
 
 template <typename T, int PS, int MODE>

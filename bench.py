@@ -935,8 +935,9 @@ def run_trsm_rltn(args, torch, bfa):
            "config": {"workload": "dtrsm_rltn_batched", "batch": batch, "m,n": "U{16..256}", "ps": ps,
                       "precision": "fp64"},
            "roofline": {"bound": bound, "achieved": achieved, "peak": peak, "unit": punit,
-                        "frac": achieved / peak, "traffic": None, "algorithmic_bytes": algo,
-                        "flops_per_launch": flops, "kernel": "k_trsm_rltn<double,%d>" % ps},
+                        "frac": achieved / peak, "traffic": read_traffic("k_trsm_rltn", "trsm-16384"),
+                        "algorithmic_bytes": algo, "flops_per_launch": flops,
+                        "kernel": "k_trsm_rltn<double,%d> (one persistent launch per n bucket)" % ps},
            "cpu_baseline": None, "checks": checks}
     print(json.dumps(out), flush=True)
 

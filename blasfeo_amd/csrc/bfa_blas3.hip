@@ -504,11 +504,12 @@ __global__ __launch_bounds__(WAVE) void k_trsm_left_thin(const typename TrsmArg<
 				const int i = lane + q * WAVE;
 				// only the rows still to be solved take the update (a solved
 				// row is never touched again, so an Inf or NaN in xj cannot
-				// reach it through a zero coefficient)
+				// reach it through a zero coefficient); row j itself keeps its
+				// unscaled value, scaled by its reciprocal after the sweep
 				const bool open = BACK ? i < j : i > j;
 #pragma unroll
 				for (int c = 0; c < THIN_RHS; c++)
-					x[q][c] = (i == j) ? xj[c] : (open ? x[q][c] - e[q] * xj[c] : x[q][c]);
+					x[q][c] = open ? x[q][c] - e[q] * xj[c] : x[q][c];
 			}
 			if (more)
 			{
@@ -519,15 +520,21 @@ __global__ __launch_bounds__(WAVE) void k_trsm_left_thin(const typename TrsmArg<
 			}
 		}
 	}
+	// row i's solved value is its unscaled value times the reciprocal the
+	// sweep used for column i (the same product the broadcast formed)
 #pragma unroll
 	for (int q = 0; q < THIN_Q; q++)
-#pragma unroll
-		for (int c = 0; c < THIN_RHS; c++)
+	{
+		const int i = lane + q * WAVE;
+		if (i < m)
 		{
-			const int i = lane + q * WAVE;
-			if (i < m && c < n)
-				gD.at(a.di + i, a.dj + c0 + c) = x[q][c];
+			const T di = recip(i);
+#pragma unroll
+			for (int c = 0; c < THIN_RHS; c++)
+				if (c < n)
+					gD.at(a.di + i, a.dj + c0 + c) = UNIT ? x[q][c] : x[q][c] * di;
 		}
+	}
 	}
 }
 

@@ -150,7 +150,7 @@ def main():
                          "in a HIP graph (the default; every replay runs the whole call: plan, "
                          "queue reset and class launches)")
     ap.add_argument("--workload", default="riccati",
-                    choices=("riccati", "ric_trs", "ric_sv", "ric_mixed", "pack", "getrs", "trsm_rltn", "potrf32", "gemm_syrk", "getrf", "sgetrf", "getrf_np",
+                    choices=("riccati", "ric_trs", "ric_sv", "ric_mixed", "pack", "getrs", "trsm_rltn", "gemm_nn", "trmm_rlnn", "syrk_potrf", "potrf32", "gemm_syrk", "getrf", "sgetrf", "getrf_np",
                              "posv256"),
                     help="riccati (default, the headline); the others measure the remaining "
                          "batched configurations")
@@ -194,6 +194,8 @@ def main():
         return run_getrs(args, torch, bfa)
     if args.workload == "trsm_rltn":
         return run_trsm_rltn(args, torch, bfa)
+    if args.workload in ("gemm_nn", "trmm_rlnn", "syrk_potrf"):
+        return run_blas3_extra(args, torch, bfa)
     if args.workload != "riccati":
         return run_other(args, torch, bfa)
     return run_riccati(args, torch, bfa)
@@ -938,6 +940,162 @@ def run_trsm_rltn(args, torch, bfa):
                         "frac": achieved / peak, "traffic": read_traffic("k_trsm_rltn", "trsm-16384"),
                         "algorithmic_bytes": algo, "flops_per_launch": flops,
                         "kernel": "k_trsm_rltn<double,%d> (one persistent launch per n bucket)" % ps},
+           "cpu_baseline": None, "checks": checks}
+    print(json.dumps(out), flush=True)
+
+
+def _diag_index(hd, base, nn, ps):
+    """Element indexes (in doubles from `base`) of the diagonal of every block of a bulk
+    layout: row t of a panel-major block sits at (t - t%ps)*cn + t*ps + t%ps."""
+    di = []
+    for i in range(len(nn)):
+        t = np.arange(int(nn[i]))
+        di.append((int(hd["pA"][i]) - base) // 8 + (t - t % ps) * int(hd["cn"][i]) + t * ps + t % ps)
+    return np.concatenate(di)
+
+
+def _block(host, hd, base, i):
+    """The flat panel-major block of instance i from a host copy of a bulk buffer."""
+    o = (int(hd["pA"][i]) - base) // 8
+    return host[o:o + int(hd["pm"][i]) * int(hd["cn"][i])]
+
+
+def run_blas3_extra(args, torch, bfa):
+    """The batched BLAS-3 calls without a workload of their own elsewhere: dgemm_nn (the
+    reference's blasfeo_dgemm_nn), dtrmm_rlnn (blasfeo_dtrmm_rlnn) and dsyrk_dpotrf_ln
+    (blasfeo_dsyrk_dpotrf_ln), each as one batched call per step on separate inputs and
+    outputs, so that every step does the same work.  The first 64 instances of the last call
+    are checked against the oracle on the same blocks (relative difference of the entries
+    that differ by more than 1e-11)."""
+    from blasfeo_amd.device import to_device
+    import oracle
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(args.seed)
+    rng = np.random.default_rng(args.seed)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    ps = args.ps
+    wl = args.workload
+    if wl == "gemm_nn":
+        # 100000 x D = C + A B, A m x k, B k x n, m, n, k ~ U{4..128}, N(0,1)
+        batch = 100000
+        m = rng.integers(4, 129, batch); n = rng.integers(4, 129, batch); k = rng.integers(4, 129, batch)
+        shapes = {"A": (m, k), "B": (k, n), "C": (m, n), "D": (m, n)}
+        fills = {"A": "normal", "B": "normal", "C": "normal", "D": None}
+    elif wl == "trmm_rlnn":
+        # 16384 x D = B A, A lower n x n U[-1,1], B m x n N(0,1), m, n ~ U{16..256}
+        batch = 16384
+        n = rng.integers(16, 257, batch); m = rng.integers(16, 257, batch)
+        shapes = {"A": (n, n), "B": (m, n), "D": (m, n)}
+        fills = {"A": "uniform", "B": "normal", "D": None}
+    else:
+        # 65536 x D = chol(C + A A^T), A m x k N(0,1), C = (m + k) I, m, k ~ U{8..128}
+        batch = 65536
+        m = rng.integers(8, 129, batch); k = rng.integers(8, 129, batch)
+        shapes = {"A": (m, k), "C": (m, m), "D": (m, m)}
+        fills = {"A": "normal", "C": "zero", "D": None}
+    bufs, descs, hds = {}, {}, {}
+    for name, (r, c) in shapes.items():
+        nb = bfa.bulk_layout(np.float64, ps, r, c, 0)[2]
+        buf = torch.empty(max(nb, 8), dtype=torch.uint8, device="cuda")
+        if fills[name] == "zero" or fills[name] is None:
+            buf.zero_()
+        else:
+            _fill(torch, buf, fills[name], gen)
+        hd = bfa.bulk_layout(np.float64, ps, r, c, buf.data_ptr())[0]
+        bufs[name], descs[name], hds[name] = buf, to_device(hd), hd
+    idx = np.arange(batch, dtype=np.uint64) * bfa.DMAT.itemsize
+    sp = {name: descs[name].data_ptr() + idx for name in shapes}
+    mf = m.astype(np.float64)
+    if wl == "gemm_nn":
+        rec = _records(bfa.DGEMM_ARG, batch)
+        rec["m"], rec["n"], rec["k"] = m, n, k
+        rec["alpha"], rec["beta"] = 1.0, 1.0
+        rec["sA"], rec["sB"], rec["sC"], rec["sD"] = sp["A"], sp["B"], sp["C"], sp["D"]
+        call, hint = "dgemm_nn", (128, 128, 128)
+        nf, kf = n.astype(np.float64), k.astype(np.float64)
+        flops = float((2 * mf * nf * kf).sum())
+        algo = float((8 * (mf * kf + kf * nf + 2 * mf * nf)).sum())
+        unit = "ops/s (dgemm_nn calls)"
+        kernel = "k_gemm_wave<double,%d,OP_NN>" % ps
+        cfg = {"workload": "dgemm_nn_batched", "batch": batch, "m,n,k": "U{4..128}"}
+    elif wl == "trmm_rlnn":
+        rec = _records(bfa.DTRSM_ARG, batch)
+        rec["m"], rec["n"], rec["alpha"] = m, n, 1.0
+        rec["sA"], rec["sB"], rec["sD"] = sp["A"], sp["B"], sp["D"]
+        call, hint = "dtrmm_rlnn", (256, 256, 0)
+        nf = n.astype(np.float64)
+        flops = float((mf * nf * (nf + 1)).sum())
+        algo = float((8 * (nf * (nf + 1) / 2 + 2 * mf * nf)).sum())
+        unit = "ops/s (dtrmm_rlnn calls)"
+        kernel = "k_trmm_rlnn<double,%d>" % ps
+        cfg = {"workload": "dtrmm_rlnn_batched", "batch": batch, "m,n": "U{16..256}"}
+    else:
+        base = bufs["C"].data_ptr()
+        di = torch.from_numpy(_diag_index(hds["C"], base, m, ps)).to("cuda")
+        bufs["C"].view(torch.float64)[di] = torch.from_numpy(
+            np.repeat((m + k).astype(np.float64), m)).to("cuda")
+        rec = _records(bfa.DSYRK_DPOTRF_ARG, batch)
+        rec["m"], rec["n"], rec["k"] = m, m, k
+        rec["sA"], rec["sB"], rec["sC"], rec["sD"] = sp["A"], sp["A"], sp["C"], sp["D"]
+        call, hint = "dsyrk_dpotrf_ln", (128, 128, 128)
+        kf = k.astype(np.float64)
+        flops = float((mf * (mf + 1) * kf + mf ** 3 / 3).sum())
+        algo = float((8 * (mf * kf + mf * (mf + 1) + mf)).sum())
+        unit = "factorizations/s (dsyrk_dpotrf_ln calls)"
+        kernel = "dsyrk_dpotrf_ln kernels"
+        cfg = {"workload": "dsyrk_dpotrf_ln_batched", "batch": batch, "m,k": "U{8..128}"}
+    drec = to_device(rec)
+    fn = lambda: bfa.call(call, ps, batch, drec.data_ptr(), hint, sh)
+    ms = time_steps(torch, stream, fn, args.steps, args.warmup)
+    value = batch / (ms * 1e-3)
+    bound, achieved, peak, punit = roof(flops, algo, ms, FP64_PEAK_TFLOPS)
+    # the first 64 instances against the oracle on copies of the same blocks
+    torch.cuda.synchronize()
+    host = {name: bufs[name][:int(hds[name]["pA"][64]) - bufs[name].data_ptr()].cpu().numpy().view(np.float64)
+            for name in bufs}                     # the blocks of the first 64 instances
+    names = list(shapes)
+    worst = 0.0
+    for i in range(64):
+        lay = bfa.Layout(np.float64, ps)
+        for name in names:
+            lay.add(int(shapes[name][0][i]), int(shapes[name][1][i]))
+        hb = lay.host_buffer()
+        for s, name in enumerate(names):
+            if name != "D":
+                blk = _block(host[name], hds[name], bufs[name].data_ptr(), i)
+                lay.view(hb, s)[:len(blk)] = blk
+        desc = lay.build(hb.ctypes.data)
+        a = {name: desc.ctypes.data + s * bfa.DMAT.itemsize for s, name in enumerate(names)}
+        mi = int(m[i])
+        if wl == "gemm_nn":
+            oracle.fn("d", "gemm_nn")(ps, mi, int(n[i]), int(k[i]), 1.0, a["A"], 0, 0, a["B"], 0, 0,
+                                      1.0, a["C"], 0, 0, a["D"], 0, 0)
+        elif wl == "trmm_rlnn":
+            oracle.fn("d", "trmm_rlnn")(ps, mi, int(n[i]), 1.0, a["A"], 0, 0, a["B"], 0, 0,
+                                        a["D"], 0, 0)
+        else:
+            oracle.fn("d", "syrk_potrf_ln")(ps, mi, int(k[i]), a["A"], 0, 0, a["A"], 0, 0,
+                                            a["C"], 0, 0, a["D"], 0, 0)
+        r = lay.get(hb, names.index("D"))
+        cn = int(hds["D"]["cn"][i])
+        g = bfa.pm_unpack(_block(host["D"], hds["D"], bufs["D"].data_ptr(), i), ps, cn, r.shape[0], r.shape[1])
+        if wl == "syrk_potrf":
+            g, r = np.tril(g), np.tril(r)
+        dd = np.abs(g - r)
+        rel = np.where(dd > 1e-11, dd / np.maximum(np.abs(g) + np.abs(r), 1e-300), 0.0)
+        worst = max(worst, float(rel.max()))
+    checks = {"instances": 64, "max_rel_diff_vs_oracle": worst, "pass": worst <= 1e-11}
+    log("checks: %s" % json.dumps(checks))
+    log("%s: %.3f ms/step, %.4g %s, %.2f %s" % (wl, ms, value, unit, achieved, punit))
+    cfg.update({"ps": ps, "precision": "fp64"})
+    out = {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic, seeded (%d)" % args.seed,
+           "config": cfg,
+           "roofline": {"bound": bound, "achieved": achieved, "peak": peak, "unit": punit,
+                        "frac": achieved / peak, "traffic": None, "algorithmic_bytes": algo,
+                        "flops_per_launch": flops, "kernel": kernel},
            "cpu_baseline": None, "checks": checks}
     print(json.dumps(out), flush=True)
 

@@ -62,6 +62,9 @@ def reset():
     set_knob("gemm_grid_y", 0, 0)
     set_knob("potrf_bw", 0, 0)
     set_knob("trsm_thin", 0, 0)
+    set_knob("gemm_wave", 0, 0)
+    for b in range(8):
+        set_knob("lds_threads", b, 0)
 
 
 def apply(tuning):

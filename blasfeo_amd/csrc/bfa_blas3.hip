@@ -687,12 +687,8 @@ template <typename T>
 __global__ void k_trsm_plan(const typename TrsmArg<T>::type *__restrict__ args, int batch, int *counts, int *lists)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= batch)
-		return;
-	const int n = args[i].n;
-	const int b = n <= 64 ? 0 : min(TRSM_NB - 1, (n - 1) / 64);
-	const int slot = atomicAdd(&counts[b], 1);
-	lists[(size_t)b * batch + slot] = i;
+	const int n = i < batch ? args[i].n : 0;
+	plan_bucket<TRSM_NB>(i, i < batch ? (n <= 64 ? 0 : min(TRSM_NB - 1, (n - 1) / 64)) : -1, batch, counts, lists);
 }
 
 // the batched entry: plan, then one persistent launch per n bucket, largest

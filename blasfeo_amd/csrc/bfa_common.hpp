@@ -56,6 +56,29 @@ __device__ __forceinline__ void batch_slot(bool xcd, int batch, int slots, int &
 	}
 }
 
+// Size plans: instance i joins bucket b (b < 0: none), lists[b * batch + k]
+// = i for k < counts[b].  The workgroup first counts its own instances per
+// bucket with LDS atomics, reserves one range per bucket with a single global
+// atomic, then writes its entries, so the global counters see one atomic per
+// workgroup and bucket rather than one per instance (per-instance atomics on a
+// few counters serialise: tens of microseconds for some ten thousand
+// instances).  Every thread of the workgroup calls it.
+template <int NB>
+__device__ __forceinline__ void plan_bucket(int i, int b, int batch, int *counts, int *lists)
+{
+	__shared__ int local[NB], base[NB];
+	if (threadIdx.x < NB)
+		local[threadIdx.x] = 0;
+	__syncthreads();
+	const int k = b >= 0 ? atomicAdd(&local[b], 1) : 0;
+	__syncthreads();
+	if (threadIdx.x < NB)
+		base[threadIdx.x] = local[threadIdx.x] > 0 ? atomicAdd(&counts[threadIdx.x], local[threadIdx.x]) : 0;
+	__syncthreads();
+	if (b >= 0)
+		lists[(size_t)b * batch + base[b] + k] = i;
+}
+
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 

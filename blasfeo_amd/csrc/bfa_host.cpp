@@ -100,19 +100,19 @@ struct WsBlock
 	size_t bytes = 0;
 };
 static std::mutex g_ws_mutex;
-static std::map<std::pair<int, hipStream_t>, WsBlock> g_ws;
+static std::map<std::pair<int, hipStream_t>, WsBlock> g_ws, g_plan_ws;
 // Blocks that a larger request replaced.  They are never freed: a HIP graph
 // captured while one was current keeps its address in its kernel arguments,
 // and replaying that graph must still find the memory there.
 static std::vector<void *> g_ws_retired;
 
-void *workspace(size_t bytes, hipStream_t stream)
+static void *ws_block(std::map<std::pair<int, hipStream_t>, WsBlock> &blocks, size_t bytes, hipStream_t stream)
 {
 	int dev = 0;
 	if (hipGetDevice(&dev) != hipSuccess)
 		return nullptr;
 	std::lock_guard<std::mutex> lock(g_ws_mutex);
-	WsBlock &b = g_ws[std::make_pair(dev, stream)];
+	WsBlock &b = blocks[std::make_pair(dev, stream)];
 	if (b.bytes >= bytes)
 		return b.ptr;
 	size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
@@ -125,6 +125,9 @@ void *workspace(size_t bytes, hipStream_t stream)
 	b.bytes = want;
 	return b.ptr;
 }
+
+void *workspace(size_t bytes, hipStream_t stream) { return ws_block(g_ws, bytes, stream); }
+void *plan_workspace(size_t bytes, hipStream_t stream) { return ws_block(g_plan_ws, bytes, stream); }
 
 // panel leading dimension (column rounding) of the reference builds:
 // double lib4 -> 4, double lib8 -> 8 (include/blasfeo_block_size.h:54,100),

@@ -54,13 +54,9 @@ __global__ __launch_bounds__(NTL) void k_potrf(const typename FactArg<T>::type *
 // stream L and dA out.  NT = 64 (one wave) for m <= 32 so that several
 // instances share a CU.
 template <typename T, int PS, int NT, bool MN>
-__global__ __launch_bounds__(NT) void k_potrf_lds(const typename FactArg<T>::type *__restrict__ args, int np)
+__device__ __forceinline__ void potrf_lds_instance(const typename FactArg<T>::type &a, int np, T *M, T *invd)
 {
-	extern __shared__ unsigned char smem_raw[];
 	const int LDM = np + 1;
-	T *M = reinterpret_cast<T *>(smem_raw);
-	T *invd = M + (size_t)np * LDM;
-	const typename FactArg<T>::type &a = args[blockIdx.x];
 	const int m = a.m;
 	int n = MN ? a.n : a.m;
 	if (m <= 0 || n <= 0 || m > np)
@@ -79,21 +75,54 @@ __global__ __launch_bounds__(NT) void k_potrf_lds(const typename FactArg<T>::typ
 		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? (MN ? 1 : m) : 0;
 }
 
+// The LDS-resident factorisations run either one instance per workgroup
+// (k_potrf_lds, k_syrk_potrf_lds: workgroup b takes instance b) or as a
+// persistent launch over one size bucket (the _q forms), each workgroup
+// taking the next instance of the bucket's list from its atomic queue head.
+// Two kernels rather than one with both modes: the loop costs registers that
+// the one-instance form (several instances per CU at 32 rows) cannot spare.  The leading barrier makes every
+// thread finish the previous instance (and its read of the slot) before the
+// slot is overwritten and the LDS staged again.  Returns -1 once the bucket
+// is drained.
+__device__ __forceinline__ int queue_next(const int *count, const int *list, int *head, int *qslot)
+{
+	__syncthreads();
+	if (threadIdx.x == 0)
+		*qslot = atomicAdd(head, 1);
+	__syncthreads();
+	const int t = *qslot;
+	return t < *count ? list[t] : -1;
+}
+
+template <typename T, int PS, int NT, bool MN>
+__global__ __launch_bounds__(NT) void k_potrf_lds(const typename FactArg<T>::type *__restrict__ args, int np)
+{
+	extern __shared__ unsigned char smem_raw[];
+	T *M = reinterpret_cast<T *>(smem_raw);
+	potrf_lds_instance<T, PS, NT, MN>(args[blockIdx.x], np, M, M + (size_t)np * (np + 1));
+}
+template <typename T, int PS, int NT, bool MN>
+__global__ __launch_bounds__(NT) void k_potrf_lds_q(const typename FactArg<T>::type *__restrict__ args, int np,
+	const int *__restrict__ count, const int *__restrict__ list, int *head)
+{
+	extern __shared__ unsigned char smem_raw[];
+	__shared__ int qslot;
+	T *M = reinterpret_cast<T *>(smem_raw);
+	for (int i; (i = queue_next(count, list, head, &qslot)) >= 0;)
+		potrf_lds_instance<T, PS, NT, MN>(args[i], np, M, M + (size_t)np * (np + 1));
+}
+
 // syrk_dpotrf_ln(_mn), m <= 128: D = chol(C + A B^T) on the m x n lower
 // trapezoid (blasfeo_hp_pm/d_lapack_lib4.c:1848-2077).  C's trapezoid is staged
 // in LDS, A B^T is added tile by tile on the matrix cores with A and B read
 // from global memory (a group of up to four row tiles shares B's operand), and
 // the sum is factored in place (chol_acc).
 template <typename T, int PS, int NT>
-__global__ __launch_bounds__(NT) void k_syrk_potrf_lds(const typename SPArg<T>::type *__restrict__ args, int np)
+__device__ __forceinline__ void syrk_potrf_lds_instance(const typename SPArg<T>::type &a, int np, T *M, T *invd)
 {
 	typedef typename Vec4<T>::type V4;
 	constexpr int NW = NT / WAVE;
-	extern __shared__ unsigned char smem_raw[];
 	const int LDM = np + 1;
-	T *M = reinterpret_cast<T *>(smem_raw);
-	T *invd = M + (size_t)np * LDM;
-	const typename SPArg<T>::type &a = args[blockIdx.x];
 	const int m = a.m, k = a.k;
 	int n = a.n;
 	if (m <= 0 || n <= 0 || m > np || k < 0)
@@ -125,22 +154,43 @@ __global__ __launch_bounds__(NT) void k_syrk_potrf_lds(const typename SPArg<T>::
 			for (int t = 0; t < 4; t++)
 				acc[t] = V4{0, 0, 0, 0};
 			const int jr = bj * 16 + (lane & 15);     // row of B for this lane's operand
-#pragma unroll 1
-			for (int kk = 0; kk < k; kk += 4)
-			{
-				const int kq = kk + (lane >> 4);
-				const bool kok = kq < k;
-				const T bv = (kok && jr < n) ? gB.at(a.bi + jr, a.bj + kq) : T(0);
+			// this lane's rows of A (one per tile) and of B at column 0 of k; a
+			// panel-major row advances by PS elements per column
+			const T *rb = jr < n ? &gB.at(a.bi + jr, a.bj) : nullptr;
+			const T *ra[4];
 #pragma unroll
-				for (int t = 0; t < 4; t++)
+			for (int t = 0; t < 4; t++)
+			{
+				const int ir = (g + t) * 16 + (lane & 15);
+				ra[t] = (g + t < mt && ir < m) ? &gA.at(a.ai + ir, a.aj) : nullptr;
+			}
+			// KU matrix-core steps per round, all their global loads issued
+			// before the first step: one memory latency per 4 KU columns of k
+			// rather than per 4 (4 KU = 8 at four waves keeps the registers of
+			// the four-wave form; the eight-wave form has twice the registers
+			// per lane to spare and takes 16)
+			constexpr int KU = NT > NTL ? 4 : 2;
+#pragma unroll 1
+			for (int kk = 0; kk < k; kk += 4 * KU)
+			{
+				T bv[KU], av[4][KU];
+#pragma unroll
+				for (int u = 0; u < KU; u++)
 				{
-					if (g + t < mt)
-					{
-						const int ir = (g + t) * 16 + (lane & 15);
-						const T av = (kok && ir < m) ? gA.at(a.ai + ir, a.aj + kq) : T(0);
-						acc[t] = mfma16(av, bv, acc[t]);
-					}
+					const int kq = kk + 4 * u + (lane >> 4);
+					const bool kok = kq < k;
+					bv[u] = (kok && rb) ? rb[(size_t)kq * PS] : T(0);
+#pragma unroll
+					for (int t = 0; t < 4; t++)
+						av[t][u] = (kok && ra[t]) ? ra[t][(size_t)kq * PS] : T(0);
 				}
+#pragma unroll
+				for (int u = 0; u < KU; u++)
+					if (kk + 4 * u < k)
+#pragma unroll
+						for (int t = 0; t < 4; t++)
+							if (g + t < mt)
+								acc[t] = mfma16(av[t][u], bv[u], acc[t]);
 			}
 #pragma unroll
 			for (int t = 0; t < 4; t++)
@@ -161,6 +211,24 @@ __global__ __launch_bounds__(NT) void k_syrk_potrf_lds(const typename SPArg<T>::
 		dD[j] = invd[j];
 	if (threadIdx.x == 0)
 		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? 1 : 0;
+}
+
+template <typename T, int PS, int NT>
+__global__ __launch_bounds__(NT) void k_syrk_potrf_lds(const typename SPArg<T>::type *__restrict__ args, int np)
+{
+	extern __shared__ unsigned char smem_raw[];
+	T *M = reinterpret_cast<T *>(smem_raw);
+	syrk_potrf_lds_instance<T, PS, NT>(args[blockIdx.x], np, M, M + (size_t)np * (np + 1));
+}
+template <typename T, int PS, int NT>
+__global__ __launch_bounds__(NT) void k_syrk_potrf_lds_q(const typename SPArg<T>::type *__restrict__ args, int np,
+	const int *__restrict__ count, const int *__restrict__ list, int *head)
+{
+	extern __shared__ unsigned char smem_raw[];
+	__shared__ int qslot;
+	T *M = reinterpret_cast<T *>(smem_raw);
+	for (int i; (i = queue_next(count, list, head, &qslot)) >= 0;)
+		syrk_potrf_lds_instance<T, PS, NT>(args[i], np, M, M + (size_t)np * (np + 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -440,12 +508,13 @@ template <typename T>
 __global__ void k_getrf_plan(const typename FactArg<T>::type *__restrict__ args, int batch, int *counts, int *lists)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= batch)
-		return;
-	const int s = max(args[i].m, args[i].n);
-	const int b = s <= 0 ? GETRF_NB - 1 : max(0, GETRF_NB - 1 - (s - 1) / 16);
-	const int slot = atomicAdd(&counts[b], 1);
-	lists[(size_t)b * batch + slot] = i;
+	int b = -1;
+	if (i < batch)
+	{
+		const int s = max(args[i].m, args[i].n);
+		b = s <= 0 ? GETRF_NB - 1 : max(0, GETRF_NB - 1 - (s - 1) / 16);
+	}
+	plan_bucket<GETRF_NB>(i, b, batch, counts, lists);
 }
 
 __global__ void k_zero_ints(int *p, int n)
@@ -912,6 +981,72 @@ static int launch_potrf_mn_split(int ps, int batch, const blasfeo_amd_dfact_arg 
 	return check_launch(name);
 }
 
+// The m <= 128 LDS-resident launches hold the whole trapezoid in LDS, sized
+// for np rows: np (np + 1) + np elements, 133 KB at np = 128, one workgroup
+// per CU.  With a bound above 32 rows a plan kernel sorts the instances into
+// buckets of 16 rows (instance i to bucket ceil(m/16) - 1) and each bucket
+// gets a persistent launch whose LDS is sized for that bucket, largest
+// first, so that an instance is charged its own footprint rather than the
+// bound's; up to 32 rows one workgroup per instance (the footprint is at
+// most 8.7 KB).  counts[0..LDS_NB-1] and the queue heads counts[16..] start
+// at 0.
+constexpr int LDS_NB = 8;
+template <typename R>
+__global__ void k_lds_plan(const R *__restrict__ args, int batch, int *counts, int *lists)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	plan_bucket<LDS_NB>(i, i < batch ? min(LDS_NB - 1, max(0, (args[i].m - 1) / 16)) : -1, batch, counts, lists);
+}
+
+// Eight waves rather than four for a bucket's queued launch: the update tasks
+// of a panel step and of the syrk are shared by twice the waves.  Tuning knob
+// lds_threads[bucket] = 512 or 256; by default eight waves from bucket
+// wide_from on (dsyrk_dpotrf_ln: rows 97 and up, where the LDS holds one
+// workgroup per CU; 65536 instances of m, k ~ U{8..128} on MI355X: 10.17 ms,
+// against 10.72 ms at four waves throughout and 10.82 ms with eight waves
+// from 81 rows, whose 16-column syrk rounds leave one workgroup per CU where
+// the LDS holds two).
+static bool lds_wide(int np, const int *queued, int wide_from)
+{
+	if (!queued || np <= 32)
+		return false;
+	const int b = (np - 1) / 16, knob = tune_value(TUNE_LDS_THREADS, b);
+	return knob ? knob == 512 : b >= wide_from;
+}
+
+// go(np, grid, shm, count, list, head) launches the kernel form for np rows
+// (the one-wave form for np <= 32)
+template <typename R, typename Go>
+static int launch_lds_buckets(const R *args, int batch, int npad, size_t elem, hipStream_t st, const char *name, Go go)
+{
+	auto shm_for = [elem](int np) { return elem * ((size_t)np * (np + 1) + np); };
+	if (npad <= 32)
+		return go(npad, batch, shm_for(npad), (const int *)nullptr, (const int *)nullptr, (int *)nullptr);
+	const size_t wsbytes = sizeof(int) * (64 + (size_t)LDS_NB * batch);
+	int *ws = (int *)plan_workspace(wsbytes, st);
+	if (!ws)
+		return set_error(BLASFEO_AMD_ENOMEM, "%s: cannot allocate the %zu-byte plan workspace", name, wsbytes);
+	int *counts = ws, *heads = ws + 16, *lists = ws + 64;
+	hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, ws, 64);
+	hipLaunchKernelGGL((k_lds_plan<R>), dim3((batch + 255) / 256), dim3(256), 0, st, args, batch, counts, lists);
+	int rc = check_launch(name);
+	if (rc)
+		return rc;
+	const int cus = num_cus() > 0 ? num_cus() : 256;
+	for (int b = LDS_NB - 1; b >= 0; b--)
+	{
+		if (16 * b >= npad)
+			continue;   // m beyond the bound: no instance runs (as in the one-launch form)
+		const int np = min(16 * (b + 1), npad);
+		const size_t shm = shm_for(np);
+		// as many workgroups as the LDS holds, at most 16 one-wave or 8 four-wave ones per CU
+		const int per_cu = max(1, min(np <= 32 ? 16 : 8, (int)(163840 / shm)));
+		if ((rc = go(np, min(batch, per_cu * cus), shm, counts + b, lists + (size_t)b * batch, heads + b)))
+			return rc;
+	}
+	return 0;
+}
+
 template <typename T, int MODE>
 static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args, const blasfeo_amd_hint *hint, hipStream_t st, const char *name)
 {
@@ -923,27 +1058,46 @@ static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args
 		return rc;
 	if (mmax <= 128)
 	{
-		const int np = (mmax + 15) / 16 * 16;
-		const size_t shm2 = sizeof(T) * ((size_t)np * (np + 1) + np);
-		if (np <= 32)
-		{
-			if (ps == 4)
-				hipLaunchKernelGGL((k_potrf_lds<T, 4, 64, MODE == 1>), dim3(batch), dim3(64), shm2, st, args, np);
-			else if (ps == 8)
-				hipLaunchKernelGGL((k_potrf_lds<T, 8, 64, MODE == 1>), dim3(batch), dim3(64), shm2, st, args, np);
+		auto go = [&](int np, int grid, size_t shm, const int *cnt, const int *lst, int *hd) {
+#define BFA_POTRF_LDS(PS_, NT_)                                                                            \
+	do                                                                                                     \
+	{                                                                                                      \
+		if (cnt)                                                                                           \
+			hipLaunchKernelGGL((k_potrf_lds_q<T, PS_, NT_, MODE == 1>), dim3(grid), dim3(NT_), shm, st, args, np, cnt, lst, hd); \
+		else                                                                                               \
+			hipLaunchKernelGGL((k_potrf_lds<T, PS_, NT_, MODE == 1>), dim3(grid), dim3(NT_), shm, st, args, np); \
+	} while (0)
+			if (np <= 32)
+			{
+				if (ps == 4)
+					BFA_POTRF_LDS(4, 64);
+				else if (ps == 8)
+					BFA_POTRF_LDS(8, 64);
+				else
+					BFA_POTRF_LDS(16, 64);
+			}
+			else if (lds_wide(np, cnt, LDS_NB))
+			{
+				if (ps == 4)
+					hipLaunchKernelGGL((k_potrf_lds_q<T, 4, 2 * NTL, MODE == 1>), dim3(grid), dim3(2 * NTL), shm, st, args, np, cnt, lst, hd);
+				else if (ps == 8)
+					hipLaunchKernelGGL((k_potrf_lds_q<T, 8, 2 * NTL, MODE == 1>), dim3(grid), dim3(2 * NTL), shm, st, args, np, cnt, lst, hd);
+				else
+					hipLaunchKernelGGL((k_potrf_lds_q<T, 16, 2 * NTL, MODE == 1>), dim3(grid), dim3(2 * NTL), shm, st, args, np, cnt, lst, hd);
+			}
 			else
-				hipLaunchKernelGGL((k_potrf_lds<T, 16, 64, MODE == 1>), dim3(batch), dim3(64), shm2, st, args, np);
-		}
-		else
-		{
-			if (ps == 4)
-				hipLaunchKernelGGL((k_potrf_lds<T, 4, NTL, MODE == 1>), dim3(batch), dim3(NTL), shm2, st, args, np);
-			else if (ps == 8)
-				hipLaunchKernelGGL((k_potrf_lds<T, 8, NTL, MODE == 1>), dim3(batch), dim3(NTL), shm2, st, args, np);
-			else
-				hipLaunchKernelGGL((k_potrf_lds<T, 16, NTL, MODE == 1>), dim3(batch), dim3(NTL), shm2, st, args, np);
-		}
-		return check_launch(name);
+			{
+				if (ps == 4)
+					BFA_POTRF_LDS(4, NTL);
+				else if (ps == 8)
+					BFA_POTRF_LDS(8, NTL);
+				else
+					BFA_POTRF_LDS(16, NTL);
+			}
+#undef BFA_POTRF_LDS
+			return check_launch(name);
+		};
+		return launch_lds_buckets(args, batch, (mmax + 15) / 16 * 16, sizeof(T), st, name, go);
 	}
 	if (MODE == 1)
 		return launch_potrf_mn_split(ps, batch, (const blasfeo_amd_dfact_arg *)args, hint, mmax, st, name);
@@ -1067,23 +1221,40 @@ static int launch_syrk_potrf(int ps, int batch, const blasfeo_amd_dsyrk_dpotrf_a
 		hipLaunchKernelGGL(k_mn_finish, dim3(nb), dim3(256), 0, st, fact, batch);
 		return check_launch(name);
 	}
-	const int np = (mmax + 15) / 16 * 16;
-	const size_t shm = sizeof(double) * ((size_t)np * (np + 1) + np);
-	if (np <= 32)
-	{
-		if (ps == 4)
-			hipLaunchKernelGGL((k_syrk_potrf_lds<double, 4, 64>), dim3(batch), dim3(64), shm, st, args, np);
+	auto go = [&](int np, int grid, size_t shm, const int *cnt, const int *lst, int *hd) {
+#define BFA_SP_LDS(PS_, NT_)                                                                               \
+	do                                                                                                     \
+	{                                                                                                      \
+		if (cnt)                                                                                           \
+			hipLaunchKernelGGL((k_syrk_potrf_lds_q<double, PS_, NT_>), dim3(grid), dim3(NT_), shm, st, args, np, cnt, lst, hd); \
+		else                                                                                               \
+			hipLaunchKernelGGL((k_syrk_potrf_lds<double, PS_, NT_>), dim3(grid), dim3(NT_), shm, st, args, np); \
+	} while (0)
+		if (np <= 32)
+		{
+			if (ps == 4)
+				BFA_SP_LDS(4, 64);
+			else
+				BFA_SP_LDS(8, 64);
+		}
+		else if (lds_wide(np, cnt, 6))
+		{
+			if (ps == 4)
+				hipLaunchKernelGGL((k_syrk_potrf_lds_q<double, 4, 2 * NTL>), dim3(grid), dim3(2 * NTL), shm, st, args, np, cnt, lst, hd);
+			else
+				hipLaunchKernelGGL((k_syrk_potrf_lds_q<double, 8, 2 * NTL>), dim3(grid), dim3(2 * NTL), shm, st, args, np, cnt, lst, hd);
+		}
 		else
-			hipLaunchKernelGGL((k_syrk_potrf_lds<double, 8, 64>), dim3(batch), dim3(64), shm, st, args, np);
-	}
-	else
-	{
-		if (ps == 4)
-			hipLaunchKernelGGL((k_syrk_potrf_lds<double, 4, NTL>), dim3(batch), dim3(NTL), shm, st, args, np);
-		else
-			hipLaunchKernelGGL((k_syrk_potrf_lds<double, 8, NTL>), dim3(batch), dim3(NTL), shm, st, args, np);
-	}
-	return check_launch(name);
+		{
+			if (ps == 4)
+				BFA_SP_LDS(4, NTL);
+			else
+				BFA_SP_LDS(8, NTL);
+		}
+#undef BFA_SP_LDS
+		return check_launch(name);
+	};
+	return launch_lds_buckets(args, batch, (mmax + 15) / 16 * 16, sizeof(double), st, name, go);
 }
 
 } // namespace bfa

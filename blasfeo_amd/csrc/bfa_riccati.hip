@@ -549,31 +549,27 @@ __global__ void k_ric_zero(int *counts)
 		counts[threadIdx.x] = 0;
 }
 
-// One thread per instance: size class from the largest stage, append to the
-// class list.  Instances with a negative horizon or size are dropped (the
-// reference has no defined behaviour for them); those larger than mmax are
-// dropped from the general class, which cannot hold them.
 // The argument record of the sweep: trf's, or the fused sweep's (AUG).
 template <bool AUG> struct RicArg { typedef blasfeo_amd_riccati_arg type; };
 template <> struct RicArg<true> { typedef blasfeo_amd_riccati_sv_arg type; };
 
+// The plan: one thread per instance takes the size class of its largest
+// stage (ric_class, -1 for none) and the workgroup appends to the class
+// lists.  Instances with a negative horizon or size are dropped (the
+// reference has no defined behaviour for them); those larger than mmax are
+// dropped from the general class, which cannot hold them.
 // AUG: stage matrices have one more row; no global-memory sweep serves them,
 // so instances that fit no LDS class are dropped (the entry refuses size
 // bounds that allow them).  T = float: the classes of the FP32 sweep (half
 // the bytes per element); it has no global-memory form either.
 template <bool AUG = false, typename T = double>
-__global__ void k_ric_plan(const typename RicArg<AUG>::type *__restrict__ args, int batch, int mmax,
-	int *counts, int *lists)
+__device__ __forceinline__ int ric_class(const typename RicArg<AUG>::type &a, int mmax)
 {
 	constexpr int XR = AUG ? 1 : 0;
 	constexpr bool NOGEN = AUG || sizeof(T) != 8;
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= batch)
-		return;
-	const typename RicArg<AUG>::type &a = args[i];
 	const int N = a.N;
 	if (N < 0)
-		return;
+		return -1;
 	int nkmax = a.nx[N], nxmax = a.nx[N];
 	bool bad = a.nx[N] < 0;
 	for (int k = 0; k < N; k++)
@@ -584,7 +580,7 @@ __global__ void k_ric_plan(const typename RicArg<AUG>::type *__restrict__ args, 
 		nxmax = max(nxmax, x);
 	}
 	if (bad)
-		return;
+		return -1;
 	int cls = RIC_GENERAL;
 	if (nxmax <= 64 && nkmax + XR <= 128)
 	{
@@ -595,9 +591,16 @@ __global__ void k_ric_plan(const typename RicArg<AUG>::type *__restrict__ args, 
 				cls = c;
 	}
 	if (cls == RIC_GENERAL && (NOGEN || nkmax > mmax))
-		return;
-	const int slot = atomicAdd(&counts[cls], 1);
-	lists[(size_t)cls * batch + slot] = i;
+		return -1;
+	return cls;
+}
+
+template <bool AUG = false, typename T = double>
+__global__ void k_ric_plan(const typename RicArg<AUG>::type *__restrict__ args, int batch, int mmax,
+	int *counts, int *lists)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	plan_bucket<RIC_GENERAL + 1>(i, i < batch ? ric_class<AUG, T>(args[i], mmax) : -1, batch, counts, lists);
 }
 
 // Next instance of the class for this workgroup from the class's work-queue

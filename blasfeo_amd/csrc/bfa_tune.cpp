@@ -16,6 +16,7 @@ std::atomic<int> g_gemm_grid_y{0};
 std::atomic<int> g_potrf_bw{0};
 std::atomic<int> g_trsm_thin{0};
 std::atomic<int> g_gemm_wave{0};
+std::atomic<int> g_lds_threads[NCLASS];
 
 std::atomic<int> *slot(const char *name, int index)
 {
@@ -33,6 +34,8 @@ std::atomic<int> *slot(const char *name, int index)
 		return index == 0 ? &g_trsm_thin : nullptr;
 	if (!std::strcmp(name, "gemm_wave"))
 		return index == 0 ? &g_gemm_wave : nullptr;
+	if (!std::strcmp(name, "lds_threads"))
+		return (index >= 0 && index < NCLASS) ? &g_lds_threads[index] : nullptr;
 	return nullptr;
 }
 
@@ -49,6 +52,12 @@ bool valid(const char *name, int index, int value)
 	// gemm_wave: 1 = the one-wavefront-per-tile gemm when the sizes allow it, 2 = never
 	if (!std::strcmp(name, "trsm_thin") || !std::strcmp(name, "gemm_wave"))
 		return value == 1 || value == 2;
+	// lds_threads: threads per workgroup of the LDS-resident potrf_l(_mn) and
+	// dsyrk_dpotrf_ln launch of 16-row bucket index (rows 16 index + 1 ..
+	// 16 index + 16); the buckets up to 32 rows always run one wave.  0: 256
+	// for potrf_l(_mn), 512 for dsyrk_dpotrf_ln from 97 rows
+	if (!std::strcmp(name, "lds_threads"))
+		return index >= 2 && (value == 256 || value == 512);
 	// ric_threads: one wave only for the classes whose stages have nu+nx <= 64
 	return value == 128 || value == 256 || (value == 64 && index <= 4);
 }
@@ -70,6 +79,8 @@ int tune_value(TuneKnob knob, int index)
 		return g_trsm_thin.load();
 	case TUNE_GEMM_WAVE:
 		return g_gemm_wave.load();
+	case TUNE_LDS_THREADS:
+		return (index >= 0 && index < NCLASS) ? g_lds_threads[index].load() : 0;
 	}
 	return 0;
 }

@@ -145,7 +145,11 @@ const char *blasfeo_amd_last_error(void);
  *                  up to 256 on one wavefront per 32 x 32 output tile
  *                  (operands from global memory): 1 or 0 (built-in) when the
  *                  hint bounds m and n by 256, 2 never (the LDS-staged
- *                  64 x 64 engine).
+ *                  64 x 64 engine);
+ *   "lds_threads", index = 16-row bucket 2..7 (rows 16 index + 1 .. 16 index + 16)
+ *                  of the LDS-resident potrf_l(_mn) and dsyrk_dpotrf_ln launches
+ *                  (size bound 33..128): threads per workgroup, 256 or 512;
+ *                  built-in 256, and 512 for dsyrk_dpotrf_ln from 97 rows.
  * blasfeo_amd_tune_set returns BLASFEO_AMD_OK or BLASFEO_AMD_EINVAL;
  * blasfeo_amd_tune_get returns the value (0 = built-in) or BLASFEO_AMD_EINVAL. */
 int blasfeo_amd_tune_set(const char *name, int index, int value);

@@ -334,5 +334,12 @@ void *workspace(size_t bytes, hipStream_t)
 		ws.resize(bytes);
 	return ws.data();
 }
+void *plan_workspace(size_t bytes, hipStream_t)
+{
+	static std::vector<unsigned char> ws;
+	if (ws.size() < bytes)
+		ws.resize(bytes);
+	return ws.data();
+}
 }
 extern "C" const char *emu_last_error() { return bfa::g_err; }

@@ -181,7 +181,14 @@ def test_tuning_knobs_without_gpu():
     assert at.get_knob("trsm_thin", 0) == 2
     with pytest.raises(RuntimeError):
         at.set_knob("trsm_thin", 0, 3)         # 0 built-in, 1 always, 2 never
+    at.set_knob("lds_threads", 7, 512)
+    assert at.get_knob("lds_threads", 7) == 512
+    with pytest.raises(RuntimeError):
+        at.set_knob("lds_threads", 1, 256)      # up to 32 rows: one wave, no knob
+    with pytest.raises(RuntimeError):
+        at.set_knob("lds_threads", 4, 128)      # 256 or 512 threads
     at.reset()
+    assert at.get_knob("lds_threads", 7) == 0
     assert at.get_knob("potrf_bw", 0) == 0 and at.get_knob("trsm_thin", 0) == 0
     assert at.get_knob("ric_per_cu", 3) == 0 and at.get_knob("ric_threads", 2) == 0
     # footprints of the resident form: largest bench stage (nu+nx = 80, nx = 64)

@@ -51,6 +51,18 @@ def test_syrk_dpotrf_ln(emu_build, ps):
     run_case("case_chol.py", "syrk_potrf", ps)
 
 
+@pytest.mark.parametrize("case", ["potrf_lds", "potrf_lds_mn", "syrk_potrf_lds", "potrf_lds_mn_wide",
+                                  "syrk_potrf_lds_wide"])
+@pytest.mark.parametrize("ps", [4, 8])
+def test_lds_size_buckets(emu_build, case, ps):
+    """potrf_l, potrf_l_mn and dsyrk_dpotrf_ln with a bound of 128 rows: the
+    plan sorts the instances into 16-row buckets and each bucket's persistent
+    launch, its LDS sized for the bucket, takes them from an atomic queue;
+    sizes 1 .. 128 so that every bucket holds instances; _wide: eight-wave
+    workgroups for the buckets above 32 rows."""
+    run_case("case_chol.py", case, ps)
+
+
 @pytest.mark.parametrize("ps", [4, 8])
 def test_potrf_global_kernel(emu_build, ps):
     """k_potrf (potrf_l for n > 256; 16-column panels at full height, the

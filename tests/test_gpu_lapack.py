@@ -188,8 +188,9 @@ def test_potrf_large(gpu, ps):
     assert (gd["use_dA"] == rd["use_dA"]).all()
 
 
+@pytest.mark.parametrize("threads", [0, 512])
 @pytest.mark.parametrize("ps", [4, 8])
-def test_potrf_mn_lds(gpu, ps):
+def test_potrf_mn_lds(gpu, ps, threads):
     """dpotrf_l_mn with m <= 128: the m x n lower trapezoid factored in LDS
     (L11 = chol(C11), L21 = C21 L11^{-T}), offsets, n = m and n < m, in place."""
     rng = np.random.default_rng(720 + ps)
@@ -214,16 +215,26 @@ def test_potrf_mn_lds(gpu, ps):
     def ocall(h, r):
         f(ps, r["m"], r["n"], h.desc_addr(r["sC"]), r["ci"], r["cj"], h.desc_addr(r["sD"]), r["di"], r["dj"])
 
-    g, gd, rr, rd = run_both(b, buf, bfa.DFACT_ARG, recs, "dpotrf_l_mn", ocall, (128, 128, 0))
-    compare_buffers(b, g, rr, "dpotrf_l_mn ps=%d" % ps)
+    # threads: the lds_threads knob of the 16-row buckets above 32 rows (0 = built-in)
+    for bk in range(2, 8):
+        assert bfa.lib().blasfeo_amd_tune_set(b"lds_threads", bk, threads) == 0
+    try:
+        g, gd, rr, rd = run_both(b, buf, bfa.DFACT_ARG, recs, "dpotrf_l_mn", ocall, (128, 128, 0))
+    finally:
+        for bk in range(2, 8):
+            bfa.lib().blasfeo_amd_tune_set(b"lds_threads", bk, 0)
+    compare_buffers(b, g, rr, "dpotrf_l_mn ps=%d threads=%d" % (ps, threads))
     assert (gd["use_dA"] == rd["use_dA"]).all()
 
 
+@pytest.mark.parametrize("threads", [0, 256, 512])
 @pytest.mark.parametrize("ps", [4, 8])
-def test_syrk_dpotrf_ln_lds(gpu, ps):
+def test_syrk_dpotrf_ln_lds(gpu, ps, threads):
     """dsyrk_dpotrf_ln(_mn) with m <= 128: D = chol(C + A B^T) on the m x n lower
     trapezoid, A B^T added on the matrix cores from global memory; k = 0,
-    A != B, offsets on every operand."""
+    A != B, offsets on every operand.  The instances go to 16-row buckets,
+    each a persistent launch; threads: the lds_threads knob for the buckets
+    above 32 rows (0 = built-in: four waves, eight from 97 rows)."""
     rng = np.random.default_rng(740 + ps)
     b = Batch("d", ps, seed=17)
     recs = []
@@ -253,8 +264,14 @@ def test_syrk_dpotrf_ln_lds(gpu, ps):
           h.desc_addr(r["sB"]), r["bi"], r["bj"], h.desc_addr(r["sC"]), r["ci"], r["cj"],
           h.desc_addr(r["sD"]), r["di"], r["dj"])
 
-    g, gd, rr, rd = run_both(b, buf, bfa.DSYRK_DPOTRF_ARG, recs, "dsyrk_dpotrf_ln", ocall, (128, 128, 100))
-    compare_buffers(b, g, rr, "dsyrk_dpotrf_ln ps=%d" % ps)
+    for bk in range(2, 8):
+        assert bfa.lib().blasfeo_amd_tune_set(b"lds_threads", bk, threads) == 0
+    try:
+        g, gd, rr, rd = run_both(b, buf, bfa.DSYRK_DPOTRF_ARG, recs, "dsyrk_dpotrf_ln", ocall, (128, 128, 100))
+    finally:
+        for bk in range(2, 8):
+            bfa.lib().blasfeo_amd_tune_set(b"lds_threads", bk, 0)
+    compare_buffers(b, g, rr, "dsyrk_dpotrf_ln ps=%d threads=%d" % (ps, threads))
     assert (gd["use_dA"] == rd["use_dA"]).all()
 
 

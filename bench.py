@@ -1018,6 +1018,7 @@ def run_blas3_extra(args, torch, bfa):
         algo = float((8 * (mf * kf + kf * nf + 2 * mf * nf)).sum())
         unit = "ops/s (dgemm_nn calls)"
         kernel = "k_gemm_wave<double,%d,OP_NN>" % ps
+        traffic = read_traffic("k_gemm_wave", "gemm-nn-100000")
         cfg = {"workload": "dgemm_nn_batched", "batch": batch, "m,n,k": "U{4..128}"}
     elif wl == "trmm_rlnn":
         rec = _records(bfa.DTRSM_ARG, batch)
@@ -1029,6 +1030,7 @@ def run_blas3_extra(args, torch, bfa):
         algo = float((8 * (nf * (nf + 1) / 2 + 2 * mf * nf)).sum())
         unit = "ops/s (dtrmm_rlnn calls)"
         kernel = "k_trmm_rlnn<double,%d>" % ps
+        traffic = read_traffic("k_trmm_rlnn", "trmm-16384")
         cfg = {"workload": "dtrmm_rlnn_batched", "batch": batch, "m,n": "U{16..256}"}
     else:
         base = bufs["C"].data_ptr()
@@ -1043,7 +1045,9 @@ def run_blas3_extra(args, torch, bfa):
         flops = float((mf * (mf + 1) * kf + mf ** 3 / 3).sum())
         algo = float((8 * (mf * kf + mf * (mf + 1) + mf)).sum())
         unit = "factorizations/s (dsyrk_dpotrf_ln calls)"
-        kernel = "dsyrk_dpotrf_ln kernels"
+        kernel = ("k_syrk_potrf_lds_q<double,%d,64|256|512> (one persistent launch per 16-row "
+                  "bucket, LDS sized for the bucket)" % ps)
+        traffic = read_traffic("k_syrk_potrf_lds", "syrk-potrf-65536")
         cfg = {"workload": "dsyrk_dpotrf_ln_batched", "batch": batch, "m,k": "U{8..128}"}
     drec = to_device(rec)
     fn = lambda: bfa.call(call, ps, batch, drec.data_ptr(), hint, sh)
@@ -1094,7 +1098,7 @@ def run_blas3_extra(args, torch, bfa):
            "vs_baseline": None, "dtype": "f64", "data": "synthetic, seeded (%d)" % args.seed,
            "config": cfg,
            "roofline": {"bound": bound, "achieved": achieved, "peak": peak, "unit": punit,
-                        "frac": achieved / peak, "traffic": None, "algorithmic_bytes": algo,
+                        "frac": achieved / peak, "traffic": traffic, "algorithmic_bytes": algo,
                         "flops_per_launch": flops, "kernel": kernel},
            "cpu_baseline": None, "checks": checks}
     print(json.dumps(out), flush=True)

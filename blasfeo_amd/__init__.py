@@ -101,6 +101,14 @@ def lib():
     if not os.path.exists(_LIB):
         raise RuntimeError("blasfeo_amd: %s is missing; run blasfeo_amd/build.py "
                            "(or __graft_entry__.build()) first" % _LIB)
+    # The library binds to the HIP runtime (libamdhip64) already in the process.
+    # PyTorch ships its own: load it first, so that both use that one runtime
+    # (loaded the other way round, the library's first device allocation
+    # failed in a process that then imported torch).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(_LIB)
     vp, ip, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
     for name in _BATCHED:

@@ -49,11 +49,30 @@ __global__ __launch_bounds__(NTL) void k_potrf(const typename FactArg<T>::type *
 		a.sD->use_dA = (a.di == 0 && a.dj == 0) ? m : 0;
 }
 
+// The trapezoid's LDS form: square column-major with leading dimension np + 1
+// (the one-instance-per-workgroup kernels), or packed lower 16x17 tiles (the
+// bucketed launches: 78 KB instead of 133 KB at 128 rows, two instances per
+// CU instead of one).
+template <bool PACKED, typename T>
+__device__ __forceinline__ auto lds_acc(T *M, int ld)
+{
+	if constexpr (PACKED)
+		return TileLowerT<T>{M};
+	else
+		return ColMajor<T>{M, ld};
+}
+// LDS elements of the trapezoid plus the reciprocal diagonal for np rows
+__host__ __device__ __forceinline__ size_t lds_elems(int np, bool packed)
+{
+	const size_t t = (size_t)(np / 16);
+	return (packed ? t * (t + 1) / 2 * RT_SZ : (size_t)np * (np + 1)) + np;
+}
+
 // potrf_l (MN = false) and potrf_l_mn (MN = true) with the whole m x n lower
 // trapezoid in LDS (m <= 128): stage it from C, factor in place (chol_acc),
 // stream L and dA out.  NT = 64 (one wave) for m <= 32 so that several
 // instances share a CU.
-template <typename T, int PS, int NT, bool MN>
+template <typename T, int PS, int NT, bool MN, bool PACKED>
 __device__ __forceinline__ void potrf_lds_instance(const typename FactArg<T>::type &a, int np, T *M, T *invd)
 {
 	const int LDM = np + 1;
@@ -64,10 +83,17 @@ __device__ __forceinline__ void potrf_lds_instance(const typename FactArg<T>::ty
 	if (n > m)
 		n = m;
 	const int mt = (m + 15) / 16, nt = (n + 15) / 16;
-	g2s<T, PS, NT, false, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, nt * 16, m, n, M, LDM);
+	const auto Ml = lds_acc<PACKED>(M, LDM);
+	if constexpr (PACKED)
+		g2s_acc<T, PS, NT, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, nt * 16, m, n, Ml);
+	else
+		g2s<T, PS, NT, false, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, nt * 16, m, n, M, LDM);
 	// the one-wave form serves m <= 32: one panel row per lane, no second row set
-	chol_acc<T, NT, NT == WAVE ? 64 : 128>(ColMajor<T>{M, LDM}, m, n, invd);
-	s2g<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, n, M, LDM, 0);
+	chol_acc<T, NT, NT == WAVE ? 64 : 128>(Ml, m, n, invd);
+	if constexpr (PACKED)
+		s2g_acc<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, n, Ml, 0);
+	else
+		s2g<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, n, M, LDM, 0);
 	T *dD = a.sD->dA;
 	for (int j = threadIdx.x; j < n; j += NT)
 		dD[j] = invd[j];
@@ -80,7 +106,11 @@ __device__ __forceinline__ void potrf_lds_instance(const typename FactArg<T>::ty
 // persistent launch over one size bucket (the _q forms), each workgroup
 // taking the next instance of the bucket's list from its atomic queue head.
 // Two kernels rather than one with both modes: the loop costs registers that
-// the one-instance form (several instances per CU at 32 rows) cannot spare.  The leading barrier makes every
+// the one-instance form (several instances per CU at 32 rows) cannot spare.
+// The queued forms are compiled for three waves per SIMD (at most 168
+// registers; the syrk form then fits in 149 VGPRs without spills, against
+// 145 VGPRs + 40 AGPRs and two waves per SIMD unconstrained), so that three
+// four-wave workgroups share a CU where the LDS holds them (up to 96 rows).  The leading barrier makes every
 // thread finish the previous instance (and its read of the slot) before the
 // slot is overwritten and the LDS staged again.  Returns -1 once the bucket
 // is drained.
@@ -99,17 +129,17 @@ __global__ __launch_bounds__(NT) void k_potrf_lds(const typename FactArg<T>::typ
 {
 	extern __shared__ unsigned char smem_raw[];
 	T *M = reinterpret_cast<T *>(smem_raw);
-	potrf_lds_instance<T, PS, NT, MN>(args[blockIdx.x], np, M, M + (size_t)np * (np + 1));
+	potrf_lds_instance<T, PS, NT, MN, false>(args[blockIdx.x], np, M, M + lds_elems(np, false) - np);
 }
 template <typename T, int PS, int NT, bool MN>
-__global__ __launch_bounds__(NT) void k_potrf_lds_q(const typename FactArg<T>::type *__restrict__ args, int np,
+__global__ __launch_bounds__(NT, 3) void k_potrf_lds_q(const typename FactArg<T>::type *__restrict__ args, int np,
 	const int *__restrict__ count, const int *__restrict__ list, int *head)
 {
 	extern __shared__ unsigned char smem_raw[];
 	__shared__ int qslot;
 	T *M = reinterpret_cast<T *>(smem_raw);
 	for (int i; (i = queue_next(count, list, head, &qslot)) >= 0;)
-		potrf_lds_instance<T, PS, NT, MN>(args[i], np, M, M + (size_t)np * (np + 1));
+		potrf_lds_instance<T, PS, NT, MN, true>(args[i], np, M, M + lds_elems(np, true) - np);
 }
 
 // syrk_dpotrf_ln(_mn), m <= 128: D = chol(C + A B^T) on the m x n lower
@@ -117,7 +147,7 @@ __global__ __launch_bounds__(NT) void k_potrf_lds_q(const typename FactArg<T>::t
 // in LDS, A B^T is added tile by tile on the matrix cores with A and B read
 // from global memory (a group of up to four row tiles shares B's operand), and
 // the sum is factored in place (chol_acc).
-template <typename T, int PS, int NT>
+template <typename T, int PS, int NT, bool PACKED>
 __device__ __forceinline__ void syrk_potrf_lds_instance(const typename SPArg<T>::type &a, int np, T *M, T *invd)
 {
 	typedef typename Vec4<T>::type V4;
@@ -131,8 +161,11 @@ __device__ __forceinline__ void syrk_potrf_lds_instance(const typename SPArg<T>:
 		n = m;
 	const int lane = lane_id(), w = wave_id();
 	const int mt = (m + 15) / 16, nt = (n + 15) / 16;
-	const ColMajor<T> Ml{M, LDM};
-	g2s<T, PS, NT, false, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, nt * 16, m, n, M, LDM);
+	const auto Ml = lds_acc<PACKED>(M, LDM);
+	if constexpr (PACKED)
+		g2s_acc<T, PS, NT, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, nt * 16, m, n, Ml);
+	else
+		g2s<T, PS, NT, false, true>(view<T, PS>(a.sC), a.ci, a.cj, mt * 16, nt * 16, m, n, M, LDM);
 	__syncthreads();
 	if (k > 0)
 	{
@@ -166,10 +199,9 @@ __device__ __forceinline__ void syrk_potrf_lds_instance(const typename SPArg<T>:
 			}
 			// KU matrix-core steps per round, all their global loads issued
 			// before the first step: one memory latency per 4 KU columns of k
-			// rather than per 4 (4 KU = 8 at four waves keeps the registers of
-			// the four-wave form; the eight-wave form has twice the registers
-			// per lane to spare and takes 16)
-			constexpr int KU = NT > NTL ? 4 : 2;
+			// rather than per 4 (KU = 2 costs no registers at four waves; 3
+			// and 4 raise the kernel from 140 to 218 and 250 VGPRs)
+			constexpr int KU = 2;
 #pragma unroll 1
 			for (int kk = 0; kk < k; kk += 4 * KU)
 			{
@@ -205,7 +237,10 @@ __device__ __forceinline__ void syrk_potrf_lds_instance(const typename SPArg<T>:
 		}
 	}
 	chol_acc<T, NT, 128>(Ml, m, n, invd);
-	s2g<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, n, M, LDM, 0);
+	if constexpr (PACKED)
+		s2g_acc<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, n, Ml, 0);
+	else
+		s2g<T, PS, NT, true>(view<T, PS>(a.sD), a.di, a.dj, m, n, M, LDM, 0);
 	T *dD = a.sD->dA;
 	for (int j = threadIdx.x; j < n; j += NT)
 		dD[j] = invd[j];
@@ -218,17 +253,17 @@ __global__ __launch_bounds__(NT) void k_syrk_potrf_lds(const typename SPArg<T>::
 {
 	extern __shared__ unsigned char smem_raw[];
 	T *M = reinterpret_cast<T *>(smem_raw);
-	syrk_potrf_lds_instance<T, PS, NT>(args[blockIdx.x], np, M, M + (size_t)np * (np + 1));
+	syrk_potrf_lds_instance<T, PS, NT, false>(args[blockIdx.x], np, M, M + lds_elems(np, false) - np);
 }
 template <typename T, int PS, int NT>
-__global__ __launch_bounds__(NT) void k_syrk_potrf_lds_q(const typename SPArg<T>::type *__restrict__ args, int np,
+__global__ __launch_bounds__(NT, 3) void k_syrk_potrf_lds_q(const typename SPArg<T>::type *__restrict__ args, int np,
 	const int *__restrict__ count, const int *__restrict__ list, int *head)
 {
 	extern __shared__ unsigned char smem_raw[];
 	__shared__ int qslot;
 	T *M = reinterpret_cast<T *>(smem_raw);
 	for (int i; (i = queue_next(count, list, head, &qslot)) >= 0;)
-		syrk_potrf_lds_instance<T, PS, NT>(args[i], np, M, M + (size_t)np * (np + 1));
+		syrk_potrf_lds_instance<T, PS, NT, true>(args[i], np, M, M + lds_elems(np, true) - np);
 }
 
 // ---------------------------------------------------------------------------
@@ -982,14 +1017,14 @@ static int launch_potrf_mn_split(int ps, int batch, const blasfeo_amd_dfact_arg 
 }
 
 // The m <= 128 LDS-resident launches hold the whole trapezoid in LDS, sized
-// for np rows: np (np + 1) + np elements, 133 KB at np = 128, one workgroup
-// per CU.  With a bound above 32 rows a plan kernel sorts the instances into
-// buckets of 16 rows (instance i to bucket ceil(m/16) - 1) and each bucket
-// gets a persistent launch whose LDS is sized for that bucket, largest
+// for np rows.  With a bound above 32 rows a plan kernel sorts the instances
+// into buckets of 16 rows (instance i to bucket ceil(m/16) - 1) and each
+// bucket gets a persistent launch whose LDS is sized for that bucket, largest
 // first, so that an instance is charged its own footprint rather than the
-// bound's; up to 32 rows one workgroup per instance (the footprint is at
-// most 8.7 KB).  counts[0..LDS_NB-1] and the queue heads counts[16..] start
-// at 0.
+// bound's, and holds it as packed lower tiles (79 KB at 128 rows: two
+// workgroups per CU; the square form, 133 KB, allowed one).  Up to 32 rows,
+// one workgroup per instance with the square form (at most 8.7 KB).
+// counts[0..LDS_NB-1] and the queue heads counts[16..] start at 0.
 constexpr int LDS_NB = 8;
 template <typename R>
 __global__ void k_lds_plan(const R *__restrict__ args, int batch, int *counts, int *lists)
@@ -998,20 +1033,17 @@ __global__ void k_lds_plan(const R *__restrict__ args, int batch, int *counts, i
 	plan_bucket<LDS_NB>(i, i < batch ? min(LDS_NB - 1, max(0, (args[i].m - 1) / 16)) : -1, batch, counts, lists);
 }
 
-// Eight waves rather than four for a bucket's queued launch: the update tasks
-// of a panel step and of the syrk are shared by twice the waves.  Tuning knob
-// lds_threads[bucket] = 512 or 256; by default eight waves from bucket
-// wide_from on (dsyrk_dpotrf_ln: rows 97 and up, where the LDS holds one
-// workgroup per CU; 65536 instances of m, k ~ U{8..128} on MI355X: 10.17 ms,
-// against 10.72 ms at four waves throughout and 10.82 ms with eight waves
-// from 81 rows, whose 16-column syrk rounds leave one workgroup per CU where
-// the LDS holds two).
-static bool lds_wide(int np, const int *queued, int wide_from)
+// Eight waves rather than four for a bucket's queued launch (tuning knob
+// lds_threads[bucket] = 512): the update tasks of a panel step and of the
+// syrk are shared by twice the waves, but the registers of the eight-wave
+// form leave one workgroup per CU.  Four waves by default: with the packed
+// tiles two 128-row instances share a CU, and 65536 x dsyrk_dpotrf_ln of
+// m, k ~ U{8..128} take 8.38 ms on MI355X against 9.97 ms with eight waves
+// from 97 rows (with the square LDS form, one instance per CU from 97 rows,
+// eight waves there had won: 10.17 against 10.72 ms).
+static bool lds_wide(int np, const int *queued)
 {
-	if (!queued || np <= 32)
-		return false;
-	const int b = (np - 1) / 16, knob = tune_value(TUNE_LDS_THREADS, b);
-	return knob ? knob == 512 : b >= wide_from;
+	return queued && np > 32 && tune_value(TUNE_LDS_THREADS, (np - 1) / 16) == 512;
 }
 
 // go(np, grid, shm, count, list, head) launches the kernel form for np rows
@@ -1019,9 +1051,8 @@ static bool lds_wide(int np, const int *queued, int wide_from)
 template <typename R, typename Go>
 static int launch_lds_buckets(const R *args, int batch, int npad, size_t elem, hipStream_t st, const char *name, Go go)
 {
-	auto shm_for = [elem](int np) { return elem * ((size_t)np * (np + 1) + np); };
 	if (npad <= 32)
-		return go(npad, batch, shm_for(npad), (const int *)nullptr, (const int *)nullptr, (int *)nullptr);
+		return go(npad, batch, elem * lds_elems(npad, false), (const int *)nullptr, (const int *)nullptr, (int *)nullptr);
 	const size_t wsbytes = sizeof(int) * (64 + (size_t)LDS_NB * batch);
 	int *ws = (int *)plan_workspace(wsbytes, st);
 	if (!ws)
@@ -1038,7 +1069,7 @@ static int launch_lds_buckets(const R *args, int batch, int npad, size_t elem, h
 		if (16 * b >= npad)
 			continue;   // m beyond the bound: no instance runs (as in the one-launch form)
 		const int np = min(16 * (b + 1), npad);
-		const size_t shm = shm_for(np);
+		const size_t shm = elem * lds_elems(np, true);
 		// as many workgroups as the LDS holds, at most 16 one-wave or 8 four-wave ones per CU
 		const int per_cu = max(1, min(np <= 32 ? 16 : 8, (int)(163840 / shm)));
 		if ((rc = go(np, min(batch, per_cu * cus), shm, counts + b, lists + (size_t)b * batch, heads + b)))
@@ -1076,7 +1107,7 @@ static int launch_potrf(int ps, int batch, const typename FactArg<T>::type *args
 				else
 					BFA_POTRF_LDS(16, 64);
 			}
-			else if (lds_wide(np, cnt, LDS_NB))
+			else if (lds_wide(np, cnt))
 			{
 				if (ps == 4)
 					hipLaunchKernelGGL((k_potrf_lds_q<T, 4, 2 * NTL, MODE == 1>), dim3(grid), dim3(2 * NTL), shm, st, args, np, cnt, lst, hd);
@@ -1237,7 +1268,7 @@ static int launch_syrk_potrf(int ps, int batch, const blasfeo_amd_dsyrk_dpotrf_a
 			else
 				BFA_SP_LDS(8, 64);
 		}
-		else if (lds_wide(np, cnt, 6))
+		else if (lds_wide(np, cnt))
 		{
 			if (ps == 4)
 				hipLaunchKernelGGL((k_syrk_potrf_lds_q<double, 4, 2 * NTL>), dim3(grid), dim3(2 * NTL), shm, st, args, np, cnt, lst, hd);

@@ -36,27 +36,11 @@ static_assert(RIC_HEAD + RIC_NCLASS <= 64, "plan workspace header holds 64 ints"
 __constant__ int c_ric_cap[RIC_NCLASS] = {8192, 16384, 24576, 32768, 40960, 54272, 81920, 163840};
 static const int h_ric_cap[RIC_NCLASS] = {8192, 16384, 24576, 32768, 40960, 54272, 81920, 163840};
 
-// LDS storage of the resident form: 16x16 tiles with a leading dimension of 17
-// inside the tile (a row of a tile then spans distinct banks), the stage
-// matrix M as packed lower tiles (tile (I,J), I >= J, at I(I+1)/2 + J) and
-// W as a rectangle of tiles.  Against a square column-major M this keeps
-// only the lower tiles, so the largest stage (nu+nx = 80, nx = 64) needs
-// 76,800 B and two instances share a CU instead of one.
-constexpr int RT_LD = 17;
-constexpr int RT_SZ = 16 * RT_LD;
-
-__host__ __device__ __forceinline__ int ric_tix(int I, int J) { return I * (I + 1) / 2 + J; }
-
-// T: the sweep's arithmetic type (double; float for the mixed-precision form)
-template <typename T>
-struct TileLowerT
-{
-	T *p;
-	__device__ __forceinline__ T &operator()(int r, int c) const
-	{
-		return p[ric_tix(r >> 4, c >> 4) * RT_SZ + (c & 15) * RT_LD + (r & 15)];
-	}
-};
+// LDS storage of the resident form: the stage matrix M as packed lower tiles
+// (TileLowerT, bfa_chol_core.hpp) and W as a rectangle of the same 16x17
+// tiles.  Against a square column-major M this keeps only the lower tiles,
+// so the largest stage (nu+nx = 80, nx = 64) needs 76,800 B and two
+// instances share a CU instead of one.
 typedef TileLowerT<double> TileLower;
 
 template <typename T>

@@ -55,7 +55,6 @@ bool valid(const char *name, int index, int value)
 	// lds_threads: threads per workgroup of the LDS-resident potrf_l(_mn) and
 	// dsyrk_dpotrf_ln launch of 16-row bucket index (rows 16 index + 1 ..
 	// 16 index + 16); the buckets up to 32 rows always run one wave.  0: 256
-	// for potrf_l(_mn), 512 for dsyrk_dpotrf_ln from 97 rows
 	if (!std::strcmp(name, "lds_threads"))
 		return index >= 2 && (value == 256 || value == 512);
 	// ric_threads: one wave only for the classes whose stages have nu+nx <= 64

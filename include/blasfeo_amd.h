@@ -149,7 +149,7 @@ const char *blasfeo_amd_last_error(void);
  *   "lds_threads", index = 16-row bucket 2..7 (rows 16 index + 1 .. 16 index + 16)
  *                  of the LDS-resident potrf_l(_mn) and dsyrk_dpotrf_ln launches
  *                  (size bound 33..128): threads per workgroup, 256 or 512;
- *                  built-in 256, and 512 for dsyrk_dpotrf_ln from 97 rows.
+ *                  built-in 256.
  * blasfeo_amd_tune_set returns BLASFEO_AMD_OK or BLASFEO_AMD_EINVAL;
  * blasfeo_amd_tune_get returns the value (0 = built-in) or BLASFEO_AMD_EINVAL. */
 int blasfeo_amd_tune_set(const char *name, int index, int value);

@@ -234,7 +234,7 @@ def test_syrk_dpotrf_ln_lds(gpu, ps, threads):
     trapezoid, A B^T added on the matrix cores from global memory; k = 0,
     A != B, offsets on every operand.  The instances go to 16-row buckets,
     each a persistent launch; threads: the lds_threads knob for the buckets
-    above 32 rows (0 = built-in: four waves, eight from 97 rows)."""
+    above 32 rows (0 = built-in: four waves)."""
     rng = np.random.default_rng(740 + ps)
     b = Batch("d", ps, seed=17)
     recs = []

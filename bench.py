@@ -1045,8 +1045,8 @@ def run_blas3_extra(args, torch, bfa):
         flops = float((mf * (mf + 1) * kf + mf ** 3 / 3).sum())
         algo = float((8 * (mf * kf + mf * (mf + 1) + mf)).sum())
         unit = "factorizations/s (dsyrk_dpotrf_ln calls)"
-        kernel = ("k_syrk_potrf_lds_q<double,%d,64|256|512> (one persistent launch per 16-row "
-                  "bucket, LDS sized for the bucket)" % ps)
+        kernel = ("k_syrk_potrf_lds_q<double,%d,64|256> (one persistent launch per 16-row "
+                  "bucket, packed lower tiles in LDS sized for the bucket)" % ps)
         traffic = read_traffic("k_syrk_potrf_lds", "syrk-potrf-65536")
         cfg = {"workload": "dsyrk_dpotrf_ln_batched", "batch": batch, "m,k": "U{8..128}"}
     drec = to_device(rec)

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(NT3) __attribute__((amdgpu_waves_per_eu(3))) void k
 
 // D = alpha * B * A, A lower n x n (blasfeo_ref/x_blas3_ref.c:3218-3330)
 template <typename T, int PS>
-__global__ __launch_bounds__(NT3) void k_trmm_rlnn(const typename TrsmArg<T>::type *__restrict__ args, int batch, int slots)
+__global__ __launch_bounds__(NT3) __attribute__((amdgpu_waves_per_eu(3))) void k_trmm_rlnn(const typename TrsmArg<T>::type *__restrict__ args, int batch, int slots)
 {
 	__shared__ T As[KC * LDT];
 	__shared__ T Bs[KC * LDT];

@@ -150,7 +150,7 @@ def main():
                          "in a HIP graph (the default; every replay runs the whole call: plan, "
                          "queue reset and class launches)")
     ap.add_argument("--workload", default="riccati",
-                    choices=("riccati", "ric_trs", "ric_sv", "ric_mixed", "pack", "getrs", "trsm_rltn", "gemm_nn", "trmm_rlnn", "syrk_potrf", "potrf32", "gemm_syrk", "getrf", "sgetrf", "getrf_np",
+                    choices=("riccati", "ric_trs", "ric_sv", "ric_mixed", "pack", "getrs", "trsm_rltn", "gemm_nn", "trmm_rlnn", "syrk_potrf", "potrf_var", "potrf32", "gemm_syrk", "getrf", "sgetrf", "getrf_np",
                              "posv256"),
                     help="riccati (default, the headline); the others measure the remaining "
                          "batched configurations")
@@ -194,7 +194,7 @@ def main():
         return run_getrs(args, torch, bfa)
     if args.workload == "trsm_rltn":
         return run_trsm_rltn(args, torch, bfa)
-    if args.workload in ("gemm_nn", "trmm_rlnn", "syrk_potrf"):
+    if args.workload in ("gemm_nn", "trmm_rlnn", "syrk_potrf", "potrf_var"):
         return run_blas3_extra(args, torch, bfa)
     if args.workload != "riccati":
         return run_other(args, torch, bfa)
@@ -988,6 +988,13 @@ def run_blas3_extra(args, torch, bfa):
         n = rng.integers(16, 257, batch); m = rng.integers(16, 257, batch)
         shapes = {"A": (n, n), "B": (m, n), "D": (m, n)}
         fills = {"A": "uniform", "B": "normal", "D": None}
+    elif wl == "potrf_var":
+        # 65536 x D = chol(C), C = 0.01 N(0,1) + m I (lower triangle read), m ~ U{8..128}
+        batch = 65536
+        m = rng.integers(8, 129, batch)
+        k = np.zeros(batch, dtype=np.int64)
+        shapes = {"C": (m, m), "D": (m, m)}
+        fills = {"C": "normal", "D": None}
     else:
         # 65536 x D = chol(C + A A^T), A m x k N(0,1), C = (m + k) I, m, k ~ U{8..128}
         batch = 65536
@@ -1032,6 +1039,22 @@ def run_blas3_extra(args, torch, bfa):
         kernel = "k_trmm_rlnn<double,%d>" % ps
         traffic = read_traffic("k_trmm_rlnn", "trmm-16384")
         cfg = {"workload": "dtrmm_rlnn_batched", "batch": batch, "m,n": "U{16..256}"}
+    elif wl == "potrf_var":
+        base = bufs["C"].data_ptr()
+        bufs["C"].view(torch.float64).mul_(0.01)
+        di = torch.from_numpy(_diag_index(hds["C"], base, m, ps)).to("cuda")
+        bufs["C"].view(torch.float64)[di] += torch.from_numpy(np.repeat(m.astype(np.float64), m)).to("cuda")
+        rec = _records(bfa.DFACT_ARG, batch)
+        rec["m"], rec["n"] = m, m
+        rec["sC"], rec["sD"] = sp["C"], sp["D"]
+        call, hint = "dpotrf_l", (128, 128, 0)
+        flops = float((mf ** 3 / 3).sum())
+        algo = float((8 * (mf * (mf + 1) + mf)).sum())
+        unit = "factorizations/s (dpotrf_l calls)"
+        kernel = ("k_potrf_lds_q<double,%d,64|256,false> (one persistent launch per 16-row bucket, "
+                  "packed lower tiles in LDS sized for the bucket)" % ps)
+        traffic = read_traffic("k_potrf_lds", "potrf-var-65536")
+        cfg = {"workload": "dpotrf_l_batched", "batch": batch, "n": "U{8..128}"}
     else:
         base = bufs["C"].data_ptr()
         di = torch.from_numpy(_diag_index(hds["C"], base, m, ps)).to("cuda")
@@ -1078,13 +1101,15 @@ def run_blas3_extra(args, torch, bfa):
         elif wl == "trmm_rlnn":
             oracle.fn("d", "trmm_rlnn")(ps, mi, int(n[i]), 1.0, a["A"], 0, 0, a["B"], 0, 0,
                                         a["D"], 0, 0)
+        elif wl == "potrf_var":
+            oracle.fn("d", "potrf_l")(ps, mi, a["C"], 0, 0, a["D"], 0, 0)
         else:
             oracle.fn("d", "syrk_potrf_ln")(ps, mi, int(k[i]), a["A"], 0, 0, a["A"], 0, 0,
                                             a["C"], 0, 0, a["D"], 0, 0)
         r = lay.get(hb, names.index("D"))
         cn = int(hds["D"]["cn"][i])
         g = bfa.pm_unpack(_block(host["D"], hds["D"], bufs["D"].data_ptr(), i), ps, cn, r.shape[0], r.shape[1])
-        if wl == "syrk_potrf":
+        if wl in ("syrk_potrf", "potrf_var"):
             g, r = np.tril(g), np.tril(r)
         dd = np.abs(g - r)
         rel = np.where(dd > 1e-11, dd / np.maximum(np.abs(g) + np.abs(r), 1e-300), 0.0)

@@ -14,6 +14,11 @@
 //              (measured on gfx950 with a known-answer product).
 #pragma once
 
+// Address space of global memory for the panel-major views (the CPU emulator of
+// the kernels defines it empty).
+#ifndef BFA_GLOBAL
+#define BFA_GLOBAL __attribute__((address_space(1)))
+#endif
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "../../include/blasfeo_amd.h"
@@ -128,7 +133,10 @@ struct PM
 	{
 		return (size_t)(i & ~(PS - 1)) * (size_t)cn + (size_t)j * PS + (i & (PS - 1));
 	}
-	__device__ __forceinline__ T &at(int i, int j) const { return p[off(i, j)]; }
+	// through a global-memory pointer: global_load/store instead of flat ones,
+	// which also count against the LDS counter (an LDS wait then waits for the
+	// wave's global loads too)
+	__device__ __forceinline__ BFA_GLOBAL T &at(int i, int j) const { return ((BFA_GLOBAL T *)p)[off(i, j)]; }
 };
 
 template <typename T, int PS>
@@ -159,7 +167,7 @@ __device__ __forceinline__ void g2s_walk(const PM<T, PS> g, int r0, int c0, int 
 	const int rr = r0 & (PS - 1);
 	const int np = (rr + R + PS - 1) / PS;
 	const int run = PS * C;
-	const T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
+	const BFA_GLOBAL T *base = (const BFA_GLOBAL T *)g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
 	const int lane = threadIdx.x & (WAVE - 1);
 	int p = (int)(threadIdx.x / WAVE) - W0, e = lane;
 	if (p < 0)
@@ -226,11 +234,11 @@ __device__ __forceinline__ void s2g(PM<T, PS> g, int r0, int c0, int Rv, int Cv,
 	const int rr = r0 & (PS - 1);
 	const int np = (rr + Rv + PS - 1) / PS;
 	const int run = PS * Cv;
-	T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
+	BFA_GLOBAL T *base = (BFA_GLOBAL T *)g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
 	const int lane = threadIdx.x & (WAVE - 1);
 	for (int p = threadIdx.x / WAVE; p < np; p += NW)
 	{
-		T *dst = base + (size_t)p * PS * g.cn;
+		BFA_GLOBAL T *dst = base + (size_t)p * PS * g.cn;
 		// with LOWER, columns beyond the panel's last row are never written
 		int cend = run;
 		if (LOWER)
@@ -270,11 +278,11 @@ __device__ __forceinline__ void s2g_acc(PM<T, PS> g, int r0, int c0, int Rv, int
 	const int rr = r0 & (PS - 1);
 	const int np = (rr + Rv + PS - 1) / PS;
 	const int run = PS * Cv;
-	T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
+	BFA_GLOBAL T *base = (BFA_GLOBAL T *)g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
 	const int lane = threadIdx.x & (WAVE - 1);
 	for (int p = threadIdx.x / WAVE; p < np; p += NW)
 	{
-		T *dst = base + (size_t)p * PS * g.cn;
+		BFA_GLOBAL T *dst = base + (size_t)p * PS * g.cn;
 		int cend = run;
 		if (LOWER)
 		{

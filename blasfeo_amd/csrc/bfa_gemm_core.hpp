@@ -52,7 +52,7 @@ struct TileLoader
 		const int np = (rr + R + PS - 1) / PS;
 		const int run = PS * C;
 		const int total = np * run;
-		const T *base = g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
+		const BFA_GLOBAL T *base = (const BFA_GLOBAL T *)g.p + (size_t)(r0 - rr) * g.cn + (size_t)c0 * PS;
 #pragma unroll
 		for (int s = 0; s < MAXL; s++)
 		{

@@ -189,8 +189,8 @@ __device__ __forceinline__ void syrk_potrf_lds_instance(const typename SPArg<T>:
 			const int jr = bj * 16 + (lane & 15);     // row of B for this lane's operand
 			// this lane's rows of A (one per tile) and of B at column 0 of k; a
 			// panel-major row advances by PS elements per column
-			const T *rb = jr < n ? &gB.at(a.bi + jr, a.bj) : nullptr;
-			const T *ra[4];
+			const BFA_GLOBAL T *rb = jr < n ? &gB.at(a.bi + jr, a.bj) : nullptr;
+			const BFA_GLOBAL T *ra[4];
 #pragma unroll
 			for (int t = 0; t < 4; t++)
 			{
@@ -872,8 +872,8 @@ __global__ __launch_bounds__(NTL) __attribute__((amdgpu_waves_per_eu(4))) void k
 				const int r1 = c0 + j, r2 = pivs[j];
 				if (r1 != r2)
 				{
-					T &u = gD.at(di + r1, dj + c);
-					T &v = gD.at(di + r2, dj + c);
+					auto &u = gD.at(di + r1, dj + c);
+					auto &v = gD.at(di + r2, dj + c);
 					const T t = u;
 					u = v;
 					v = t;

@@ -160,6 +160,7 @@ inline void emu_launch(void (*k)(KA...), dim3 grid, dim3 block, size_t shm, A...
 }
 #define hipLaunchKernelGGL(k, grid, block, shm, stream, ...) emu_launch(k, dim3(grid), dim3(block), (size_t)(shm), __VA_ARGS__)
 
+#define BFA_GLOBAL   // host memory: no address-space qualifier
 #define __align__(n) __attribute__((aligned(n)))
 #define HIP_SYMBOL(x) (&(x))
 inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }

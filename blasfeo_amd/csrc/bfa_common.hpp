@@ -142,9 +142,11 @@ struct PM
 template <typename T, int PS>
 __device__ __forceinline__ PM<T, PS> view(const typename MatOf<T>::type *s)
 {
+	// the descriptor read as global memory as well (no flat loads)
+	const BFA_GLOBAL typename MatOf<T>::type *gs = (const BFA_GLOBAL typename MatOf<T>::type *)s;
 	PM<T, PS> v;
-	v.p = s->pA;
-	v.cn = s->cn;
+	v.p = gs->pA;
+	v.cn = gs->cn;
 	return v;
 }
 

@@ -436,9 +436,9 @@ __global__ __launch_bounds__(WAVE) void k_trsm_left_thin(const typename TrsmArg<
 	const int m = a.m;
 	if (m <= 0 || a.n <= 0 || m > THIN_Q * WAVE)
 		return;
-	const PM<T, PS> gA = view<T, PS>(a.sA);
-	const PM<T, PS> gB = view<T, PS>(a.sB);
-	PM<T, PS> gD = view<T, PS>(a.sD);
+	const PM<T, PS> gA = view_flat<T, PS>(a.sA);
+	const PM<T, PS> gB = view_flat<T, PS>(a.sB);
+	PM<T, PS> gD = view_flat<T, PS>(a.sD);
 	const int ai = a.ai, aj = a.aj;
 	const int lane = threadIdx.x;
 	const bool cached = !UNIT && ai == 0 && aj == 0 && a.sD != a.sA && a.sA->use_dA >= m;
